@@ -1,0 +1,78 @@
+"""ctypes binding of libdcx.so (the C ABI declared in include/dcx.h).
+
+There is no fallback: if the library is missing or a call fails, an exception
+is raised.  Arguments are passed as raw device pointers (``Tensor.data_ptr()``)
+and the current HIP stream, exactly like the reference's TF kernels hand
+tensor pointers to their CUDA launchers.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdcx.so")
+
+# Signature codes: p = pointer, i = int, z = size_t, s = stream, f = float
+# Every function returns dcx_status (int) unless listed in _SPECIAL.
+SIGNATURES: dict[str, str] = {
+    "dcx_dqs_fwd": "pppppiiiippppS",
+    "dcx_dqs_bwd": "pppppppiiiipppiS",
+    "dcx_dof_grad": "pppppiiiiipS",
+    "dcx_lbs_fwd": "ppppppiiiippppS",
+    "dcx_lbs_bwd": "ppppppiiiipppiS",
+}
+
+_CODES = {
+    "p": ctypes.c_void_p,
+    "i": ctypes.c_int,
+    "z": ctypes.c_size_t,
+    "S": ctypes.c_void_p,
+    "f": ctypes.c_float,
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class DcxError(RuntimeError):
+    """A dcx entry point returned a non-zero status."""
+
+    def __init__(self, name: str, status: int, message: str):
+        super().__init__(f"{name} failed with status {status}: {message}")
+        self.status = status
+
+
+def load() -> ctypes.CDLL:
+    """Loads libdcx.so once; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m deepcharacters_amd.build` (or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.dcx_version.restype = ctypes.c_int
+        lib.dcx_last_error.restype = ctypes.c_char_p
+        for name, sig in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [_CODES[c] for c in sig]
+        _lib = lib
+        return lib
+
+
+def call(name: str, *args) -> None:
+    """Calls a dcx entry point and raises DcxError on a non-zero status."""
+    lib = load()
+    status = getattr(lib, name)(*args)
+    if status != 0:
+        msg = lib.dcx_last_error().decode(errors="replace")
+        raise DcxError(name, status, msg)
+
+
+def exported_symbols() -> list[str]:
+    return list(SIGNATURES.keys()) + ["dcx_version", "dcx_last_error"]

@@ -1,0 +1,149 @@
+"""Skinning ops over the dcx C ABI (torch tensors on the HIP device).
+
+Mirrors the reference's DqSkinningGpu / DqSkinningGpuGrad and LbSkinningGpu /
+LbSkinningGpuGrad op pairs
+(CustomTensorFlowCPPOperators/src/CustomGPUOperators/DQSkinningOperator/*.cpp,
+LBSkinningOperator/*.cpp) from the point where forward kinematics has produced
+per-joint dual quaternions / transforms: output order and tensor layouts are
+the reference's.  The autograd Functions register the gradient the reference
+op pair implies (the reference has no Python RegisterGradient for these ops).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from ._torch import dev, opt, ptr, stream_ptr
+
+REF_COMPAT = 1
+
+
+def _sizes(dq_or_tr: torch.Tensor, joint_idx: torch.Tensor):
+    B, J = dq_or_tr.shape[0], dq_or_tr.shape[1]
+    N, K = joint_idx.shape
+    return B, N, K, J
+
+
+def dq_skinning_gpu(dual_quaternions: torch.Tensor, skinning_weights: torch.Tensor,
+                    joint_idx: torch.Tensor, rest_pos: torch.Tensor,
+                    rest_nrm: torch.Tensor | None = None):
+    """Forward of DqSkinningGpu given joint DQs.
+
+    Returns (skinned_vertex_positions [B,N,3], skinned_vertex_normals [B,N,3] or
+    None, dual_quaternions [B,J,8], output_skinning_weights [B,N,K]) — outputs
+    0, 1, 4 and 5 of the reference op (DQSkinningGPUOp.cpp:5-15).
+    """
+    ji = dev(joint_idx, "joint_idx", torch.int32)
+    B, N, K, J = _sizes(dual_quaternions, ji)
+    dq = dev(dual_quaternions, "dual_quaternions", shape=(B, J, 8))
+    w = dev(skinning_weights, "skinning_weights", shape=(B, N, K))
+    rp = dev(rest_pos, "rest_pos", shape=(N, 3))
+    rn = opt(rest_nrm, "rest_nrm", shape=(N, 3))
+    pos = torch.empty((B, N, 3), device=dq.device, dtype=torch.float32)
+    nrm = torch.empty_like(pos) if rn is not None else None
+    dq_out = torch.empty_like(dq)
+    w_out = torch.empty_like(w)
+    _lib.call("dcx_dqs_fwd", ptr(dq), ptr(w), ptr(ji), ptr(rp), ptr(rn), B, N, K, J, ptr(pos),
+              ptr(nrm), ptr(dq_out), ptr(w_out), stream_ptr())
+    return pos, nrm, dq_out, w_out
+
+
+def dq_skinning_backward(g_pos: torch.Tensor, dual_quaternions: torch.Tensor,
+                         skinning_weights: torch.Tensor, joint_idx: torch.Tensor,
+                         rest_pos: torch.Tensor, rest_nrm: torch.Tensor | None = None,
+                         g_nrm: torch.Tensor | None = None, ref_compat: bool = False,
+                         need_dq: bool = True, need_w: bool = True, need_rest: bool = False):
+    """Adjoint of dq_skinning_gpu: (g_dq [B,J,8], g_w [B,N,K], g_rest [B,N,3])."""
+    ji = dev(joint_idx, "joint_idx", torch.int32)
+    B, N, K, J = _sizes(dual_quaternions, ji)
+    dq = dev(dual_quaternions, "dual_quaternions", shape=(B, J, 8))
+    w = dev(skinning_weights, "skinning_weights", shape=(B, N, K))
+    rp = dev(rest_pos, "rest_pos", shape=(N, 3))
+    rn = opt(rest_nrm, "rest_nrm", shape=(N, 3))
+    gp = dev(g_pos, "g_pos", shape=(B, N, 3))
+    gn = opt(g_nrm, "g_nrm", shape=(B, N, 3))
+    g_dq = torch.empty_like(dq) if need_dq else None
+    g_w = torch.empty_like(w) if need_w else None
+    g_rest = torch.empty((B, N, 3), device=dq.device) if need_rest else None
+    _lib.call("dcx_dqs_bwd", ptr(gp), ptr(gn), ptr(dq), ptr(w), ptr(ji), ptr(rp), ptr(rn), B, N, K,
+              J, ptr(g_dq), ptr(g_w), ptr(g_rest), REF_COMPAT if ref_compat else 0, stream_ptr())
+    return g_dq, g_w, g_rest
+
+
+def dof_grad(g_pos: torch.Tensor, skinned_pos: torch.Tensor, joint_pos: torch.Tensor,
+             joint_axis: torch.Tensor, influence: torch.Tensor) -> torch.Tensor:
+    """dofs_grad output of DqSkinningGpuGrad / LbSkinningGpuGrad (rigid-joint approximation)."""
+    gp = dev(g_pos, "g_pos")
+    B, N = gp.shape[0], gp.shape[1]
+    sp = dev(skinned_pos, "skinned_pos", shape=(B, N, 3))
+    jp = dev(joint_pos, "joint_pos")
+    Jall = jp.shape[1]
+    ja = dev(joint_axis, "joint_axis", shape=(B, Jall, 3))
+    inf = dev(influence, "influence")
+    D, E = inf.shape[0], inf.shape[1]
+    out = torch.empty((B, D), device=gp.device)
+    _lib.call("dcx_dof_grad", ptr(gp), ptr(sp), ptr(jp), ptr(ja), ptr(inf), B, N, Jall, D, E,
+              ptr(out), stream_ptr())
+    return out
+
+
+def lb_skinning_gpu(transformation: torch.Tensor, skinning_weights: torch.Tensor,
+                    joint_idx: torch.Tensor, rest_pos: torch.Tensor, rest_nrm: torch.Tensor | None,
+                    displacement: torch.Tensor):
+    """Forward of LbSkinningGpu given joint transforms [B,J,12] (R row-major, t).
+
+    Returns (skinned_vertex_positions, skinned_vertex_normals, transformation,
+    output_skinning_weights) — outputs 0, 1, 4, 5 (LBSkinningGPUOp.cpp:5-16).
+    """
+    ji = dev(joint_idx, "joint_idx", torch.int32)
+    B, N, K, J = _sizes(transformation, ji)
+    tr = dev(transformation, "transformation", shape=(B, J, 12))
+    w = dev(skinning_weights, "skinning_weights", shape=(B, N, K))
+    rp = dev(rest_pos, "rest_pos", shape=(N, 3))
+    rn = opt(rest_nrm, "rest_nrm", shape=(N, 3))
+    dp = dev(displacement, "displacement", shape=(B, N, 3))
+    pos = torch.empty((B, N, 3), device=tr.device)
+    nrm = torch.empty_like(pos) if rn is not None else None
+    tr_out = torch.empty_like(tr)
+    w_out = torch.empty_like(w)
+    _lib.call("dcx_lbs_fwd", ptr(tr), ptr(w), ptr(ji), ptr(rp), ptr(rn), ptr(dp), B, N, K, J,
+              ptr(pos), ptr(nrm), ptr(tr_out), ptr(w_out), stream_ptr())
+    return pos, nrm, tr_out, w_out
+
+
+def lb_skinning_backward(g_pos, transformation, skinning_weights, joint_idx, rest_pos, displacement,
+                         ref_compat: bool = False):
+    """(g_w [B,N,K], g_disp [B,N,3], g_trans [B,J,12])."""
+    ji = dev(joint_idx, "joint_idx", torch.int32)
+    B, N, K, J = _sizes(transformation, ji)
+    tr = dev(transformation, "transformation", shape=(B, J, 12))
+    w = dev(skinning_weights, "skinning_weights", shape=(B, N, K))
+    rp = dev(rest_pos, "rest_pos", shape=(N, 3))
+    dp = dev(displacement, "displacement", shape=(B, N, 3))
+    gp = dev(g_pos, "g_pos", shape=(B, N, 3))
+    g_w = torch.empty_like(w)
+    g_d = torch.empty_like(dp)
+    g_t = torch.empty_like(tr)
+    _lib.call("dcx_lbs_bwd", ptr(gp), ptr(tr), ptr(w), ptr(ji), ptr(rp), ptr(dp), B, N, K, J,
+              ptr(g_w), ptr(g_d), ptr(g_t), REF_COMPAT if ref_compat else 0, stream_ptr())
+    return g_w, g_d, g_t
+
+
+class DQSkinFn(torch.autograd.Function):
+    """Differentiable DQ skinning: (dq [B,J,8], weights [B,N,K], rest_pos [N,3]) -> pos [B,N,3]."""
+
+    @staticmethod
+    def forward(ctx, dq, weights, rest_pos, joint_idx):
+        pos, _, _, _ = dq_skinning_gpu(dq, weights, joint_idx, rest_pos)
+        ctx.save_for_backward(dq, weights, rest_pos, joint_idx)
+        return pos
+
+    @staticmethod
+    def backward(ctx, g_pos):
+        dq, w, rp, ji = ctx.saved_tensors
+        g_dq, g_w, g_rest = dq_skinning_backward(
+            g_pos.contiguous(), dq, w, ji, rp, need_dq=ctx.needs_input_grad[0],
+            need_w=ctx.needs_input_grad[1], need_rest=ctx.needs_input_grad[2])
+        if g_rest is not None:
+            g_rest = g_rest.sum(0)
+        return g_dq, g_w, g_rest, None
