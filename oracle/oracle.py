@@ -127,3 +127,82 @@ def lbs_bwd(g_pos, trans, w, jidx, rest_pos, disp, ref_compat=False, dtype=np.fl
     _call(f"orc_lbs_bwd_{_suf(dtype)}", gp, tr, w, ji, rp, dp, B, N, K, J, gw, gd, gt,
           1 if ref_compat else 0)
     return gw, gd, gt
+
+
+# ----------------------------------------------------------------------------
+# Rasterizer
+# ----------------------------------------------------------------------------
+
+ALBEDO_MODES = {"vertexColor": 0, "textured": 1, "normal": 2, "lighting": 3,
+                "foregroundMask": 4, "position": 5, "depth": 6, "uv": 7}
+SHADING_MODES = {"shadeless": 0, "shaded": 1}
+RULES = {"native": 0, "reference": 1}
+
+
+def raster_fwd(faces, vpos, ext, intr, H, W, vcol=None, albedo="vertexColor", shading="shadeless",
+               rule="native", tex=None, sh=None, texcoords=None, dtype=np.float32):
+    """Returns face [B,C,H,W] int32, bary [B,C,H,W,2], render [B,C,H,W,3], vnormal [B,C,N,3]."""
+    faces = _c(faces, np.int32).reshape(-1, 3)
+    B, N = vpos.shape[0], vpos.shape[1]
+    C = ext.shape[1] // 12
+    F = faces.shape[0]
+    TH, TW = (tex.shape[1], tex.shape[2]) if tex is not None else (1, 1)
+    cfg = np.array([B, C, H, W, N, F, ALBEDO_MODES[albedo], SHADING_MODES[shading], RULES[rule], TH, TW],
+                   np.int32)
+    if vcol is None:
+        vcol = np.zeros((B, N, 3))
+    if tex is None:
+        tex = np.zeros((B, 1, 1, 3))
+    if sh is None:
+        sh = np.zeros((B, C, 27))
+    if texcoords is None:
+        texcoords = np.zeros((F, 6))
+    args = [_c(x, dtype) for x in (texcoords, vpos, vcol, tex, sh, ext, intr)]
+    face = np.zeros((B, C, H, W), np.int32)
+    bary = np.zeros((B, C, H, W, 2), dtype)
+    render = np.zeros((B, C, H, W, 3), dtype)
+    vnormal = np.zeros((B, C, N, 3), dtype)
+    _call(f"orc_raster_fwd_{_suf(dtype)}", cfg, faces, *args, face, bary, render, vnormal)
+    return face, bary, render, vnormal
+
+
+def raster_bwd(faces, g_render, vpos, ext, intr, face, bary, vnormal, vcol=None, albedo="vertexColor",
+               shading="shadeless", tex=None, sh=None, target=None, texcoords=None, image_filter=2,
+               texture_filter=2, dtype=np.float64):
+    """Returns (g_vpos [B,N,3], g_vcol [B,N,3], g_tex [B,TH,TW,3], g_sh [B,C,27])."""
+    faces = _c(faces, np.int32).reshape(-1, 3)
+    B, N = vpos.shape[0], vpos.shape[1]
+    C = ext.shape[1] // 12
+    H, W = face.shape[2], face.shape[3]
+    F = faces.shape[0]
+    TH, TW = (tex.shape[1], tex.shape[2]) if tex is not None else (1, 1)
+    cfg = np.array([B, C, H, W, N, F, ALBEDO_MODES[albedo], SHADING_MODES[shading], TH, TW,
+                    image_filter, texture_filter], np.int32)
+    if vcol is None:
+        vcol = np.zeros((B, N, 3))
+    if tex is None:
+        tex = np.zeros((B, 1, 1, 3))
+    if sh is None:
+        sh = np.zeros((B, C, 27))
+    if texcoords is None:
+        texcoords = np.zeros((F, 6))
+    tgt = _c(target, dtype) if target is not None else None
+    args = [_c(x, dtype) for x in (texcoords, g_render, vpos, vcol, tex, sh)]
+    gvp = np.zeros((B, N, 3), dtype)
+    gvc = np.zeros((B, N, 3), dtype)
+    gtex = np.zeros((B, TH, TW, 3), dtype)
+    gsh = np.zeros((B, C, 27), dtype)
+    _call(f"orc_raster_bwd_{_suf(dtype)}", cfg, faces, *args, tgt, _c(vnormal, dtype), _c(bary, dtype),
+          _c(face, np.int32), _c(ext, dtype), _c(intr, dtype), gvp, gvc, gtex, gsh)
+    return gvp, gvc, gtex, gsh
+
+
+def coverage_count(faces, vpos, ext, intr, H, W):
+    """[B,C,H,W] number of triangles covering each pixel centre (native rule)."""
+    faces = _c(faces, np.int32).reshape(-1, 3)
+    B, N = vpos.shape[0], vpos.shape[1]
+    C = ext.shape[1] // 12
+    out = np.zeros((B, C, H, W), np.int32)
+    _call("orc_coverage_count_f32", B, C, H, W, N, faces.shape[0], faces, _c(vpos, np.float32),
+          _c(ext, np.float32), _c(intr, np.float32), out)
+    return out
