@@ -22,7 +22,15 @@ SIGNATURES: dict[str, str] = {
     "dcx_dof_grad": "pppppiiiiipS",
     "dcx_lbs_fwd": "ppppppiiiippppS",
     "dcx_lbs_bwd": "ppppppiiiipppiS",
+    "dcx_topo_create": "piipp",
+    "dcx_topo_destroy": "p",
+    "dcx_render_fwd": "ppipppiippppppppzS",
+    "dcx_render_bwd": "ppippppiipppppppppppS",
 }
+
+# Functions whose return type is not dcx_status.
+_RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t}
+_EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]}
 
 _CODES = {
     "p": ctypes.c_void_p,
@@ -61,6 +69,10 @@ def load() -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [_CODES[c] for c in sig]
+        for name, rt in _RESTYPES.items():
+            fn = getattr(lib, name)
+            fn.restype = rt
+            fn.argtypes = _EXTRA_ARGTYPES[name]
         _lib = lib
         return lib
 
@@ -75,4 +87,4 @@ def call(name: str, *args) -> None:
 
 
 def exported_symbols() -> list[str]:
-    return list(SIGNATURES.keys()) + ["dcx_version", "dcx_last_error"]
+    return list(SIGNATURES.keys()) + list(_RESTYPES.keys()) + ["dcx_version", "dcx_last_error"]

@@ -134,6 +134,97 @@ dcx_status dcx_lbs_bwd(const float* g_pos, const float* transforms, const float*
                        int B, int N, int K, int J, float* g_w, float* g_disp, float* g_trans,
                        int flags, void* stream);
 
+
+/* ------------------------------------------------------------------------ */
+/* Rasterizer (CudaRendererGpu / CudaRendererGradGpu)                       */
+/* ------------------------------------------------------------------------ */
+
+/* Immutable mesh topology resident on the device: faces, optional texture
+ * coordinates and the vertex -> face adjacency (built once, O(N + F)).
+ * Replaces the constructor work of CUDABasedRasterization
+ * (CudaRenderer/cpp/src/Renderer/CUDABasedRasterization.cpp:11-183). */
+typedef struct dcx_topo dcx_topo;
+
+/* albedo_mode values (attribute albedo_mode, CudaRenderer.cpp:58) */
+#define DCX_ALBEDO_VERTEX_COLOR 0
+#define DCX_ALBEDO_TEXTURED 1
+#define DCX_ALBEDO_NORMAL 2
+#define DCX_ALBEDO_LIGHTING 3
+#define DCX_ALBEDO_FOREGROUND_MASK 4
+#define DCX_ALBEDO_POSITION 5
+#define DCX_ALBEDO_DEPTH 6
+#define DCX_ALBEDO_UV 7
+/* shading_mode values */
+#define DCX_SHADING_SHADELESS 0
+#define DCX_SHADING_SHADED 1
+/* coverage / visibility rule */
+#define DCX_RULE_NATIVE 0    /* watertight fixed-point edge functions, exact depth key */
+#define DCX_RULE_REFERENCE 1 /* the reference's ray test + int(t*1e4) depth (oracle only) */
+/* execution path */
+#define DCX_PATH_BINNED 0 /* all batch elements + views per launch, tile-binned LDS depth */
+#define DCX_PATH_PLAIN 1  /* reference launch structure: host loop, lane per (camera, face) */
+
+typedef struct {
+  int num_cameras;         /* C  (attribute number_of_cameras)            */
+  int width;               /* W  (attribute render_resolution_u)          */
+  int height;              /* H  (attribute render_resolution_v)          */
+  int albedo_mode;         /* DCX_ALBEDO_*                                 */
+  int shading_mode;        /* DCX_SHADING_*                                */
+  int image_filter_size;   /* target-image gradient half width (attr)     */
+  int texture_filter_size; /* texture gradient half width (attr)          */
+  int rule;                /* DCX_RULE_*                                   */
+  int path;                /* DCX_PATH_*                                   */
+} dcx_raster_cfg;
+
+/* faces: host int32 [F,3]; texcoords: host float [F,6] (u,v per corner) or NULL. */
+dcx_status dcx_topo_create(const int32_t* faces, int F, int N, const float* texcoords,
+                           dcx_topo** out);
+dcx_status dcx_topo_destroy(dcx_topo* topo);
+
+/* Bytes of device workspace dcx_render_fwd needs (0 on invalid arguments).
+ * A larger workspace raises the binned path's tile-list capacity. */
+size_t dcx_render_workspace_size(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B);
+
+/*
+ * Forward of CudaRendererGpu (CudaRenderer.cpp:5-31,316-345 ->
+ * CUDABasedRasterization.cu:553-576) for compute_normal_map = 'none'.
+ *   vertex_pos [B,N,3], vertex_color [B,N,3], texture [B,TH,TW,3] (may be NULL
+ *   unless textured), sh_coeff [B,C,27] (may be NULL unless shaded/lighting),
+ *   extrinsics [B,12C] (3x4 row-major [R|t] per camera), intrinsics [B,9C]
+ * Outputs: barycentric [B,C,H,W,2] (weights of corners 0 and 1), face
+ * [B,C,H,W] int32 (-1 = background), render [B,C,H,W,3], vertex_normal
+ * [B,C,N,3] (area-weighted, unnormalised; may be NULL).
+ */
+dcx_status dcx_render_fwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,
+                          const float* vertex_pos, const float* vertex_color, const float* texture,
+                          int tex_h, int tex_w, const float* sh_coeff, const float* extrinsics,
+                          const float* intrinsics, float* barycentric, int32_t* face,
+                          float* render, float* vertex_normal, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/*
+ * Backward of the rasterizer: CudaRendererGradGpu (CudaRendererGrad.cpp:6-34,
+ * 241-274 -> CUDABasedRasterizationGrad.cu:114-514) for albedo vertexColor,
+ * textured and foregroundMask, shadeless or shaded.  Per covered pixel the
+ * barycentric Jacobian is recomputed from the stored face id; contributions
+ * are the reference's: colour, texture (nearest texel) and SH gradients, the
+ * position gradient through the barycentrics (and, shaded, through the
+ * interpolated normal), plus the target-image "model-to-data" term when
+ * target_image is given.  The shaded-mode derivative of the vertex normals
+ * w.r.t. positions (adjacent-face loop) is not included.
+ *   render_grad [B,C,H,W,3]; vertex_normal, barycentric, face: forward outputs
+ *   outputs (overwritten): vertex_pos_grad [B,N,3], vertex_color_grad [B,N,3],
+ *   texture_grad [B,TH,TW,3] (may be NULL), sh_coeff_grad [B,C,27] (may be NULL)
+ */
+dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,
+                          const float* render_grad, const float* vertex_pos,
+                          const float* vertex_color, const float* texture, int tex_h, int tex_w,
+                          const float* sh_coeff, const float* target_image,
+                          const float* vertex_normal, const float* barycentric,
+                          const int32_t* face, const float* extrinsics, const float* intrinsics,
+                          float* vertex_pos_grad, float* vertex_color_grad, float* texture_grad,
+                          float* sh_coeff_grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,3 +55,21 @@ def test_invalid_arguments_return_error_status_without_gpu():
     with pytest.raises(_lib.DcxError):
         _lib.call("dcx_lbs_fwd", None, None, None, None, None, None, 1, 1, 9, 1, None, None, None,
                   None, None)
+
+
+def declared_arg_counts():
+    counts = {}
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"\b(dcx_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text):
+            args = m.group(2).strip()
+            counts[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return counts
+
+
+def test_python_signatures_match_header_arity():
+    counts = declared_arg_counts()
+    for name, sig in _lib.SIGNATURES.items():
+        assert len(sig) == counts[name], (name, len(sig), counts[name])
+    for name, args in _lib._EXTRA_ARGTYPES.items():
+        assert len(args) == counts[name], name
