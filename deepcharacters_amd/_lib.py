@@ -26,11 +26,16 @@ SIGNATURES: dict[str, str] = {
     "dcx_topo_destroy": "p",
     "dcx_render_fwd": "ppipppiippppppppzS",
     "dcx_render_bwd": "ppippppiipppppppppppS",
+    "dcx_edgraph_create": "piipppppippp",
+    "dcx_edgraph_destroy": "p",
+    "dcx_ed_fwd": "pippppppppppppzS",
+    "dcx_ed_bwd": "pippppppppppzS",
 }
 
 # Functions whose return type is not dcx_status.
-_RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t}
-_EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]}
+_RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_size": ctypes.c_size_t}
+_EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+                   "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int]}
 
 _CODES = {
     "p": ctypes.c_void_p,

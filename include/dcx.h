@@ -225,6 +225,58 @@ dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B
                           float* vertex_pos_grad, float* vertex_color_grad, float* texture_grad,
                           float* sh_coeff_grad, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Embedded-graph deformation (EmbeddedGraphGpu / EmbeddedGraphGpuGrad)      */
+/* ------------------------------------------------------------------------ */
+
+/* Device-resident embedded graph: rest mesh, node -> base-vertex map, the
+ * vertex -> node adjacency with weights (and its node -> vertex transpose,
+ * built here), optional markers, optional topology for normals.  Replaces
+ * the constructor work of EmbeddedGraphGPUOp (EmbeddedGraphGPUOp.cpp:33-198)
+ * and the CSR upload of EmbeddedGraph.cpp:379-460. */
+typedef struct dcx_edgraph dcx_edgraph;
+
+/* All pointers are HOST arrays: base_pos [N,3], node_vertex [M],
+ * v2n_offsets [N+1], v2n_nodes / v2n_weights [v2n_offsets[N]],
+ * base_markers [num_markers,3], marker_node [num_markers]. */
+dcx_status dcx_edgraph_create(const float* base_pos, int N, int M, const int32_t* node_vertex,
+                              const int32_t* v2n_offsets, const int32_t* v2n_nodes,
+                              const float* v2n_weights, int num_markers, const float* base_markers,
+                              const int32_t* marker_node, const dcx_topo* topo, dcx_edgraph** out);
+dcx_status dcx_edgraph_destroy(dcx_edgraph* graph);
+size_t dcx_ed_workspace_size(const dcx_edgraph* graph, int B);
+
+/*
+ * Forward of EmbeddedGraphGpu (EmbeddedGraphGPUOp.cpp:6-25,341-346 ->
+ * EmbeddedGraphGPUOp.cu:11-240):
+ *   v_nr = sum_n w_n (R(dA_n)(v - g_n) + g_n + dT_n) + disp_v
+ *   v'   = sum_s w_s (R(sA_s) v_nr + sT_s),   R(a) = Rz(a.z) Ry(a.y) Rx(a.x)
+ * Inputs [B,M,3]: delta_t, delta_a, skinned_t, skinned_a; displacements [B,N,3].
+ * Outputs: deformed_vertices [B,N,3]; deformed_normals [B,N,3] (normalised
+ * face-normal sums, needs a topology; may be NULL); deformed_markers
+ * [B,Mk,3] (may be NULL); deformed_graph [B,M,3] (may be NULL); delta_a_out
+ * and skinned_a_out [B,M,3] copies (op outputs 4, 5; may be NULL).
+ */
+dcx_status dcx_ed_fwd(const dcx_edgraph* graph, int B, const float* delta_t, const float* delta_a,
+                      const float* skinned_t, const float* skinned_a, const float* displacements,
+                      float* deformed_vertices, float* deformed_normals, float* deformed_markers,
+                      float* deformed_graph, float* delta_a_out, float* skinned_a_out,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Backward of EmbeddedGraphGpu: EmbeddedGraphGpuGrad (EmbeddedGraphGPUOpGrad.cpp:5-21
+ * -> EmbeddedGraphGPUOpGrad.cu:9-213).  Node translation / Euler-angle
+ * gradients including the marker term, displacement gradient; the skinned
+ * node gradients are zero as in the reference (may be NULL).
+ *   g_vertices [B,N,3], g_markers [B,Mk,3] (may be NULL), delta_a / skinned_a
+ *   [B,M,3]; outputs g_delta_t, g_delta_a, g_skinned_t, g_skinned_a [B,M,3],
+ *   g_displacements [B,N,3].
+ */
+dcx_status dcx_ed_bwd(const dcx_edgraph* graph, int B, const float* g_vertices, const float* g_markers,
+                      const float* delta_a, const float* skinned_a, float* g_delta_t, float* g_delta_a,
+                      float* g_skinned_t, float* g_skinned_a, float* g_displacements, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -206,3 +206,41 @@ def coverage_count(faces, vpos, ext, intr, H, W):
     _call("orc_coverage_count_f32", B, C, H, W, N, faces.shape[0], faces, _c(vpos, np.float32),
           _c(ext, np.float32), _c(intr, np.float32), out)
     return out
+
+
+# ----------------------------------------------------------------------------
+# Embedded graph
+# ----------------------------------------------------------------------------
+
+def ed_fwd(graph, base, dT, dA, sT, sA, disp, faces=None, base_markers=None, marker_node=None,
+           dtype=np.float64):
+    """(vertices [B,N,3], normals [B,N,3] | None, markers [B,Mk,3] | None, graph [B,M,3])."""
+    B, M = dT.shape[0], dT.shape[1]
+    N = base.shape[0]
+    Mk = 0 if base_markers is None else base_markers.shape[0]
+    dims = np.array([N, M, Mk, B], np.int32)
+    F = 0 if faces is None else np.asarray(faces).reshape(-1, 3).shape[0]
+    verts = np.zeros((B, N, 3), dtype)
+    nrms = np.zeros((B, N, 3), dtype) if faces is not None else None
+    mk = np.zeros((B, Mk, 3), dtype) if Mk else None
+    gr = np.zeros((B, M, 3), dtype)
+    _call(f"orc_ed_fwd_{_suf(dtype)}", dims, _c(graph.node_vertex, np.int32), _c(graph.v2n_offsets, np.int32),
+          _c(graph.v2n_nodes, np.int32), _c(graph.v2n_weights, dtype),
+          _c(marker_node, np.int32) if Mk else None, _c(base, dtype), _c(base_markers, dtype) if Mk else None,
+          _c(faces, np.int32) if faces is not None else None, F, _c(dT, dtype), _c(dA, dtype), _c(sT, dtype),
+          _c(sA, dtype), _c(disp, dtype), verts, nrms, mk, gr)
+    return verts, nrms, mk, gr
+
+
+def ed_bwd(graph, base, g_verts, dA, sA, g_markers=None, base_markers=None, node_marker=None, dtype=np.float64):
+    """(nodes_t_grad, nodes_r_grad, nodes_skinned_t_grad, nodes_skinned_r_grad, displacement_grad)."""
+    B, M = dA.shape[0], dA.shape[1]
+    N = base.shape[0]
+    Mk = 0 if base_markers is None else base_markers.shape[0]
+    dims = np.array([N, M, Mk, B], np.int32)
+    outs = [np.zeros((B, M, 3), dtype) for _ in range(4)] + [np.zeros((B, N, 3), dtype)]
+    _call(f"orc_ed_bwd_{_suf(dtype)}", dims, _c(graph.node_vertex, np.int32), _c(graph.v2n_offsets, np.int32),
+          _c(graph.v2n_nodes, np.int32), _c(graph.v2n_weights, dtype),
+          _c(node_marker, np.int32) if Mk else None, _c(base, dtype), _c(base_markers, dtype) if Mk else None,
+          _c(g_verts, dtype), _c(g_markers, dtype) if Mk else None, _c(dA, dtype), _c(sA, dtype), *outs)
+    return tuple(outs)

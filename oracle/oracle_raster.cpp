@@ -544,7 +544,7 @@ static void image_gradient_f(const T* img, T x, T y, int W, int H, int fs, T du[
   T norm = 0;
   for (int yy = -fs; yy <= fs; ++yy)
     for (int xx = -fs; xx <= fs; ++xx) {
-      const T* I = img + (size_t)(int)((y + T(yy)) * T(W) + (x + T(xx))) * 3;
+      const T* I = img + (size_t)(int)(((float)y + (float)yy) * (float)W + ((float)x + (float)xx)) * 3;
       T den = T(xx * xx + yy * yy);
       T gu = 0, gv = 0;
       if (den != T(0)) { gu = T(xx) / den; gv = T(yy) / den; }
@@ -652,11 +652,17 @@ void raster_bwd(const BwdCfg& cf, const int32_t* faces, const T* tc, const T* g_
           if (cf.albedo == ALB_VERTEX_COLOR) {
             for (int d = 0; d < 3; ++d) alb[d] = w[0] * cols[0][d] + w[1] * cols[1][d] + w[2] * cols[2][d];
           } else if (cf.albedo == ALB_TEXTURED) {
+            // The texel choices are discrete functions of the texture coordinate;
+            // they are made in float as in the reference so that FP64 evaluation
+            // samples the same texels.
             const T* t6 = tc + (size_t)f * 6;
-            U0 = (t6[0] * w[0] + t6[2] * w[1] + t6[4] * w[2]) * T(cf.TW);
-            V0 = ((T(1) - t6[1]) * w[0] + (T(1) - t6[3]) * w[1] + (T(1) - t6[5]) * w[2]) * T(cf.TH);
-            U0 = std::fmin(std::fmax(U0, T(0)), T(cf.TW - 1));
-            V0 = std::fmin(std::fmax(V0, T(0)), T(cf.TH - 1));
+            const float wf2 = 1.f - (float)bary[2 * o] - (float)bary[2 * o + 1];
+            float u0 = ((float)t6[0] * (float)bary[2 * o] + (float)t6[2] * (float)bary[2 * o + 1] + (float)t6[4] * wf2) * (float)cf.TW;
+            float v0 = ((1.f - (float)t6[1]) * (float)bary[2 * o] + (1.f - (float)t6[3]) * (float)bary[2 * o + 1] + (1.f - (float)t6[5]) * wf2) * (float)cf.TH;
+            u0 = std::fmin(std::fmax(u0, 0.f), (float)(cf.TW - 1));
+            v0 = std::fmin(std::fmax(v0, 0.f), (float)(cf.TH - 1));
+            U0 = T(u0);
+            V0 = T(v0);
             // swapped-weight "bilinear" albedo of CUDABasedRasterizationGrad.cu:216-243
             T LU = T((int)(U0 - T(0.5))) + T(0.5), HU = LU + T(1);
             T LV = T((int)(V0 - T(0.5))) + T(0.5), HV = LV + T(1);
