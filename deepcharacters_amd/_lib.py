@@ -26,16 +26,23 @@ SIGNATURES: dict[str, str] = {
     "dcx_topo_destroy": "p",
     "dcx_render_fwd": "ppipppiippppppppzS",
     "dcx_render_bwd": "ppippppiipppppppppppS",
-    "dcx_edgraph_create": "piipppppippp",
+    "dcx_edgraph_create": "piippppipppp",
     "dcx_edgraph_destroy": "p",
     "dcx_ed_fwd": "pippppppppppppzS",
     "dcx_ed_bwd": "pippppppppppzS",
+    "dcx_mvsil_fwd": "pppppiiiiiipppppzS",
+    "dcx_mvsil_bwd": "pppppiiiiipS",
+    "dcx_camproj_fwd": "ppppiiiipS",
+    "dcx_camproj_bwd": "ppppiiipS",
+    "dcx_mesh_boundary": "ppiiiiipS",
 }
 
 # Functions whose return type is not dcx_status.
-_RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_size": ctypes.c_size_t}
+_RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_size": ctypes.c_size_t,
+             "dcx_mvsil_workspace_size": ctypes.c_size_t}
 _EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
-                   "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int]}
+                   "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int],
+                   "dcx_mvsil_workspace_size": [ctypes.c_int] * 5}
 
 _CODES = {
     "p": ctypes.c_void_p,

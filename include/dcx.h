@@ -277,6 +277,60 @@ dcx_status dcx_ed_bwd(const dcx_edgraph* graph, int B, const float* g_vertices, 
                       float* g_skinned_t, float* g_skinned_a, float* g_displacements, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Silhouette loss and its feeders                                          */
+/* ------------------------------------------------------------------------ */
+
+/* data->model nearest boundary-point search */
+#define DCX_SIL_BRUTE 0  /* every point, as the reference */
+#define DCX_SIL_BINNED 1 /* uniform grid of 30*scale cells, identical results */
+
+/* Workspace bytes for dcx_mvsil_fwd with DCX_SIL_BINNED (0 on bad sizes). */
+size_t dcx_mvsil_workspace_size(int B, int C, int N, int U, int V);
+
+/*
+ * Forward of MultiViewSilhouetteLossGpu (MultiViewSilhouetteLossGPUOp.cpp:5-14,
+ * 204-219 -> MultiViewSilhouetteLossGPUOp.cu:11-251).  U = dt_image dim 2,
+ * V = dt_image dim 3, indexed as in the reference (row-major with width U).
+ *   points [B,C,N,2], normals [B,C,N,2] (image space), is_boundary uint8
+ *   [B,C,N], dt_image uint8 [B,C,U,V], crops [B,C,7] (border offset x,y,
+ *   scale, crop min x,y, crop max x,y)
+ * Outputs: residual [B,C,N], residual1 [B,C,U,V,2], dt_image_gradient
+ * [B,C,N,2], closest_vertex_id [B,C,U,V] (float, -1 = none).
+ * Points whose 3x3 DT window leaves the image are treated as out of view.
+ * workspace may be NULL for DCX_SIL_BRUTE.
+ */
+dcx_status dcx_mvsil_fwd(const float* points, const float* normals, const uint8_t* is_boundary,
+                         const uint8_t* dt_image, const float* crops, int B, int C, int N, int U, int V,
+                         int search, float* residual, float* residual1, float* dt_image_gradient,
+                         float* closest_vertex_id, void* workspace, size_t workspace_bytes,
+                         void* stream);
+
+/* MultiViewSilhouetteLossGpuGrad (MultiViewSilhouetteLossGPUOpGrad.cpp:5-11 ->
+ * MultiViewSilhouetteLossGPUOpGrad.cu:11-72): points_grad [B,C,N,2]. */
+dcx_status dcx_mvsil_bwd(const float* mv_dt_grad, const float* mv_dt_grad_1, const float* dt_image_grad,
+                         const float* closest_vertex_id, const float* crops, int B, int C, int N, int U,
+                         int V, float* points_grad, void* stream);
+
+/* CameraProjectionGpu (CameraProjectionGPUOp.cpp:5-12 -> CameraProjectionGPUOp.cu:12-102):
+ * points [B,N,3] (and vectors [B,N,3] when is_point == 0) -> [B,C,N,2]. */
+dcx_status dcx_camproj_fwd(const float* points, const float* vectors, const float* extrinsics,
+                           const float* intrinsics, int B, int C, int N, int is_point, float* out,
+                           void* stream);
+
+/* CameraProjectionGpuGrad, point mode (CameraProjectionGPUOpGrad.cu:12-79):
+ * [B,C,N,2] image-space gradient -> [B,N,3]. */
+dcx_status dcx_camproj_bwd(const float* points_image_grad, const float* points,
+                           const float* extrinsics, const float* intrinsics, int B, int C, int N,
+                           float* points_grad, void* stream);
+
+/* ProjectedMeshBoundaryGpu flags (Skeletool/Mesh/CUDABasedRasterization.cu:168-200)
+ * from a rendered face buffer [B,C,H,W] (face >= 0 = covered) and the
+ * projected points [B,C,N,2]: a point is on the boundary when its pixel's 3x3
+ * window lies inside the image and contains an uncovered pixel. */
+dcx_status dcx_mesh_boundary(const int32_t* face, const float* points_image, int B, int C, int N,
+                             int H, int W, uint8_t* is_boundary, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

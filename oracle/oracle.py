@@ -244,3 +244,58 @@ def ed_bwd(graph, base, g_verts, dA, sA, g_markers=None, base_markers=None, node
           _c(node_marker, np.int32) if Mk else None, _c(base, dtype), _c(base_markers, dtype) if Mk else None,
           _c(g_verts, dtype), _c(g_markers, dtype) if Mk else None, _c(dA, dtype), _c(sA, dtype), *outs)
     return tuple(outs)
+
+
+# ----------------------------------------------------------------------------
+# Multi-view silhouette loss, camera projection, mesh boundary
+# ----------------------------------------------------------------------------
+
+def mvsil_fwd(points, normals, is_boundary, dt, crops):
+    """(mv_dt_residual [B,C,N], mv_dt_residual_1 [B,C,U,V,2], dt_image_gradient [B,C,N,2],
+    closest_vertex_id [B,C,U,V]) with U = dt.shape[2], V = dt.shape[3]."""
+    B, C, N = points.shape[0], points.shape[1], points.shape[2]
+    U, V = dt.shape[2], dt.shape[3]
+    res = np.zeros((B, C, N), np.float32)
+    res1 = np.zeros((B, C, U, V, 2), np.float32)
+    dtg = np.zeros((B, C, N, 2), np.float32)
+    cid = np.zeros((B, C, U, V), np.float32)
+    _call("orc_mvsil_fwd", _c(points, np.float32), _c(normals, np.float32), _c(is_boundary, np.uint8),
+          _c(dt, np.uint8), _c(crops, np.float32), B, C, N, U, V, res, res1, dtg, cid)
+    return res, res1, dtg, cid
+
+
+def mvsil_bwd(mv_dt_grad, mv_dt_grad_1, dt_image_grad, closest_vertex_id, crops):
+    B, C, N = mv_dt_grad.shape
+    U, V = closest_vertex_id.shape[2], closest_vertex_id.shape[3]
+    out = np.zeros((B, C, N, 2), np.float64)
+    _call("orc_mvsil_bwd", _c(mv_dt_grad, np.float32), _c(mv_dt_grad_1, np.float32), _c(dt_image_grad, np.float32),
+          _c(closest_vertex_id, np.float32), _c(crops, np.float32), B, C, N, U, V, out)
+    return out
+
+
+def camproj(points, vectors, ext, intr, is_point, dtype=np.float64):
+    """[B,C,N,2] image-space points (is_point) or unit image-space directions."""
+    B, N = points.shape[0], points.shape[1]
+    C = ext.shape[1] // 12
+    out = np.zeros((B, C, N, 2), dtype)
+    _call(f"orc_camproj_{_suf(dtype)}", _c(points, dtype), _c(vectors if vectors is not None else points, dtype),
+          _c(ext, dtype), _c(intr, dtype), B, C, N, 1 if is_point else 0, out)
+    return out
+
+
+def camproj_bwd(g, points, ext, intr):
+    B, N = points.shape[0], points.shape[1]
+    C = ext.shape[1] // 12
+    out = np.zeros((B, N, 3), np.float64)
+    _call("orc_camproj_bwd_f64", _c(g, np.float64), _c(points, np.float64), _c(ext, np.float64),
+          _c(intr, np.float64), B, C, N, out)
+    return out
+
+
+def boundary_flags(proj_uv, covered):
+    """ProjectedMeshBoundaryGpu semantics on a coverage bitmap [B,C,H,W] -> bool [B,C,N]."""
+    B, C, N = proj_uv.shape[0], proj_uv.shape[1], proj_uv.shape[2]
+    H, W = covered.shape[2], covered.shape[3]
+    out = np.zeros((B, C, N), np.uint8)
+    _call("orc_boundary_flags", _c(proj_uv, np.float32), _c(covered, np.uint8), B, C, N, H, W, out)
+    return out.astype(bool)

@@ -57,19 +57,32 @@ def test_invalid_arguments_return_error_status_without_gpu():
                   None, None)
 
 
-def declared_arg_counts():
-    counts = {}
+def declared_params():
+    """{function: [kind per parameter]}, kind in p (pointer), i (int), z (size_t), f (float)."""
+    out = {}
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
         for m in re.finditer(r"\b(dcx_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text):
             args = m.group(2).strip()
-            counts[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
-    return counts
+            kinds = []
+            if args not in ("", "void"):
+                for a in args.split(","):
+                    if "*" in a:
+                        kinds.append("p")
+                    elif "size_t" in a:
+                        kinds.append("z")
+                    elif "float" in a:
+                        kinds.append("f")
+                    else:
+                        kinds.append("i")
+            out[m.group(1)] = kinds
+    return out
 
 
-def test_python_signatures_match_header_arity():
-    counts = declared_arg_counts()
+def test_python_signatures_match_header_parameter_kinds():
+    decl = declared_params()
     for name, sig in _lib.SIGNATURES.items():
-        assert len(sig) == counts[name], (name, len(sig), counts[name])
+        got = ["p" if c == "S" else c for c in sig]
+        assert got == decl[name], (name, "".join(got), "".join(decl[name]))
     for name, args in _lib._EXTRA_ARGTYPES.items():
-        assert len(args) == counts[name], name
+        assert len(args) == len(decl[name]), name
