@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Differentiable-character fwd+bwd throughput on one MI355X.
+
+Workload: 16 characters (vertices uniform in [8k, 60k], faces ~2x, 500-1500
+embedded-graph nodes, 18-joint dual-quaternion skinning with 4 joints per
+vertex), 4 views each at 1024x1024, 256x256 distance-transform images, seed 2.
+One step = embedded-graph deformation -> DQ skinning -> rasterisation (tri-id,
+barycentrics, vertex-colour buffer) -> L1 colour loss + multi-view silhouette
+loss -> gradients to joint DQs, node translations/rotations and vertex
+colours, for all 16 characters.
+
+* batched path: dcx_pipeline captured once into a hipGraph and replayed
+  (the number reported as `value`, in character-frames/s);
+* plain path: the reference's launch structure (a host loop over characters,
+  one op per launch, losses as framework ops), timed beside it;
+* roofline: the fused raster/colour-loss/raster-backward kernel ("fine"),
+  timed with HIP events recorded on its launch stream;
+* cpu_baseline: the CPU restatement of the same step (kind "port"), one
+  character per worker process, one thread per worker.
+
+`python bench.py --gpus N --steps K --warmup W`; only N = 1 is supported.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fwd+bwd frames/s (skin+ED+raster+silhouette), batched vs plain; HBM GB/s %"
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "fine_kernel_16chars_4x1024sq_traffic.txt")
+
+
+def fine_kernel_bytes(works) -> float:
+    """Algorithmic HBM bytes of one launch of the fused fine kernel: per view,
+    36 B per pixel (tri-id 4 + barycentrics 8 + colour 12 written, target
+    colour 12 read), 12 B per face (indices) and 16 B per vertex (projected
+    position) read; per character 24 B per vertex of gradient output
+    (position and colour)."""
+    b = 0.0
+    for wk in works:
+        C = wk.ext.shape[0]
+        b += C * (36.0 * wk.W * wk.H + 12.0 * wk.ch.F + 16.0 * wk.ch.N) + 24.0 * wk.ch.N
+    return b
+
+
+def step_bytes(works) -> float:
+    """Algorithmic bytes of a whole fwd+bwd character step:
+    41 B per view-pixel + 24 B per view x (vertex + face) + 212 B per vertex."""
+    b = 0.0
+    for wk in works:
+        C = wk.ext.shape[0]
+        b += 41.0 * C * wk.W * wk.H + 24.0 * C * (wk.ch.N + wk.ch.F) + 212.0 * wk.ch.N
+    return b
+
+
+def read_traffic():
+    """HBM bytes per fine-kernel launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (scaled by a copy-kernel calibration), if that summary exists."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
+
+
+# ----------------------------------------------------------------------------
+# CPU baseline (oracle port), one process per character
+# ----------------------------------------------------------------------------
+
+def _cpu_worker_init():
+    import oracle.pipeline_ref  # noqa: F401  (load the C library before timing)
+    from oracle import oracle as orc
+    orc.lib()
+
+
+def _cpu_step(args):
+    wk, w_col, w_sil = args
+    from oracle import pipeline_ref
+    t0 = time.perf_counter()
+    r = pipeline_ref.step(wk, w_col, w_sil)
+    return time.perf_counter() - t0, float(r["loss"][0] + r["loss"][1])
+
+
+def _cpu_ready(_):
+    return os.getpid()
+
+
+def cpu_baseline(works, w_col, w_sil, max_workers=16):
+    workers = max(1, min(max_workers, os.cpu_count() or 1, len(works)))
+    old = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(workers, initializer=_cpu_worker_init) as pool:
+            pool.map(_cpu_ready, range(workers))
+            t0 = time.perf_counter()
+            res = pool.map(_cpu_step, [(wk, w_col, w_sil) for wk in works], chunksize=1)
+            wall = time.perf_counter() - t0
+    finally:
+        if old is None:
+            os.environ.pop("OMP_NUM_THREADS", None)
+        else:
+            os.environ["OMP_NUM_THREADS"] = old
+    cpu_s = sum(r[0] for r in res)
+    return dict(value=len(works) / wall, unit="char-frames/s", cores=workers, kind="port",
+                sample=f"one full step of the same {len(works)} characters (fwd+bwd, 4 views at 1024x1024), "
+                       f"one character per process, 1 thread each; {cpu_s:.1f} s of CPU work, {wall:.1f} s wall")
+
+
+# ----------------------------------------------------------------------------
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--chars", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the character batch is split across GPUs by assigning whole "
+              "characters to ranks, with no exchange in the step, so per-GPU throughput is the figure; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from deepcharacters_amd import pipeline
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    works = pipeline.build_works(args.chars, seed=2)
+    log(f"built {len(works)} characters in {time.time() - t0:.1f} s "
+        f"(N={sum(w.ch.N for w in works)}, F={sum(w.ch.F for w in works)})")
+    pipeline.make_targets(works, pipeline.gpu_pose_renderer(), np.random.default_rng(2))
+    torch.cuda.synchronize()
+    log("targets rendered")
+
+    bp = pipeline.BatchedPipeline(works)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        bp.step()
+        stream.synchronize()
+        bp.capture(stream)
+        for _ in range(args.warmup):
+            bp.replay()
+        stream.synchronize()
+        loss = bp.loss.cpu()
+        if not torch.isfinite(loss).all():
+            raise RuntimeError(f"non-finite loss: {loss}")
+        log("batched warmup done")
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            bp.replay()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        ms_graph = ev0.elapsed_time(ev1) / args.steps
+        stages = bp.stage_times(args.profile_steps)
+    ms_per_step = wall * 1000.0 / args.steps
+    value = len(works) * args.steps / wall
+    log(f"batched: {ms_per_step:.3f} ms/step (events {ms_graph:.3f}), {value:.1f} char-frames/s")
+    log("stages (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+
+    plain = None
+    if not args.no_plain:
+        pp = pipeline.PlainPipeline(works)
+        pp.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            pp.step()
+        torch.cuda.synchronize()
+        pw = time.perf_counter() - t1
+        plain = dict(value=len(works) * args.plain_steps / pw, ms_per_step=pw * 1000.0 / args.plain_steps,
+                     steps=args.plain_steps)
+        log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['value']:.1f} char-frames/s")
+        del pp
+
+    fine_ms = stages["fine"]
+    fine_b = fine_kernel_bytes(works)
+    achieved = fine_b / (fine_ms * 1e-3) / 1e9
+    sb = step_bytes(works)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "char-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural capsule bodies, random poses, targets rendered from perturbed poses)",
+        "config": {"workload": "batched fwd+bwd, 16 characters x 4 views at 1024x1024 (N~U[8k,60k], "
+                               "M~U[500,1500], DQS top-4, DT 256x256), hipGraph replay",
+                   "characters": len(works), "views_per_character": int(works[0].ext.shape[0]),
+                   "resolution": [works[0].H, works[0].W], "vertices": int(sum(w.ch.N for w in works)),
+                   "faces": int(sum(w.ch.F for w in works)),
+                   "ed_nodes": int(sum(w.ch.graph.num_nodes for w in works))},
+        "plain": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "roofline": {"bound": "hbm", "kernel": "fine_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "alg_bytes_per_launch": fine_b, "avg_launch_ms": fine_ms},
+        "step_roofline": {"alg_bytes_per_step": sb, "achieved": sb / (ms_per_step * 1e-3) / 1e9,
+                          "frac": sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "stage_ms": stages,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(works, bp.w_color, bp.w_sil)
+        log(f"cpu: {out['cpu_baseline']['value']:.3f} char-frames/s on {out['cpu_baseline']['cores']} processes")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

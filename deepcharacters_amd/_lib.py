@@ -35,14 +35,23 @@ SIGNATURES: dict[str, str] = {
     "dcx_camproj_fwd": "ppppiiiipS",
     "dcx_camproj_bwd": "ppppiiipS",
     "dcx_mesh_boundary": "ppiiiiipS",
+    "dcx_pipeline_create": "iplp",
+    "dcx_pipeline_destroy": "p",
+    "dcx_pipeline_sizes": "pp",
+    "dcx_pipeline_step": "ppS",
+    "dcx_pipeline_capture": "ppS",
+    "dcx_pipeline_replay": "pS",
+    "dcx_pipeline_profile": "pppiS",
 }
 
 # Functions whose return type is not dcx_status.
 _RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_size": ctypes.c_size_t,
-             "dcx_mvsil_workspace_size": ctypes.c_size_t}
+             "dcx_mvsil_workspace_size": ctypes.c_size_t, "dcx_pipeline_num_stages": ctypes.c_int,
+             "dcx_pipeline_stage_name": ctypes.c_char_p}
 _EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
                    "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int],
-                   "dcx_mvsil_workspace_size": [ctypes.c_int] * 5}
+                   "dcx_mvsil_workspace_size": [ctypes.c_int] * 5, "dcx_pipeline_num_stages": [],
+                   "dcx_pipeline_stage_name": [ctypes.c_int]}
 
 _CODES = {
     "p": ctypes.c_void_p,
@@ -50,6 +59,7 @@ _CODES = {
     "z": ctypes.c_size_t,
     "S": ctypes.c_void_p,
     "f": ctypes.c_float,
+    "l": ctypes.c_longlong,
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,1072 @@
+// Batched differentiable-character step on gfx950: many characters with
+// uneven meshes, several views each, forward + backward in one launch
+// sequence that can be captured once into a hipGraph and replayed.
+//
+// Per step (all characters at once, indexed through per-character and
+// per-view descriptor tables):
+//   1 clear            zero accumulators and tile counters (fill kernel, no memset)
+//   2 node_fwd         per ED node: Euler delta rotation -> 15-float node row
+//   3 vertex_fwd       per vertex: ED inner sum (K_ed nodes) then DQ skinning
+//   4 normals          per vertex: normalised face-normal sum of the skinned mesh
+//   5 project          per (view, vertex): image position / depth and the
+//                      image-space normal direction (CameraProjection point +
+//                      vector modes)
+//   6 tri_setup_count  per (view, face): triangle record, tile counts
+//   7 scan, 8 scatter  tile lists
+//   9 fine             per 32x8 tile: LDS depth (64-bit ds_min on packed
+//                      (depth, face) keys), resolve, write face / barycentric /
+//                      colour, L1 colour loss vs the target image and, fused,
+//                      the rasterizer backward for that loss (barycentric
+//                      chain rule + target-image model-to-data term) with
+//                      wave-segmented atomics
+//  10 boundary         per (view, vertex): silhouette-boundary flags
+//  11 sil_point        model->data DT residual, squared loss, its gradient
+//  12 sil grid + sil_pixel   data->model residual via the binned nearest
+//                      boundary-point search, squared loss, gradient
+//  13 vertex_bwd       per vertex: camera-projection adjoint over the views,
+//                      DQ-skinning adjoint (joint-DQ gradients reduced in LDS),
+//                      gradient w.r.t. the ED-deformed position
+//  14 node_bwd         one wavefront per ED node: translation / Euler gradients
+#include <algorithm>
+#include <vector>
+
+#include "raster_internal.hpp"
+#include "skin_common.hpp"
+
+namespace dcx {
+
+dcx_status exclusive_scan(const unsigned int* in, unsigned int* out, unsigned int* aux, long long n,
+                          hipStream_t s);
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int KS = 4;   // skinning joints per vertex
+constexpr int KE = 4;   // ED nodes per vertex
+constexpr int kNodeRow = 15;  // Rd(9) g(3) td(3)
+constexpr int kImgFs = 2;     // image_filter_size of the model-to-data term
+
+struct CharD {
+  int N, F, M, J;
+  int v0, f0, m0, j0;  // global offsets
+  int view0, C;
+};
+
+struct ViewD {
+  int ch, W, H, S;
+  int tiles_x, tiles;
+  long long vn0, vf0, pix0, sil0, tile0, cell0;
+  int gx, gy;  // silhouette grid
+};
+
+struct Dev {
+  int B, V;
+  int maxN, maxF, maxM, maxS2;
+  long long totVN, totVF, totPix, totTiles, totCells;
+  const CharD* ch;
+  const ViewD* vw;
+  const long long* tile0;  // [V+1] for tile -> view lookup
+  // static mesh data (global arrays, local indices)
+  const float* rest;      // [sumN,3]
+  const int* faces;       // [sumF,3] local vertex ids
+  const int* jidx;        // [sumN,4] local joint ids
+  const float* jw;        // [sumN,4]
+  const int* enode;       // [sumN,4] local node ids
+  const float* ew;        // [sumN,4]
+  const int* node_vtx;    // [sumM] local vertex id
+  const int* n2v_off;     // [sumM+1] global offsets into n2v arrays
+  const int* n2v_vtx;     // local vertex ids
+  const float* n2v_w;
+  const int* vf_off;      // [sumN+1] global offsets
+  const int* vf_list;     // local face ids
+  // scratch
+  float* node_tab;        // [sumM,15]
+  float* vdef;            // [sumN,3] skinned vertices
+  float* vnrm;            // [sumN,3]
+  f4* proj;               // [totVN]
+  f2* nrm2;               // [totVN]
+  TriRec* tri;            // [totVF]
+  unsigned int* tile_cnt; // [totTiles]
+  unsigned int* tile_off; // [totTiles+1]
+  unsigned int* tile_fill;
+  unsigned int* scan_aux;
+  unsigned int* bins;
+  unsigned int capacity;
+  unsigned char* isb;     // [totVN]
+  unsigned int* cell_cnt; // [totCells]
+  unsigned int* cell_off; // [totCells+1]
+  unsigned int* cell_fill;
+  unsigned int* cell_items;  // [totVN]
+  unsigned int* scan_aux2;
+  float* g_v;             // [sumN,3] position gradient accumulator
+  float* g_pts;           // [totVN,2]
+  float* u;               // [sumN,3] gradient w.r.t. the ED-deformed position
+};
+
+struct IO {
+  const float* dq;        // [sumJ,8]
+  const float* dT;        // [sumM,3]
+  const float* dA;        // [sumM,3]
+  const float* col;       // [sumN,3]
+  const float* ext;       // [V,12]
+  const float* intr;      // [V,9]
+  const float* target;    // [totPix,3]
+  const unsigned char* dt;  // [sum S^2]
+  const float* crops;     // [V,7]
+  float w_col, w_sil;
+  int* face;
+  float* bary;
+  float* render;
+  float* vout;            // [sumN,3] (may be null)
+  float* loss;            // [B,2] colour, silhouette
+  float* g_dq;
+  float* g_dT;
+  float* g_dA;
+  float* g_col;
+};
+
+struct ClearList {
+  unsigned int* p[10];
+  long long n[10];
+  int count;
+};
+
+__global__ __launch_bounds__(kB) void clear_kernel(ClearList L) {
+  const long long stride = (long long)gridDim.x * kB;
+  for (int k = 0; k < L.count; ++k)
+    for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < L.n[k]; i += stride) L.p[k][i] = 0u;
+}
+
+__global__ __launch_bounds__(kB) void node_fwd_kernel(Dev d, IO io) {
+  const CharD c = d.ch[blockIdx.y];
+  const int n = blockIdx.x * kB + threadIdx.x;
+  if (n >= c.M) return;
+  const int gi = c.m0 + n;
+  const m3 R = euler_mat(euler_sc(ld3(io.dA + 3 * (size_t)gi)));
+  const f3 g = ld3(d.rest + 3 * ((size_t)c.v0 + d.node_vtx[gi]));
+  const f3 t = ld3(io.dT + 3 * (size_t)gi);
+  float* row = d.node_tab + (size_t)gi * kNodeRow;
+  for (int k = 0; k < 9; ++k) row[k] = R.m[k];
+  row[9] = g.x; row[10] = g.y; row[11] = g.z;
+  row[12] = t.x; row[13] = t.y; row[14] = t.z;
+}
+
+__device__ __forceinline__ f3 ed_inner(const Dev& d, const CharD& c, int gv) {
+  const f3 p = ld3(d.rest + 3 * (size_t)gv);
+  f3 v = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const float* r = d.node_tab + ((size_t)c.m0 + d.enode[(size_t)gv * KE + k]) * kNodeRow;
+    const f3 g = ld3(r + 9);
+    m3 R;
+    for (int q = 0; q < 9; ++q) R.m[q] = r[q];
+    v += d.ew[(size_t)gv * KE + k] * (mul(R, p - g) + g + ld3(r + 12));
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kB) void vertex_fwd_kernel(Dev d, IO io) {
+  __shared__ float s_tab[256 * 8];
+  const CharD c = d.ch[blockIdx.y];
+  for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
+  __syncthreads();
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N) return;
+  const int gv = c.v0 + v;
+  const f3 vnr = ed_inner(d, c, gv);
+  int idx[KS];
+  float w[KS];
+  for (int k = 0; k < KS; ++k) { idx[k] = d.jidx[(size_t)gv * KS + k]; w[k] = d.jw[(size_t)gv * KS + k]; }
+  const DqBlend bl = dq_blend(s_tab, idx, w, KS);
+  const f3 o = mul(quat_to_mat(bl.r), vnr) + dq_translation(bl.r, bl.d);
+  d.vdef[3 * (size_t)gv] = o.x; d.vdef[3 * (size_t)gv + 1] = o.y; d.vdef[3 * (size_t)gv + 2] = o.z;
+  if (io.vout) { io.vout[3 * (size_t)gv] = o.x; io.vout[3 * (size_t)gv + 1] = o.y; io.vout[3 * (size_t)gv + 2] = o.z; }
+}
+
+__global__ __launch_bounds__(kB) void normals_kernel(Dev d) {
+  const CharD c = d.ch[blockIdx.y];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N) return;
+  const int gv = c.v0 + v;
+  const float* vb = d.vdef + 3 * (size_t)c.v0;
+  f3 s = mk3(0.f, 0.f, 0.f);
+  for (int k = d.vf_off[gv]; k < d.vf_off[gv + 1]; ++k) {
+    const int* fv = d.faces + 3 * ((size_t)c.f0 + d.vf_list[k]);
+    const f3 a = ld3(vb + 3 * fv[0]), b1 = ld3(vb + 3 * fv[1]), b2 = ld3(vb + 3 * fv[2]);
+    s += cross(b1 - a, b2 - a);
+  }
+  const float l = sqrtf(dot(s, s));
+  const f3 n = l > 0.f ? mk3(s.x / l, s.y / l, s.z / l) : mk3(0.f, 0.f, 0.f);
+  d.vnrm[3 * (size_t)gv] = n.x; d.vnrm[3 * (size_t)gv + 1] = n.y; d.vnrm[3 * (size_t)gv + 2] = n.z;
+}
+
+__device__ __forceinline__ f2 proj2(const float* K, f3 x) {
+  const float h0 = K[0] * x.x + K[1] * x.y + K[2] * x.z;
+  const float h1 = K[3] * x.x + K[4] * x.y + K[5] * x.z;
+  float h2 = K[6] * x.x + K[7] * x.y + K[8] * x.z;
+  if (!(h2 > 0.0000001f)) h2 = 0.00001f;
+  return mk2(h0 / h2, h1 / h2);
+}
+
+__global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N) return;
+  const float* E = io.ext + (size_t)w * 12;
+  const float* K = io.intr + (size_t)w * 9;
+  const int gv = c.v0 + v;
+  const f3 X = ld3(d.vdef + 3 * (size_t)gv);
+  d.proj[vw.vn0 + v] = project_vertex(E, K, d.vdef + 3 * (size_t)gv);
+  // image-space normal direction (CameraProjectionGPUOp.cu:45-86)
+  const f3 n = ld3(d.vnrm + 3 * (size_t)gv);
+  const f3 xc = mk3(E[0] * X.x + E[1] * X.y + E[2] * X.z + E[3], E[4] * X.x + E[5] * X.y + E[6] * X.z + E[7],
+                    E[8] * X.x + E[9] * X.y + E[10] * X.z + E[11]);
+  const f3 x2 = mk3(xc.x + (E[0] * n.x + E[1] * n.y + E[2] * n.z), xc.y + (E[4] * n.x + E[5] * n.y + E[6] * n.z),
+                    xc.z + (E[8] * n.x + E[9] * n.y + E[10] * n.z));
+  const f2 a = proj2(K, xc), b = proj2(K, x2);
+  const float du = b.x - a.x, dv = b.y - a.y;
+  const float l = sqrtf(du * du + dv * dv);
+  d.nrm2[vw.vn0 + v] = l > 0.f ? mk2(du / l, dv / l) : mk2(0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int f = blockIdx.x * kB + threadIdx.x;
+  if (f >= c.F) return;
+  const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
+  const f4* pv = d.proj + vw.vn0;
+  TriRec t;
+  TriRec* dst = d.tri + vw.vf0 + f;
+  if (!setup_tri(pv[fv[0]], pv[fv[1]], pv[fv[2]], f, vw.W, vw.H, t)) {
+    t.bx0_by0 = 0x7fff7fff;
+    t.bx1_by1 = 0;
+    *dst = t;
+    return;
+  }
+  *dst = t;
+  const int x0 = t.bx0_by0 & 0xffff, y0 = t.bx0_by0 >> 16, x1 = t.bx1_by1 & 0xffff, y1 = t.bx1_by1 >> 16;
+  unsigned int* cnt = d.tile_cnt + vw.tile0;
+  for (int ty = y0 / kTileH; ty <= y1 / kTileH; ++ty)
+    for (int tx = x0 / kTileW; tx <= x1 / kTileW; ++tx) atomicAdd(cnt + ty * vw.tiles_x + tx, 1u);
+}
+
+__global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int f = blockIdx.x * kB + threadIdx.x;
+  if (f >= c.F) return;
+  const long long ri = vw.vf0 + f;
+  const TriRec& t = d.tri[ri];
+  if (t.bx0_by0 == 0x7fff7fff) return;
+  const int x0 = t.bx0_by0 & 0xffff, y0 = t.bx0_by0 >> 16, x1 = t.bx1_by1 & 0xffff, y1 = t.bx1_by1 >> 16;
+  for (int ty = y0 / kTileH; ty <= y1 / kTileH; ++ty)
+    for (int tx = x0 / kTileW; tx <= x1 / kTileW; ++tx) {
+      const long long tile = vw.tile0 + ty * vw.tiles_x + tx;
+      const unsigned int slot = d.tile_off[tile] + atomicAdd(d.tile_fill + tile, 1u);
+      if (slot < d.capacity) d.bins[slot] = (unsigned int)ri;
+    }
+}
+
+__device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, unsigned long long* s_key) {
+  const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
+  const int x1 = min(t.bx1_by1 & 0xffff, tx0 + kTileW - 1), y1 = min(t.bx1_by1 >> 16, ty0 + kTileH - 1);
+  if (x0 > x1 || y0 > y1) return;
+  EdgeWalk w;
+  w.init(t, x0, y0);
+  const int face = t.face_flags & kFaceMask;
+  long long row[3] = {w.e[0], w.e[1], w.e[2]};
+  for (int y = y0; y <= y1; ++y) {
+    long long e[3] = {row[0], row[1], row[2]};
+    for (int x = x0; x <= x1; ++x) {
+      if (w.inside(e))
+        atomicMin(s_key + (y - ty0) * kTileW + (x - tx0), make_key(interp_iz(e[0], e[1], e[2], t.iz), face));
+      e[0] += w.dx[0]; e[1] += w.dx[1]; e[2] += w.dx[2];
+    }
+    row[0] += w.dy[0]; row[1] += w.dy[1]; row[2] += w.dy[2];
+  }
+}
+
+// Barycentric VJP (same derivation as the standalone backward).
+__device__ void bary_vjp_p(const float* E, const float* K, const f3 X[3], float px, float py, const float gb_in[3],
+                           f3 out[3]) {
+  float h2[3], u[3], v[3], iz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float xc = E[0] * X[i].x + E[1] * X[i].y + E[2] * X[i].z + E[3];
+    const float yc = E[4] * X[i].x + E[5] * X[i].y + E[6] * X[i].z + E[7];
+    const float zc = E[8] * X[i].x + E[9] * X[i].y + E[10] * X[i].z + E[11];
+    const float h0 = K[0] * xc + K[1] * yc + K[2] * zc;
+    const float h1 = K[3] * xc + K[4] * yc + K[5] * zc;
+    h2[i] = K[6] * xc + K[7] * yc + K[8] * zc;
+    u[i] = h0 / h2[i];
+    v[i] = h1 / h2[i];
+    iz[i] = 1.f / h2[i];
+  }
+  float e[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int j = i == 2 ? 0 : i + 1, k = i == 0 ? 2 : i - 1;
+    e[i] = (u[k] - u[j]) * (py - v[j]) - (v[k] - v[j]) * (px - u[j]);
+  }
+  const float q0 = e[0] * iz[0], q1 = e[1] * iz[1], q2 = e[2] * iz[2];
+  const float Q = q0 + q1 + q2;
+  const float b0 = q0 / Q, b1 = q1 / Q;
+  const float g0 = gb_in[0] - gb_in[2], g1 = gb_in[1] - gb_in[2];
+  const float s = g0 * b0 + g1 * b1;
+  const float gq[3] = {(g0 - s) / Q, (g1 - s) / Q, (0.f - s) / Q};
+  float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f}, giz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    giz[i] = gq[i] * e[i];
+    const float ge = gq[i] * iz[i];
+    const int j = i == 2 ? 0 : i + 1, k = i == 0 ? 2 : i - 1;
+    gu[j] += ge * (v[k] - py);
+    gv[j] += ge * (px - u[k]);
+    gu[k] += ge * (py - v[j]);
+    gv[k] += ge * (u[j] - px);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float gh0 = gu[i] / h2[i], gh1 = gv[i] / h2[i];
+    const float gh2 = -(gu[i] * u[i] + gv[i] * v[i]) / h2[i] - giz[i] * iz[i] * iz[i];
+    const float gx = K[0] * gh0 + K[3] * gh1 + K[6] * gh2;
+    const float gy = K[1] * gh0 + K[4] * gh1 + K[7] * gh2;
+    const float gz = K[2] * gh0 + K[5] * gh1 + K[8] * gh2;
+    out[i] = mk3(E[0] * gx + E[4] * gy + E[8] * gz, E[1] * gx + E[5] * gy + E[9] * gz,
+                 E[2] * gx + E[6] * gy + E[10] * gz);
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ bool seg_reduce_p(float (&v)[NV], int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const int next = __shfl_down(key, 1, 64);
+  int start = (lane == 0 || prev != key) ? lane : 0;
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const int o = __shfl_up(start, dd, 64);
+    if (lane >= dd) start = max(start, o);
+  }
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const bool take = lane - dd >= start;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const float o = __shfl_up(v[j], dd, 64);
+      if (take) v[j] += o;
+    }
+  }
+  return lane == 63 || next != key;
+}
+
+__device__ __forceinline__ float block_sum(float v, float* s_red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float t = 0.f;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += s_red[i];
+  return t;
+}
+
+__global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
+  __shared__ unsigned long long s_key[kTilePix];
+  __shared__ int s_view;
+  __shared__ float s_red[kTilePix / 64];
+  const long long tile = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = d.V;  // tile0[lo] <= tile < tile0[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (d.tile0[mid] <= tile) lo = mid; else hi = mid;
+    }
+    s_view = lo;
+  }
+  s_key[threadIdx.x] = kEmptyKey;
+  __syncthreads();
+  const int w = s_view;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int lt = (int)(tile - vw.tile0);
+  const int tx0 = (lt % vw.tiles_x) * kTileW, ty0 = (lt / vw.tiles_x) * kTileH;
+  const unsigned int off = d.tile_off[tile], end = d.tile_off[tile + 1];
+  if (end <= d.capacity) {
+    for (unsigned int k = off + threadIdx.x; k < end; k += kTilePix) raster_in_tile(d.tri[d.bins[k]], tx0, ty0, s_key);
+  } else {
+    const TriRec* tv = d.tri + vw.vf0;
+    for (int f = threadIdx.x; f < c.F; f += kTilePix) raster_in_tile(tv[f], tx0, ty0, s_key);
+  }
+  __syncthreads();
+  const int x = tx0 + (threadIdx.x % kTileW), y = ty0 + (threadIdx.x / kTileW);
+  const bool inimg = x < vw.W && y < vw.H;
+  const unsigned long long key = s_key[threadIdx.x];
+  const int f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
+  float acc[18];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) acc[k] = 0.f;
+  float lsum = 0.f;
+  int i0 = 0, i1 = 0, i2 = 0;
+  if (inimg) {
+    const long long o = vw.pix0 + (long long)y * vw.W + x;
+    const f3 tgt = ld3(io.target + 3 * o);
+    if (f < 0) {
+      io.face[o] = -1;
+      io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
+      io.render[3 * o] = 0.f; io.render[3 * o + 1] = 0.f; io.render[3 * o + 2] = 0.f;
+      lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
+    } else {
+      const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
+      i0 = fv[0]; i1 = fv[1]; i2 = fv[2];
+      const f4* pv = d.proj + vw.vn0;
+      const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+      const f2 ab = native_bary(pv[i0], pv[i1], pv[i2], px, py);
+      const float wgt[3] = {ab.x, ab.y, 1.f - ab.x - ab.y};
+      const float* cb = io.col + 3 * (size_t)c.v0;
+      const f3 c0 = ld3(cb + 3 * i0), c1 = ld3(cb + 3 * i1), c2 = ld3(cb + 3 * i2);
+      const f3 col = mk3(c0.x * wgt[0] + c1.x * wgt[1] + c2.x * wgt[2], c0.y * wgt[0] + c1.y * wgt[1] + c2.y * wgt[2],
+                         c0.z * wgt[0] + c1.z * wgt[1] + c2.z * wgt[2]);
+      io.face[o] = f;
+      io.bary[2 * o] = ab.x; io.bary[2 * o + 1] = ab.y;
+      io.render[3 * o] = col.x; io.render[3 * o + 1] = col.y; io.render[3 * o + 2] = col.z;
+      const f3 df = col - tgt;
+      lsum = io.w_col * (fabsf(df.x) + fabsf(df.y) + fabsf(df.z));
+      auto sgn = [](float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); };
+      const f3 g = mk3(io.w_col * sgn(df.x), io.w_col * sgn(df.y), io.w_col * sgn(df.z));
+      // colour gradient and dL/dweights (vertexColor, shadeless)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { acc[9 + 3 * k] = g.x * wgt[k]; acc[10 + 3 * k] = g.y * wgt[k]; acc[11 + 3 * k] = g.z * wgt[k]; }
+      const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
+      const float* E = io.ext + (size_t)w * 12;
+      const float* K = io.intr + (size_t)w * 9;
+      const float* vb = d.vdef + 3 * (size_t)c.v0;
+      const f3 X[3] = {ld3(vb + 3 * i0), ld3(vb + 3 * i1), ld3(vb + 3 * i2)};
+      f3 gx[3];
+      bary_vjp_p(E, K, X, px, py, gw, gx);
+      // model-to-data term (CUDABasedRasterizationGrad.cu:413-437)
+      const int fs = kImgFs;
+      if (x >= fs + 1 && y >= fs + 1 && x < vw.W - (fs + 1) && y < vw.H - (fs + 1)) {
+        f3 du = mk3(0.f, 0.f, 0.f), dvv = du;
+        float norm = 0.f;
+        for (int yy = -fs; yy <= fs; ++yy)
+          for (int xx = -fs; xx <= fs; ++xx) {
+            const f3 I = ld3(io.target + 3 * (vw.pix0 + (long long)(y + yy) * vw.W + (x + xx)));
+            const float den = (float)(xx * xx + yy * yy);
+            float gu = 0.f, gvv = 0.f;
+            if (den != 0.f) { gu = (float)xx / den; gvv = (float)yy / den; }
+            du = du + I * gu;
+            dvv = dvv + I * gvv;
+            norm += fabsf(gu);
+          }
+        du = mk3(du.x / norm, du.y / norm, du.z / norm);
+        dvv = mk3(dvv.x / norm, dvv.y / norm, dvv.z / norm);
+        const float gu = -dot(g, du), gvv = -dot(g, dvv);
+        const f3 fp = X[0] * wgt[0] + X[1] * wgt[1] + X[2] * wgt[2];
+        const float xc = E[0] * fp.x + E[1] * fp.y + E[2] * fp.z + E[3];
+        const float yc = E[4] * fp.x + E[5] * fp.y + E[6] * fp.z + E[7];
+        const float zc = E[8] * fp.x + E[9] * fp.y + E[10] * fp.z + E[11];
+        const float hx = K[0] * xc + K[1] * yc + K[2] * zc;
+        const float hy = K[3] * xc + K[4] * yc + K[5] * zc;
+        const float hz = K[6] * xc + K[7] * yc + K[8] * zc;
+        if (fabsf(hz) > 0.0001f) {
+          const float gh0 = gu / hz, gh1 = gvv / hz, gh2 = -(gu * hx + gvv * hy) / (hz * hz);
+          const float cx = K[0] * gh0 + K[3] * gh1 + K[6] * gh2;
+          const float cy = K[1] * gh0 + K[4] * gh1 + K[7] * gh2;
+          const float cz = K[2] * gh0 + K[5] * gh1 + K[8] * gh2;
+          const f3 gX = mk3(E[0] * cx + E[4] * cy + E[8] * cz, E[1] * cx + E[5] * cy + E[9] * cz,
+                            E[2] * cx + E[6] * cy + E[10] * cz);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) gx[k] = gx[k] + gX * wgt[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { acc[3 * k] = gx[k].x; acc[3 * k + 1] = gx[k].y; acc[3 * k + 2] = gx[k].z; }
+    }
+  }
+  const bool tail = seg_reduce_p<18>(acc, f);
+  if (f >= 0 && tail) {
+    const int idx[3] = {i0, i1, i2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float* gp = d.g_v + 3 * ((size_t)c.v0 + idx[k]);
+      atomicAdd(gp, acc[3 * k]); atomicAdd(gp + 1, acc[3 * k + 1]); atomicAdd(gp + 2, acc[3 * k + 2]);
+      float* gc = io.g_col + 3 * ((size_t)c.v0 + idx[k]);
+      atomicAdd(gc, acc[9 + 3 * k]); atomicAdd(gc + 1, acc[10 + 3 * k]); atomicAdd(gc + 2, acc[11 + 3 * k]);
+    }
+  }
+  const float t = block_sum(lsum, s_red);
+  if (threadIdx.x == 0 && t != 0.f) atomicAdd(io.loss + 2 * vw.ch, t);
+}
+
+__global__ __launch_bounds__(kB) void boundary_kernel_p(Dev d, IO io) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N) return;
+  const f4 p = d.proj[vw.vn0 + v];
+  unsigned char fl = 0;
+  // (int)p - 1 >= 0 && (int)p + 1 < W, tested in float (NaN / huge safe)
+  if (p.x >= 1.f && p.x < (float)(vw.W - 1) && p.y >= 1.f && p.y < (float)(vw.H - 1)) {
+    const int u = (int)p.x, r0 = (int)p.y;
+    const int* fb = io.face + vw.pix0;
+    for (int cc = u - 1; cc <= u + 1; ++cc)
+      for (int r = r0 - 1; r <= r0 + 1; ++r)
+        if (fb[(size_t)r * vw.W + cc] < 0) fl = 1;
+  }
+  d.isb[vw.vn0 + v] = fl;
+}
+
+__device__ __forceinline__ float remap_p(int v) { return v >= 127 ? (float)((v - 127) * 2) : (float)v; }
+
+// Model->data residual (MultiViewSilhouetteLossGPUOp.cu:11-138) + loss + gradient.
+__global__ __launch_bounds__(kB) void sil_point_kernel_p(Dev d, IO io) {
+  __shared__ float s_red[kB / 64];
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  float lsum = 0.f;
+  if (v < c.N) {
+    const long long pi = vw.vn0 + v;
+    const float* cr = io.crops + (size_t)w * 7;
+    const f4 p = d.proj[pi];
+    const float pu = p.x, pv = p.y;
+    const bool in_view = pu - 2 > cr[3] && pu + 2 < cr[5] && pv - 2 > cr[4] && pv + 2 < cr[6];
+    const int xi = (int)((pu - cr[3]) / cr[2] + cr[0]), yi = (int)((pv - cr[4]) / cr[2] + cr[1]);
+    const int U = vw.S, Vv = vw.S;
+    float gx = 0.f, gy = 0.f;
+    if (in_view && xi - 1 >= 0 && xi + 1 < U && yi - 1 >= 0 && yi + 1 < Vv) {
+      const unsigned char* img = io.dt + vw.sil0;
+      auto at = [&](int xx, int yy) { return (int)img[(size_t)yy * U + xx]; };
+      float dv = remap_p(at(xi, yi));
+      const float f00 = remap_p(at(xi - 1, yi - 1)), f01 = remap_p(at(xi, yi - 1)), f02 = remap_p(at(xi + 1, yi - 1));
+      const float f10 = remap_p(at(xi - 1, yi)), f12 = remap_p(at(xi + 1, yi));
+      const float f20 = remap_p(at(xi - 1, yi + 1)), f21 = remap_p(at(xi, yi + 1)), f22 = remap_p(at(xi + 1, yi + 1));
+      const float du = (1.f / 8.f) * ((-1.f) * f00 + (+1.f) * f02 + (-2.f) * f10 + (+2.f) * f12 + (-1.f) * f20 + (+1.f) * f22);
+      const float dvv = (1.f / 8.f) * ((-1.f) * f00 + (-2.f) * f01 + (-1.f) * f02 + (+1.f) * f20 + (+2.f) * f21 + (+1.f) * f22);
+      float nx = du, ny = dvv, ux = du, uy = dvv;
+      const bool inside_fg = at(xi, yi) <= 127;
+      const float nonvan = (du != 0.f || dvv != 0.f) ? 1.f : 0.f;
+      const float bnd = d.isb[pi] ? 1.f : 0.f;
+      if (inside_fg) { nx = -nx; ny = -ny; }
+      const f2 nn = d.nrm2[pi];
+      if (!(nx * nn.x + ny * nn.y >= 0.f) && inside_fg) { ux = -ux; uy = -uy; }
+      dv *= bnd;
+      dv *= nonvan;
+      ux *= bnd; uy *= bnd;
+      ux *= nonvan; uy *= nonvan;
+      const float res = dv * (1.f / (255.f));
+      lsum = io.w_sil * res * res;
+      const float gres = 2.f * io.w_sil * res;
+      gx = ux * (1.f / (255.f)) * gres;
+      gy = uy * (1.f / (255.f)) * gres;
+    }
+    d.g_pts[2 * pi] = gx;
+    d.g_pts[2 * pi + 1] = gy;
+  }
+  const float t = block_sum(lsum, s_red);
+  if (threadIdx.x == 0 && t != 0.f) atomicAdd(io.loss + 2 * vw.ch + 1, t);
+}
+
+struct GridGeomP { float ox, oy, cell; };
+__device__ __forceinline__ GridGeomP grid_geom_p(const float* cr) {
+  const float sc = cr[2];
+  GridGeomP g;
+  g.cell = 30.f * sc * (1.f + 1.f / 1024.f);
+  g.ox = (0.f - cr[0]) * sc + cr[3] - 2.f * g.cell;
+  g.oy = (0.f - cr[1]) * sc + cr[4] - 2.f * g.cell;
+  return g;
+}
+__device__ __forceinline__ int cell_of_p(const GridGeomP& g, int gx, int gy, float x, float y) {
+  const float fx = floorf((x - g.ox) / g.cell), fy = floorf((y - g.oy) / g.cell);
+  if (!(fx >= 0.f && fy >= 0.f && fx < (float)gx && fy < (float)gy)) return -1;
+  return (int)fy * gx + (int)fx;
+}
+
+__global__ __launch_bounds__(kB) void sil_grid_count_p(Dev d, IO io) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N || !d.isb[vw.vn0 + v]) return;
+  const f4 p = d.proj[vw.vn0 + v];
+  const int cl = cell_of_p(grid_geom_p(io.crops + (size_t)w * 7), vw.gx, vw.gy, p.x, p.y);
+  if (cl >= 0) atomicAdd(d.cell_cnt + vw.cell0 + cl, 1u);
+}
+
+__global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N || !d.isb[vw.vn0 + v]) return;
+  const f4 p = d.proj[vw.vn0 + v];
+  const int cl = cell_of_p(grid_geom_p(io.crops + (size_t)w * 7), vw.gx, vw.gy, p.x, p.y);
+  if (cl < 0) return;
+  const long long cell = vw.cell0 + cl;
+  d.cell_items[d.cell_off[cell] + atomicAdd(d.cell_fill + cell, 1u)] = (unsigned int)v;
+}
+
+// Data->model residual via the binned search + loss + gradient (atomic, as the reference).
+__global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
+  __shared__ float s_red[kB / 64];
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const int S = vw.S;
+  const int p = blockIdx.x * kB + threadIdx.x;
+  float lsum = 0.f;
+  if (p < S * S) {
+    const int ui = p % S, vi = p / S;
+    const unsigned char* img = io.dt + vw.sil0;
+    bool bnd = img[(size_t)vi * S + ui] == 127;
+    if (bnd) {
+      int counter = 0;
+      for (int j = -2; j <= 2; ++j)
+        for (int i = -2; i <= 2; ++i) {
+          const int yy = vi + j, xx = ui + i;
+          if (yy >= 0 && yy < S && xx >= 0 && xx < S && img[(size_t)yy * S + xx] == 127) counter++;
+        }
+      bnd = counter >= 4;
+    }
+    if (bnd) {
+      const float* cr = io.crops + (size_t)w * 7;
+      const float sc = cr[2];
+      const float ix = ((float)ui - cr[0]) * sc + cr[3], iy = ((float)vi - cr[1]) * sc + cr[4];
+      float bd = 30.f * sc;
+      int best = -1;
+      const GridGeomP gg = grid_geom_p(cr);
+      const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
+      const f4* pv = d.proj + vw.vn0;
+      for (int yy = max(cy - 1, 0); yy <= min(cy + 1, vw.gy - 1); ++yy)
+        for (int xx = max(cx - 1, 0); xx <= min(cx + 1, vw.gx - 1); ++xx) {
+          const long long cell = vw.cell0 + yy * vw.gx + xx;
+          for (unsigned int k = d.cell_off[cell]; k < d.cell_off[cell + 1]; ++k) {
+            const int v = (int)d.cell_items[k];
+            const float dx = pv[v].x - ix, dy = pv[v].y - iy;
+            const float dd = sqrtf(dx * dx + dy * dy);
+            if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; }
+          }
+        }
+      if (best >= 0) {
+        const float r0 = (pv[best].x - ix) / (30.f * sc), r1 = (pv[best].y - iy) / (30.f * sc);
+        lsum = io.w_sil * (r0 * r0 + r1 * r1);
+        atomicAdd(d.g_pts + 2 * (vw.vn0 + best), 2.f * io.w_sil * r0 / (30.f * sc));
+        atomicAdd(d.g_pts + 2 * (vw.vn0 + best) + 1, 2.f * io.w_sil * r1 / (30.f * sc));
+      }
+    }
+  }
+  const float t = block_sum(lsum, s_red);
+  if (threadIdx.x == 0 && t != 0.f) atomicAdd(io.loss + 2 * vw.ch + 1, t);
+}
+
+// dL/dv (raster + projection adjoint over views) -> DQ-skinning adjoint.
+__global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
+  __shared__ float s_tab[256 * 8];
+  __shared__ float s_acc[256 * 8];
+  const CharD c = d.ch[blockIdx.y];
+  for (int i = threadIdx.x; i < c.J * 8; i += kB) { s_tab[i] = io.dq[(size_t)c.j0 * 8 + i]; s_acc[i] = 0.f; }
+  __syncthreads();
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v < c.N) {
+    const int gv = c.v0 + v;
+    f3 g = ld3(d.g_v + 3 * (size_t)gv);
+    const f3 X = ld3(d.vdef + 3 * (size_t)gv);
+    for (int k = 0; k < c.C; ++k) {
+      const int w = c.view0 + k;
+      const float* E = io.ext + (size_t)w * 12;
+      const float* K = io.intr + (size_t)w * 9;
+      const f2 gp = ((const f2*)d.g_pts)[d.vw[w].vn0 + v];
+      if (gp.x == 0.f && gp.y == 0.f) continue;
+      const float xc = E[0] * X.x + E[1] * X.y + E[2] * X.z + E[3];
+      const float yc = E[4] * X.x + E[5] * X.y + E[6] * X.z + E[7];
+      const float zc = E[8] * X.x + E[9] * X.y + E[10] * X.z + E[11];
+      const float ax = K[0], ay = K[4], gm = K[1];
+      const float j00 = ax / zc, j01 = gm / zc, j02 = (-ax * xc - gm * yc) / (zc * zc);
+      const float j11 = ay / zc, j12 = (-ay * yc) / (zc * zc);
+      const float g0 = gp.x * j00, g1 = gp.x * j01 + gp.y * j11, g2 = gp.x * j02 + gp.y * j12;
+      g = g + mk3(E[0] * g0 + E[4] * g1 + E[8] * g2, E[1] * g0 + E[5] * g1 + E[9] * g2, E[2] * g0 + E[6] * g1 + E[10] * g2);
+    }
+    const f3 vnr = ed_inner(d, c, gv);
+    int idx[KS];
+    float w[KS];
+    for (int k = 0; k < KS; ++k) { idx[k] = d.jidx[(size_t)gv * KS + k]; w[k] = d.jw[(size_t)gv * KS + k]; }
+    const DqBlend bl = dq_blend(s_tab, idx, w, KS);
+    const m3 R = quat_to_mat(bl.r);
+    const float G[9] = {g.x * vnr.x, g.x * vnr.y, g.x * vnr.z, g.y * vnr.x, g.y * vnr.y, g.y * vnr.z,
+                        g.z * vnr.x, g.z * vnr.y, g.z * vnr.z};
+    const f4 r = bl.r, dd = bl.d;
+    f4 gr = dR_dq_contract(r, G);
+    gr.x += 2.f * (-dd.w * g.x - dd.z * g.y + dd.y * g.z);
+    gr.y += 2.f * (dd.z * g.x - dd.w * g.y - dd.x * g.z);
+    gr.z += 2.f * (-dd.y * g.x + dd.x * g.y - dd.w * g.z);
+    gr.w += 2.f * (dd.x * g.x + dd.y * g.y + dd.z * g.z);
+    const f4 gd = mk4(2.f * (r.w * g.x + r.z * g.y - r.y * g.z), 2.f * (-r.z * g.x + r.w * g.y + r.x * g.z),
+                      2.f * (r.y * g.x - r.x * g.y + r.w * g.z), 2.f * (-r.x * g.x - r.y * g.y - r.z * g.z));
+    f4 gbr, gbd;
+    if (bl.small) {
+      gbr = gr; gbd = gd;
+    } else {
+      const float proj = dot(gr, r) + dot(gd, dd);
+      gbr = (gr - r * proj) * bl.scale;
+      gbd = gd * bl.scale;
+    }
+    for (int k = 0; k < KS; ++k) {
+      const float ws = w[k] * bl.sgn[k];
+      float* a = s_acc + idx[k] * 8;
+      atomicAdd(a, gbr.x * ws); atomicAdd(a + 1, gbr.y * ws); atomicAdd(a + 2, gbr.z * ws); atomicAdd(a + 3, gbr.w * ws);
+      atomicAdd(a + 4, gbd.x * ws); atomicAdd(a + 5, gbd.y * ws); atomicAdd(a + 6, gbd.z * ws); atomicAdd(a + 7, gbd.w * ws);
+    }
+    const f3 u = mulT(R, g);
+    d.u[3 * (size_t)gv] = u.x; d.u[3 * (size_t)gv + 1] = u.y; d.u[3 * (size_t)gv + 2] = u.z;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < c.J * 8; i += kB)
+    if (s_acc[i] != 0.f) atomicAdd(io.g_dq + (size_t)c.j0 * 8 + i, s_acc[i]);
+}
+
+__global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
+  const int node = (blockIdx.x * kB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (node >= totM) return;
+  // character of this node: linear scan over the (few) characters
+  int b = 0;
+  while (b + 1 < d.B && d.ch[b + 1].m0 <= node) ++b;
+  const CharD c = d.ch[b];
+  const EulerSC sd = euler_sc(ld3(io.dA + 3 * (size_t)node));
+  const m3 da = euler_dA(sd), db = euler_dB(sd), dg = euler_dG(sd);
+  const f3 gn = ld3(d.rest + 3 * ((size_t)c.v0 + d.node_vtx[node]));
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int e = d.n2v_off[node] + lane; e < d.n2v_off[node + 1]; e += 64) {
+    const int gv = c.v0 + d.n2v_vtx[e];
+    const float wgt = d.n2v_w[e];
+    const f3 uv = ld3(d.u + 3 * (size_t)gv);
+    const f3 q = ld3(d.rest + 3 * (size_t)gv) - gn;
+    acc[0] += wgt * uv.x; acc[1] += wgt * uv.y; acc[2] += wgt * uv.z;
+    acc[3] += wgt * dot(mul(da, q), uv);
+    acc[4] += wgt * dot(mul(db, q), uv);
+    acc[5] += wgt * dot(mul(dg, q), uv);
+  }
+  for (int k = 0; k < 6; ++k) acc[k] = wave_sum(acc[k]);
+  if (lane == 0) {
+    io.g_dT[3 * (size_t)node] = acc[0]; io.g_dT[3 * (size_t)node + 1] = acc[1]; io.g_dT[3 * (size_t)node + 2] = acc[2];
+    io.g_dA[3 * (size_t)node] = acc[3]; io.g_dA[3 * (size_t)node + 1] = acc[4]; io.g_dA[3 * (size_t)node + 2] = acc[5];
+  }
+}
+
+template <class T>
+hipError_t up(T** dst, const std::vector<T>& src) {
+  hipError_t e = hipMalloc(dst, sizeof(T) * std::max<size_t>(src.size(), 1));
+  if (e != hipSuccess || src.empty()) return e;
+  return hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+}  // namespace dcx
+
+using namespace dcx;
+
+struct dcx_pipeline {
+  Dev d{};
+  int B = 0, V = 0, sumN = 0, sumF = 0, sumM = 0, sumJ = 0;
+  long long sumS2 = 0;
+  std::vector<void*> allocs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void pipe_free(dcx_pipeline* p) {
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  for (void* a : p->allocs) hipFree(a);
+  delete p;
+}
+
+extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* chars, long long bin_capacity,
+                                          dcx_pipeline** out) {
+  DCX_REQUIRE(out && chars && n_chars > 0, "dcx_pipeline_create: bad arguments");
+  dcx_pipeline* p = new dcx_pipeline();
+  p->B = n_chars;
+  std::vector<CharD> ch(n_chars);
+  std::vector<ViewD> vw;
+  std::vector<float> rest, jw, ew;
+  std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list;
+  std::vector<float> n2v_w;
+  std::vector<long long> tile0;
+  long long vn = 0, vf = 0, pix = 0, sil = 0, tiles = 0, cells = 0;
+  int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0;
+  for (int b = 0; b < n_chars; ++b) {
+    const dcx_char_desc& c = chars[b];
+    if (!(c.num_vertices > 0 && c.num_faces > 0 && c.num_nodes > 0 && c.num_joints > 0 && c.num_joints <= 256 &&
+          c.num_cameras > 0 && c.width > 0 && c.height > 0 && c.dt_size > 2 && c.width <= 32767 && c.height <= 32767)) {
+      delete p;
+      return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: character %d has invalid sizes", b);
+    }
+    CharD& d = ch[b];
+    d.N = c.num_vertices; d.F = c.num_faces; d.M = c.num_nodes; d.J = c.num_joints;
+    d.v0 = p->sumN; d.f0 = p->sumF; d.m0 = p->sumM; d.j0 = p->sumJ;
+    d.view0 = (int)vw.size(); d.C = c.num_cameras;
+    for (int i = 0; i < 3 * d.N; ++i) rest.push_back(c.rest_pos[i]);
+    for (int i = 0; i < 3 * d.F; ++i) {
+      if (c.faces[i] < 0 || c.faces[i] >= d.N) { delete p; return set_error(DCX_ERR_INVALID, "character %d: bad face index", b); }
+      faces.push_back(c.faces[i]);
+    }
+    for (int i = 0; i < KS * d.N; ++i) {
+      if (c.joint_idx[i] < 0 || c.joint_idx[i] >= d.J) { delete p; return set_error(DCX_ERR_INVALID, "character %d: bad joint index", b); }
+      jidx.push_back(c.joint_idx[i]);
+      jw.push_back(c.joint_w[i]);
+    }
+    for (int i = 0; i < KE * d.N; ++i) {
+      if (c.node_idx[i] < 0 || c.node_idx[i] >= d.M) { delete p; return set_error(DCX_ERR_INVALID, "character %d: bad node index", b); }
+      enode.push_back(c.node_idx[i]);
+      ew.push_back(c.node_w[i]);
+    }
+    for (int i = 0; i < d.M; ++i) {
+      if (c.node_vertex[i] < 0 || c.node_vertex[i] >= d.N) { delete p; return set_error(DCX_ERR_INVALID, "character %d: bad node vertex", b); }
+      node_vtx.push_back(c.node_vertex[i]);
+    }
+    // node -> vertex CSR (vertices ascending)
+    std::vector<int> cnt(d.M + 1, 0);
+    for (int i = 0; i < KE * d.N; ++i) cnt[c.node_idx[i] + 1]++;
+    for (int m = 0; m < d.M; ++m) cnt[m + 1] += cnt[m];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    const int base = (int)n2v_vtx.size();
+    n2v_vtx.resize(base + KE * d.N);
+    n2v_w.resize(base + KE * d.N);
+    for (int v = 0; v < d.N; ++v)
+      for (int k = 0; k < KE; ++k) {
+        const int m = c.node_idx[KE * v + k];
+        const int slot = base + fill[m]++;
+        n2v_vtx[slot] = v;
+        n2v_w[slot] = c.node_w[KE * v + k];
+      }
+    for (int m = 0; m < d.M; ++m) n2v_off.push_back(base + cnt[m + 1]);
+    // vertex -> face CSR
+    std::vector<int> vc(d.N + 1, 0);
+    for (int i = 0; i < 3 * d.F; ++i) vc[c.faces[i] + 1]++;
+    for (int v = 0; v < d.N; ++v) vc[v + 1] += vc[v];
+    std::vector<int> vfill(vc.begin(), vc.end() - 1);
+    const int vbase = (int)vf_list.size();
+    vf_list.resize(vbase + 3 * d.F);
+    for (int f = 0; f < d.F; ++f)
+      for (int k = 0; k < 3; ++k) vf_list[vbase + vfill[c.faces[3 * f + k]]++] = f;
+    for (int v = 0; v < d.N; ++v) vf_off.push_back(vbase + vc[v + 1]);
+    for (int k = 0; k < d.C; ++k) {
+      ViewD v{};
+      v.ch = b; v.W = c.width; v.H = c.height; v.S = c.dt_size;
+      v.tiles_x = (c.width + kTileW - 1) / kTileW;
+      v.tiles = v.tiles_x * ((c.height + kTileH - 1) / kTileH);
+      v.vn0 = vn; v.vf0 = vf; v.pix0 = pix; v.sil0 = sil; v.tile0 = tiles; v.cell0 = cells;
+      v.gx = c.dt_size / 30 + 6; v.gy = c.dt_size / 30 + 6;
+      vn += d.N; vf += d.F; pix += (long long)c.width * c.height; sil += (long long)c.dt_size * c.dt_size;
+      tile0.push_back(tiles);
+      tiles += v.tiles; cells += (long long)v.gx * v.gy;
+      vw.push_back(v);
+    }
+    p->sumN += d.N; p->sumF += d.F; p->sumM += d.M; p->sumJ += d.J;
+    maxN = std::max(maxN, d.N); maxF = std::max(maxF, d.F); maxM = std::max(maxM, d.M);
+    maxS2 = std::max(maxS2, c.dt_size * c.dt_size);
+  }
+  tile0.push_back(tiles);
+  p->V = (int)vw.size();
+  p->sumS2 = sil;
+  Dev& d = p->d;
+  d.B = n_chars; d.V = p->V; d.maxN = maxN; d.maxF = maxF; d.maxM = maxM; d.maxS2 = maxS2;
+  d.totVN = vn; d.totVF = vf; d.totPix = pix; d.totTiles = tiles; d.totCells = cells;
+  const long long cap = bin_capacity > 0 ? bin_capacity : std::max<long long>(4 * vf, 1 << 20);
+  if (cap >= (1ll << 32) || vf >= (1ll << 32) || (tiles + 1023) / 1024 > 4096 || (cells + 1023) / 1024 > 4096) {
+    delete p;
+    return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: batch too large (bins %lld, tiles %lld)", cap, tiles);
+  }
+  d.capacity = (unsigned int)cap;
+  hipError_t e = hipSuccess;
+  auto track = [&](void* a) { p->allocs.push_back(a); };
+  auto alloc = [&](auto** dst, size_t bytes) {
+    if (e != hipSuccess) return;
+    e = hipMalloc((void**)dst, std::max<size_t>(bytes, 1));
+    if (e == hipSuccess) track((void*)*dst);
+  };
+  auto upload = [&](auto** dst, const auto& v) {
+    if (e != hipSuccess) return;
+    e = up(dst, v);
+    if (e == hipSuccess) track((void*)*dst);
+  };
+  CharD* dch; ViewD* dvw; long long* dt0;
+  float* drest; int *dfaces, *djidx, *denode, *dnv, *dn2o, *dn2v, *dvfo, *dvfl; float *djw, *dew, *dn2w;
+  upload(&dch, ch); upload(&dvw, vw); upload(&dt0, tile0);
+  upload(&drest, rest); upload(&dfaces, faces); upload(&djidx, jidx); upload(&djw, jw);
+  upload(&denode, enode); upload(&dew, ew); upload(&dnv, node_vtx); upload(&dn2o, n2v_off);
+  upload(&dn2v, n2v_vtx); upload(&dn2w, n2v_w); upload(&dvfo, vf_off); upload(&dvfl, vf_list);
+  d.ch = dch; d.vw = dvw; d.tile0 = dt0; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
+  d.enode = denode; d.ew = dew; d.node_vtx = dnv; d.n2v_off = dn2o; d.n2v_vtx = dn2v; d.n2v_w = dn2w;
+  d.vf_off = dvfo; d.vf_list = dvfl;
+  alloc(&d.node_tab, sizeof(float) * kNodeRow * (size_t)p->sumM);
+  alloc(&d.vdef, sizeof(float) * 3 * (size_t)p->sumN);
+  alloc(&d.vnrm, sizeof(float) * 3 * (size_t)p->sumN);
+  alloc(&d.proj, sizeof(f4) * (size_t)vn);
+  alloc(&d.nrm2, sizeof(f2) * (size_t)vn);
+  alloc(&d.tri, sizeof(TriRec) * (size_t)vf);
+  alloc(&d.tile_cnt, 4 * (size_t)tiles);
+  alloc(&d.tile_off, 4 * (size_t)(tiles + 1));
+  alloc(&d.tile_fill, 4 * (size_t)tiles);
+  alloc(&d.scan_aux, 4 * (size_t)((tiles + 1023) / 1024 + 1));
+  alloc(&d.bins, 4 * (size_t)cap);
+  alloc(&d.isb, (size_t)vn);
+  alloc(&d.cell_cnt, 4 * (size_t)cells);
+  alloc(&d.cell_off, 4 * (size_t)(cells + 1));
+  alloc(&d.cell_fill, 4 * (size_t)cells);
+  alloc(&d.cell_items, 4 * (size_t)vn);
+  alloc(&d.scan_aux2, 4 * (size_t)((cells + 1023) / 1024 + 1));
+  alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
+  alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
+  alloc(&d.u, sizeof(float) * 3 * (size_t)p->sumN);
+  if (e != hipSuccess) {
+    pipe_free(p);
+    return set_error(DCX_ERR_OOM, "dcx_pipeline_create: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return DCX_OK;
+}
+
+extern "C" dcx_status dcx_pipeline_destroy(dcx_pipeline* p) {
+  if (p) pipe_free(p);
+  return DCX_OK;
+}
+
+extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes) {
+  DCX_REQUIRE(p && sizes, "dcx_pipeline_sizes: null argument");
+  sizes[0] = p->B; sizes[1] = p->V; sizes[2] = p->sumN; sizes[3] = p->sumF; sizes[4] = p->sumM; sizes[5] = p->sumJ;
+  sizes[6] = p->d.totPix; sizes[7] = p->sumS2; sizes[8] = p->d.totTiles; sizes[9] = p->d.totVF;
+  return DCX_OK;
+}
+
+// Launch order of one step; dcx_pipeline_profile times each entry.
+static const char* const kStageNames[] = {
+    "clear",         "node_fwd",      "vertex_fwd",   "normals",        "project",       "tri_count",
+    "tile_scan",     "bin_scatter",   "fine",         "boundary",       "sil_point",     "sil_grid_count",
+    "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "vertex_bwd",     "node_bwd"};
+constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
+
+// ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
+// and ev[kNumStages] after the last one.
+static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipStream_t s,
+                               hipEvent_t* ev = nullptr) {
+  DCX_REQUIRE(in->dq && in->delta_t && in->delta_a && in->colors && in->extrinsics && in->intrinsics && in->target &&
+                  in->dt && in->crops && in->face && in->bary && in->render && in->loss && in->g_dq &&
+                  in->g_delta_t && in->g_delta_a && in->g_colors,
+              "dcx_pipeline: null buffer in io");
+  Dev d = p->d;
+  IO io{in->dq, in->delta_t, in->delta_a, in->colors, in->extrinsics, in->intrinsics, in->target, in->dt, in->crops,
+        in->w_color, in->w_sil, in->face, in->bary, in->render, in->vertices, in->loss, in->g_dq, in->g_delta_t,
+        in->g_delta_a, in->g_colors};
+  ClearList L{};
+  auto add = [&](void* ptr, long long n) { L.p[L.count] = (unsigned int*)ptr; L.n[L.count] = n; L.count++; };
+  add(d.g_v, 3ll * p->sumN);
+  add(in->g_colors, 3ll * p->sumN);
+  add(in->g_dq, 8ll * p->sumJ);
+  add(in->loss, 2ll * p->B);
+  add(d.tile_cnt, d.totTiles);
+  add(d.tile_fill, d.totTiles);
+  add(d.cell_cnt, d.totCells);
+  add(d.cell_fill, d.totCells);
+  int stage = 0;
+  auto mark = [&]() { if (ev) hipEventRecord(ev[stage], s); stage++; };
+  mark();
+  hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, s, L);
+  const int B = p->B, V = p->V;
+  mark();
+  hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d);
+  mark();
+  hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
+  mark();
+  dcx_status st = exclusive_scan(d.tile_cnt, d.tile_off, d.scan_aux, d.totTiles, s);
+  if (st) return st;
+  mark();
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
+  mark();
+  hipLaunchKernelGGL(fine_kernel, dim3((unsigned)d.totTiles), dim3(kTilePix), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(boundary_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(sil_point_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(sil_grid_count_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  mark();
+  st = exclusive_scan(d.cell_cnt, d.cell_off, d.scan_aux2, d.totCells, s);
+  if (st) return st;
+  mark();
+  hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(ceil_div(d.maxS2, kB), V), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(vertex_bwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
+  mark();
+  return check_launch("dcx_pipeline_step");
+}
+
+extern "C" dcx_status dcx_pipeline_step(dcx_pipeline* p, const dcx_pipeline_io* io, void* stream) {
+  DCX_REQUIRE(p && io, "dcx_pipeline_step: null argument");
+  return pipe_enqueue(p, io, as_stream(stream));
+}
+
+extern "C" dcx_status dcx_pipeline_capture(dcx_pipeline* p, const dcx_pipeline_io* io, void* stream) {
+  DCX_REQUIRE(p && io, "dcx_pipeline_capture: null argument");
+  DCX_REQUIRE(stream, "dcx_pipeline_capture: capture needs a non-default stream");
+  hipStream_t s = as_stream(stream);
+  if (p->exec) { hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+  if (p->graph) { hipGraphDestroy(p->graph); p->graph = nullptr; }
+  DCX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  dcx_status st = pipe_enqueue(p, io, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (st) { if (g) hipGraphDestroy(g); return st; }
+  if (e != hipSuccess) return set_error(DCX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  p->graph = g;
+  DCX_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+  return DCX_OK;
+}
+
+extern "C" dcx_status dcx_pipeline_replay(dcx_pipeline* p, void* stream) {
+  DCX_REQUIRE(p && p->exec, "dcx_pipeline_replay: no captured graph");
+  DCX_HIP(hipGraphLaunch(p->exec, as_stream(stream)));
+  return DCX_OK;
+}
+
+extern "C" int dcx_pipeline_num_stages(void) { return kNumStages; }
+
+extern "C" const char* dcx_pipeline_stage_name(int i) {
+  return (i >= 0 && i < kNumStages) ? kStageNames[i] : nullptr;
+}
+
+extern "C" dcx_status dcx_pipeline_profile(dcx_pipeline* p, const dcx_pipeline_io* io, float* stage_ms, int n_stages,
+                                           void* stream) {
+  DCX_REQUIRE(p && io && stage_ms, "dcx_pipeline_profile: null argument");
+  DCX_REQUIRE(n_stages == kNumStages, "dcx_pipeline_profile: n_stages must be %d", kNumStages);
+  hipStream_t s = as_stream(stream);
+  hipEvent_t ev[kNumStages + 1] = {};
+  dcx_status st = DCX_OK;
+  for (int i = 0; i <= kNumStages && st == DCX_OK; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) st = set_error(DCX_ERR_HIP, "dcx_pipeline_profile: hipEventCreate");
+  if (st == DCX_OK) st = pipe_enqueue(p, io, s, ev);
+  if (st == DCX_OK && hipEventSynchronize(ev[kNumStages]) != hipSuccess)
+    st = set_error(DCX_ERR_HIP, "dcx_pipeline_profile: hipEventSynchronize");
+  for (int i = 0; i < kNumStages && st == DCX_OK; ++i)
+    if (hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]) != hipSuccess)
+      st = set_error(DCX_ERR_HIP, "dcx_pipeline_profile: hipEventElapsedTime");
+  for (int i = 0; i <= kNumStages; ++i)
+    if (ev[i]) hipEventDestroy(ev[i]);
+  return st;
+}

@@ -299,9 +299,12 @@ __global__ __launch_bounds__(kB) void boundary_kernel(const int* __restrict__ fa
   const long long i = (long long)blockIdx.x * kB + threadIdx.x;
   if (i >= (long long)B * C * N) return;
   const int view = (int)(i / N);
-  const int u = (int)uv[2 * i], w = (int)uv[2 * i + 1];
+  const float pu = uv[2 * i], pw = uv[2 * i + 1];
   unsigned char f = 0;
-  if (u - 1 >= 0 && u + 1 < W && w - 1 >= 0 && w + 1 < H) {
+  // (int)p - 1 >= 0 && (int)p + 1 < W, tested in float so that huge or NaN
+  // positions never reach the integer conversion
+  if (pu >= 1.f && pu < (float)(W - 1) && pw >= 1.f && pw < (float)(H - 1)) {
+    const int u = (int)pu, w = (int)pw;
     const int* fb = face + (size_t)view * H * W;
     for (int cc = u - 1; cc <= u + 1; ++cc)
       for (int r = w - 1; r <= w + 1; ++r)

@@ -206,10 +206,15 @@ def farthest_point_sample(verts: np.ndarray, M: int, seed_index: int = 0) -> np.
     M = min(M, N)
     sel = np.empty(M, dtype=np.int64)
     sel[0] = seed_index
-    d = np.linalg.norm(verts - verts[seed_index], axis=1)
+    x = np.ascontiguousarray(verts, dtype=np.float64)
+    d = np.einsum("ij,ij->i", x - x[seed_index], x - x[seed_index])
+    diff = np.empty_like(x)
+    dn = np.empty(N)
     for i in range(1, M):
         sel[i] = int(np.argmax(d))
-        d = np.minimum(d, np.linalg.norm(verts - verts[sel[i]], axis=1))
+        np.subtract(x, x[sel[i]], out=diff)
+        np.einsum("ij,ij->i", diff, diff, out=dn)
+        np.minimum(d, dn, out=d)
     return sel
 
 
@@ -234,14 +239,9 @@ def embedded_graph(verts: np.ndarray, M: int, K: int = 4, sigma: float = 0.05) -
     gpos = verts[nodes]
     N = verts.shape[0]
     K = min(K, len(nodes))
-    idx = np.empty((N, K), dtype=np.int64)
-    dist = np.empty((N, K))
-    chunk = 8192
-    for s in range(0, N, chunk):
-        d = np.linalg.norm(verts[s:s + chunk, None, :] - gpos[None], axis=2)
-        ii = np.argsort(d, axis=1, kind="stable")[:, :K]
-        idx[s:s + chunk] = ii
-        dist[s:s + chunk] = np.take_along_axis(d, ii, axis=1)
+    from scipy.spatial import cKDTree
+    dist, idx = cKDTree(gpos).query(verts, k=K)
+    dist, idx = dist.reshape(N, K), idx.reshape(N, K)
     w = np.exp(-dist ** 2 / (2 * sigma ** 2))
     tot = w.sum(axis=1, keepdims=True)
     bad = tot[:, 0] <= 1e-12

@@ -331,6 +331,84 @@ dcx_status dcx_camproj_bwd(const float* points_image_grad, const float* points,
 dcx_status dcx_mesh_boundary(const int32_t* face, const float* points_image, int B, int C, int N,
                              int H, int W, uint8_t* is_boundary, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched character pipeline (fused forward + backward, graph-capturable)  */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * One step for many characters of different sizes, each seen by several
+ * cameras, with no counterpart op in the reference (which loops over batch
+ * elements on the host and chains separate ops):
+ *   ED non-rigid deformation of the rest mesh (delta node rotation as Euler
+ *   angles + translation, 4 nodes per vertex) -> dual-quaternion skinning
+ *   (4 joints per vertex) -> rasterization (vertexColor, shadeless) -> L1
+ *   colour loss against target images (+ the rasterizer's model-to-data
+ *   term) and the multi-view DT silhouette loss (squared residuals of both
+ *   MultiViewSilhouetteLossGpu terms, boundary flags from the rendered face
+ *   buffer) -> gradients w.r.t. joint DQs, ED node translations / Euler
+ *   angles and vertex colours.
+ */
+typedef struct dcx_pipeline dcx_pipeline;
+
+typedef struct {
+  int num_vertices, num_faces, num_nodes, num_joints; /* J <= 256 */
+  int num_cameras, width, height;                     /* views of this character */
+  int dt_size;                                        /* DT images are dt_size x dt_size */
+  /* HOST arrays, local indices: */
+  const float* rest_pos;      /* [N,3] */
+  const int32_t* faces;       /* [F,3] */
+  const int32_t* joint_idx;   /* [N,4] */
+  const float* joint_w;       /* [N,4] */
+  const int32_t* node_vertex; /* [M]  base-mesh vertex of each ED node */
+  const int32_t* node_idx;    /* [N,4] ED nodes of each vertex */
+  const float* node_w;        /* [N,4] */
+} dcx_char_desc;
+
+typedef struct {
+  /* DEVICE inputs, characters (and their views) concatenated in order */
+  const float* dq;          /* [sum J,8] */
+  const float* delta_t;     /* [sum M,3] */
+  const float* delta_a;     /* [sum M,3] Euler angles */
+  const float* colors;      /* [sum N,3] */
+  const float* extrinsics;  /* [views,12] */
+  const float* intrinsics;  /* [views,9] */
+  const float* target;      /* [sum over views of H*W, 3] */
+  const uint8_t* dt;        /* [sum over views of S*S] */
+  const float* crops;       /* [views,7] */
+  float w_color, w_sil;
+  /* DEVICE outputs */
+  int32_t* face;            /* [sum H*W] */
+  float* bary;              /* [sum H*W,2] */
+  float* render;            /* [sum H*W,3] */
+  float* vertices;          /* [sum N,3] skinned vertices (may be NULL) */
+  float* loss;              /* [n_chars,2] (colour, silhouette) */
+  float* g_dq;              /* [sum J,8] */
+  float* g_delta_t;         /* [sum M,3] */
+  float* g_delta_a;         /* [sum M,3] */
+  float* g_colors;          /* [sum N,3] */
+} dcx_pipeline_io;
+
+/* bin_capacity <= 0 selects 4 tile entries per (view, face). */
+dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* chars, long long bin_capacity,
+                               dcx_pipeline** out);
+dcx_status dcx_pipeline_destroy(dcx_pipeline* pipe);
+/* sizes[10]: chars, views, sum N, sum F, sum M, sum J, pixels, DT pixels, tiles, view-faces */
+dcx_status dcx_pipeline_sizes(const dcx_pipeline* pipe, long long* sizes);
+/* Enqueue one forward + backward step on `stream`. */
+dcx_status dcx_pipeline_step(dcx_pipeline* pipe, const dcx_pipeline_io* io, void* stream);
+/* Capture one step into a hipGraph (stream must not be the legacy stream);
+ * the io pointers are baked into the graph. */
+dcx_status dcx_pipeline_capture(dcx_pipeline* pipe, const dcx_pipeline_io* io, void* stream);
+dcx_status dcx_pipeline_replay(dcx_pipeline* pipe, void* stream);
+/* Per-stage timing of one eager step: HIP events recorded on `stream` between
+ * the launches; stage_ms[i] is the elapsed time of stage i (n_stages must be
+ * dcx_pipeline_num_stages()).  Synchronises the stream.  No counterpart in the
+ * reference (its ops are timed by the framework profiler). */
+int dcx_pipeline_num_stages(void);
+const char* dcx_pipeline_stage_name(int stage);
+dcx_status dcx_pipeline_profile(dcx_pipeline* pipe, const dcx_pipeline_io* io, float* stage_ms, int n_stages,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,50 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement of one differentiable-character step (ED -> DQ skinning ->
+raster -> L1 colour loss + multi-view DT silhouette loss -> gradients),
+composed from the oracle's op restatements in the order the reference's ops
+are chained.  Used by tests/ as the checker of the batched pipeline and by
+bench.py's cpu_baseline leg as the timed CPU path.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import oracle as orc
+
+
+def step(wk, w_col: float, w_sil: float, dtype=np.float64):
+    """Returns dict(loss=[colour, silhouette], g_dq, g_dT, g_dA, g_col, face, render)."""
+    ch = wk.ch
+    g = ch.graph
+    M, N = g.num_nodes, ch.N
+    ext = wk.ext.reshape(1, -1)
+    intr = wk.intr.reshape(1, -1)
+    zM = np.zeros((1, M, 3))
+    # forward
+    # the forward runs in float32 like the product so that the discrete
+    # decisions (tri ids, boundary flags, nearest points) see the same inputs
+    v_nr = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], zM, zM, np.zeros((1, N, 3)), dtype=np.float32)[0]
+    pos, _ = orc.dqs_fwd(wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=np.float32)
+    face, bary, render, vn = orc.raster_fwd(ch.faces, pos.astype(np.float32), ext, intr, wk.H, wk.W,
+                                            vcol=ch.colors[None], dtype=np.float32)
+    nrm = vn[:, 0].astype(np.float64)
+    nrm = nrm / np.maximum(np.linalg.norm(nrm, axis=-1, keepdims=True), 1e-30)
+    pts = orc.camproj(pos, None, ext, intr, True, dtype=np.float32)
+    nrm2 = orc.camproj(pos, nrm, ext, intr, False, dtype=np.float32)
+    isb = orc.boundary_flags(pts, (face >= 0).astype(np.uint8))
+    res, res1, dtg, cid = orc.mvsil_fwd(pts, nrm2, isb, wk.dt[None], wk.crops[None])
+    diff = render.astype(np.float64) - wk.target[None]
+    loss_col = w_col * np.abs(diff).sum()
+    loss_sil = w_sil * (np.sum(res.astype(np.float64) ** 2) + np.sum(res1.astype(np.float64) ** 2))
+    # backward
+    g_render = w_col * np.sign(diff) * (face >= 0)[..., None]
+    g_pts = orc.mvsil_bwd(2 * w_sil * res, 2 * w_sil * res1, dtg, cid, wk.crops[None])
+    g_pos = orc.camproj_bwd(g_pts, pos, ext, intr)
+    g_pos_r, g_col, _, _ = orc.raster_bwd(ch.faces, g_render, pos, ext, intr, face, bary, vn, vcol=ch.colors[None],
+                                          target=wk.target[None], dtype=dtype)
+    g_pos = g_pos + g_pos_r
+    g_dq, _, g_rest = orc.dqs_bwd(g_pos, wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=dtype)
+    gT, gA, _, _, _ = orc.ed_bwd(g, ch.verts, g_rest, wk.dA[None], zM, dtype=dtype)
+    return dict(loss=np.array([loss_col, loss_sil]), g_dq=g_dq[0], g_dT=gT[0], g_dA=gA[0], g_col=g_col[0],
+                face=face[0], render=render[0])

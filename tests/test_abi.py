@@ -73,6 +73,8 @@ def declared_params():
                         kinds.append("z")
                     elif "float" in a:
                         kinds.append("f")
+                    elif "long long" in a:
+                        kinds.append("l")
                     else:
                         kinds.append("i")
             out[m.group(1)] = kinds

@@ -27,7 +27,7 @@ def setup(seed=0, subdiv=4, M=300, B=3, markers=True):
     sT = rng.normal(size=(B, M, 3)) * 0.05
     sA = rng.uniform(-0.5, 0.5, size=(B, M, 3))
     disp = rng.normal(size=(B, ch.N, 3)) * 0.002
-    mk_node = np.array([1, 10, 50, 120], np.int32) if markers else None
+    mk_node = (np.array([1, 10, 50, 120]) % M).astype(np.int32) if markers else None
     bm = ch.verts[g.node_vertex[mk_node]] + 0.01 if markers else None
     topo = renderer.Topology(ch.faces, ch.N)
     gd = edm.GraphData(ch.verts, g, bm, mk_node, topo)
