@@ -118,8 +118,11 @@ def make_targets(works: list[CharacterWork], render_fn, rng, pert=0.05):
         dq = scenes.dq_from_rt(qr, t)
         face, rend = render_fn(wk, dq, wk.dT, wk.dA)
         wk.target = rend.astype(np.float32)
-        step = wk.W // wk.S
-        masks = face[:, ::step, ::step][:, :wk.S, :wk.S] >= 0
+        # square S x S grid with spacing `step` px covering the whole image
+        step = -(-max(wk.W, wk.H) // wk.S)
+        sub = face[:, ::step, ::step][:, :wk.S, :wk.S] >= 0
+        masks = np.zeros((face.shape[0], wk.S, wk.S), bool)
+        masks[:, :sub.shape[1], :sub.shape[2]] = sub
         wk.dt = np.stack([encode_dt(m) for m in masks])
         sc = float(step)
         wk.crops = np.tile(np.array([0.0, 0.0, sc, 0.0, 0.0, wk.S * sc, wk.S * sc], np.float32), (wk.ext.shape[0], 1))
@@ -142,10 +145,41 @@ def build_works(n_chars: int, verts_range=(8000, 60000), nodes_range=(500, 1500)
     return works
 
 
+def check_works(works: list[CharacterWork]) -> None:
+    """Raises ValueError unless every array has the shape the batched kernels
+    index it with (per character: dq [J,8], dT/dA [M,3], colours [N,3],
+    cameras [C,12]/[C,9], target [C,H,W,3], DT [C,S,S] uint8, crops [C,7],
+    skinning/ED tables [N,4], faces [F,3])."""
+    if not works:
+        raise ValueError("no characters")
+    for i, wk in enumerate(works):
+        ch = wk.ch
+        C = wk.ext.shape[0] if wk.ext.ndim == 2 else -1
+        M = ch.graph.num_nodes
+        expect = {
+            "dq": (wk.dq, (ch.J, 8)), "dT": (wk.dT, (M, 3)), "dA": (wk.dA, (M, 3)),
+            "colors": (ch.colors, (ch.N, 3)), "verts": (ch.verts, (ch.N, 3)), "faces": (ch.faces, (ch.F, 3)),
+            "joint_idx": (ch.joint_idx, (ch.N, 4)), "joint_w": (ch.joint_w, (ch.N, 4)),
+            "ext": (wk.ext, (C, 12)), "intr": (wk.intr, (C, 9)),
+            "target": (wk.target, (C, wk.H, wk.W, 3)), "dt": (wk.dt, (C, wk.S, wk.S)), "crops": (wk.crops, (C, 7)),
+        }
+        for name, (a, shape) in expect.items():
+            if a is None or tuple(np.shape(a)) != shape:
+                raise ValueError(f"character {i}: {name} has shape {None if a is None else np.shape(a)}, "
+                                 f"expected {shape}")
+        if wk.dt.dtype != np.uint8:
+            raise ValueError(f"character {i}: dt must be uint8")
+        if not (1 <= ch.J <= 256 and C >= 1 and wk.S > 2 and 0 < wk.W <= 32767 and 0 < wk.H <= 32767):
+            raise ValueError(f"character {i}: unsupported sizes J={ch.J} C={C} S={wk.S} W={wk.W} H={wk.H}")
+        if ch.faces.min() < 0 or ch.faces.max() >= ch.N or ch.joint_idx.min() < 0 or ch.joint_idx.max() >= ch.J:
+            raise ValueError(f"character {i}: face or joint index out of range")
+
+
 class BatchedPipeline:
     """Fused step over all characters (dcx_pipeline_*)."""
 
     def __init__(self, works: list[CharacterWork], device="cuda", w_color=None, w_sil=1.0, bin_capacity=0):
+        check_works(works)
         self.works = works
         self.device = torch.device(device)
         keep = []
@@ -180,6 +214,13 @@ class BatchedPipeline:
         self.dt = t([wk.dt.reshape(-1) for wk in works], torch.uint8)
         self.crops = t([wk.crops for wk in works])
         npix = self.sizes[6]
+        B, V, sumN, sumF, sumM, sumJ, _, sumS2 = self.sizes[:8]
+        for name, tsr, n in (("dq", self.dq, 8 * sumJ), ("dT", self.dT, 3 * sumM), ("dA", self.dA, 3 * sumM),
+                             ("colors", self.colors, 3 * sumN), ("ext", self.ext, 12 * V),
+                             ("intr", self.intr, 9 * V), ("target", self.target, 3 * npix), ("dt", self.dt, sumS2),
+                             ("crops", self.crops, 7 * V)):
+            if tsr.numel() != n:
+                raise ValueError(f"{name}: {tsr.numel()} elements, the pipeline indexes {n}")
         self.face = torch.empty(npix, dtype=torch.int32, device=d)
         self.bary = torch.empty((npix, 2), device=d)
         self.render = torch.empty((npix, 3), device=d)

@@ -1,0 +1,67 @@
+"""Host-side checks of the batched pipeline's inputs (no GPU needed)."""
+import numpy as np
+import pytest
+
+from deepcharacters_amd import pipeline
+from oracle import oracle as orc
+
+
+def _cpu_render(wk, dq, dT, dA):
+    ch = wk.ch
+    M = ch.graph.num_nodes
+    zM = np.zeros((1, M, 3))
+    v = orc.ed_fwd(ch.graph, ch.verts, dT[None], dA[None], zM, zM, np.zeros((1, ch.N, 3)), dtype=np.float32)[0]
+    pos, _ = orc.dqs_fwd(dq[None], ch.joint_w[None], ch.joint_idx, v[0], dtype=np.float32)
+    face, _, rend, _ = orc.raster_fwd(ch.faces, pos, wk.ext.reshape(1, -1), wk.intr.reshape(1, -1), wk.H, wk.W,
+                                      vcol=ch.colors[None])
+    return face[0], rend[0]
+
+
+@pytest.fixture(scope="module")
+def small_works():
+    ws = pipeline.build_works(2, verts_range=(1500, 2500), nodes_range=(40, 60), C=2, W=96, H=64, S=32, f=200.0,
+                              seed=7)
+    pipeline.make_targets(ws, _cpu_render, np.random.default_rng(0))
+    return ws
+
+
+def test_targets_have_square_dt_for_non_square_images(small_works):
+    """The kernels index each view's DT image as S x S bytes; a 96x64 image
+    must still give a [C, S, S] DT whose grid covers the whole image."""
+    for wk in small_works:
+        C = wk.ext.shape[0]
+        assert wk.dt.shape == (C, wk.S, wk.S) and wk.dt.dtype == np.uint8
+        step = wk.crops[0, 2]
+        assert step * wk.S >= max(wk.W, wk.H)
+        assert (wk.dt == 127).any()
+    pipeline.check_works(small_works)
+
+
+def test_check_works_rejects_mismatched_shapes(small_works):
+    wk = small_works[0]
+    good = wk.dt
+    try:
+        wk.dt = good[:, :-4, :]
+        with pytest.raises(ValueError, match="dt"):
+            pipeline.check_works(small_works)
+        wk.dt = good.astype(np.float32)
+        with pytest.raises(ValueError, match="uint8"):
+            pipeline.check_works(small_works)
+    finally:
+        wk.dt = good
+    tgt = wk.target
+    try:
+        wk.target = tgt[:, :, :-1]
+        with pytest.raises(ValueError, match="target"):
+            pipeline.check_works(small_works)
+    finally:
+        wk.target = tgt
+    pipeline.check_works(small_works)
+
+
+def test_cpu_step_runs_and_is_finite(small_works):
+    from oracle import pipeline_ref
+    for wk in small_works:
+        r = pipeline_ref.step(wk, 1e-4, 1.0)
+        assert np.isfinite(r["loss"]).all() and r["loss"][0] > 0
+        assert np.isfinite(r["g_dq"]).all() and np.abs(r["g_dq"]).max() > 0
