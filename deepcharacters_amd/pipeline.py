@@ -300,16 +300,22 @@ class PlainPipeline:
                 target=t(wk.target[None]), dt=t(wk.dt[None], torch.uint8), crops=t(wk.crops[None]),
                 ws=renderer.workspace(topo, cfg, 1, d)))
 
-    def step(self):
-        """Forward + backward; returns per-character (loss, g_dq, g_dT, g_dA, g_col)."""
+    def step(self, positions=None):
+        """Forward + backward; returns per-character (loss, g_dq, g_dT, g_dA, g_col, face, render, pos).
+
+        positions: optional per-character skinned positions [N,3] to evaluate
+        the raster / loss / adjoint chain at instead of the skinning output
+        (the ED and skinning adjoints still linearise at the current pose)."""
         from . import embedded_graph as edm
         from . import renderer, silhouette, skinning
         out = []
-        for it in self.items:
+        for b, it in enumerate(self.items):
             # forward
             v_nr, _, _, _, a_out, s_out = edm.embedded_graph_gpu(it["g"], it["dT"], it["dA"], it["zeroM"],
                                                                  it["zeroM"], it["zeroN"])
             pos, _, _, _ = skinning.dq_skinning_gpu(it["dq"], it["jw"], it["jidx"], v_nr[0])
+            if positions is not None:
+                pos = positions[b].reshape(pos.shape).contiguous()
             bary, face, render, vnorm = renderer.render_forward(it["topo"], it["cfg"], pos, it["col"], None, None,
                                                                 it["ext"], it["intr"], ws=it["ws"])
             nrm = vnorm[:, 0] / vnorm[:, 0].norm(dim=-1, keepdim=True).clamp_min(1e-30)
@@ -332,7 +338,7 @@ class PlainPipeline:
             g_dq, _, g_rest = skinning.dq_skinning_backward(g_pos, it["dq"], it["jw"], it["jidx"], v_nr[0],
                                                             need_w=False, need_rest=True)
             gT, gA, _, _, _ = edm.embedded_graph_gpu_grad(it["g"], g_rest, None, a_out, s_out)
-            out.append((torch.stack([loss_col, loss_sil]), g_dq[0], gT[0], gA[0], g_col[0], face, render))
+            out.append((torch.stack([loss_col, loss_sil]), g_dq[0], gT[0], gA[0], g_col[0], face, render, pos[0]))
         return out
 
 

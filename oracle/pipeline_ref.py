@@ -13,8 +13,12 @@ import numpy as np
 from . import oracle as orc
 
 
-def step(wk, w_col: float, w_sil: float, dtype=np.float64):
-    """Returns dict(loss=[colour, silhouette], g_dq, g_dT, g_dA, g_col, face, render)."""
+def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None):
+    """Returns dict(loss=[colour, silhouette], g_dq, g_dT, g_dA, g_col, face, render, pos).
+
+    positions: optional skinned positions [N,3] at which the raster, losses
+    and their adjoints are evaluated instead of the oracle's own skinning
+    output (so that discrete decisions match a given producer exactly)."""
     ch = wk.ch
     g = ch.graph
     M, N = g.num_nodes, ch.N
@@ -26,6 +30,8 @@ def step(wk, w_col: float, w_sil: float, dtype=np.float64):
     # decisions (tri ids, boundary flags, nearest points) see the same inputs
     v_nr = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], zM, zM, np.zeros((1, N, 3)), dtype=np.float32)[0]
     pos, _ = orc.dqs_fwd(wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=np.float32)
+    if positions is not None:
+        pos = np.asarray(positions, np.float32).reshape(pos.shape)
     face, bary, render, vn = orc.raster_fwd(ch.faces, pos.astype(np.float32), ext, intr, wk.H, wk.W,
                                             vcol=ch.colors[None], dtype=np.float32)
     nrm = vn[:, 0].astype(np.float64)
@@ -47,4 +53,4 @@ def step(wk, w_col: float, w_sil: float, dtype=np.float64):
     g_dq, _, g_rest = orc.dqs_bwd(g_pos, wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=dtype)
     gT, gA, _, _, _ = orc.ed_bwd(g, ch.verts, g_rest, wk.dA[None], zM, dtype=dtype)
     return dict(loss=np.array([loss_col, loss_sil]), g_dq=g_dq[0], g_dT=gT[0], g_dA=gA[0], g_col=g_col[0],
-                face=face[0], render=render[0])
+                face=face[0], render=render[0], pos=pos[0])

@@ -44,22 +44,62 @@ def test_batched_raster_matches_op_renderer_bit_exact(works):
         p0 += n
 
 
-def test_batched_matches_plain_losses_and_gradients(works):
+def _split(bp, works):
+    j0 = m0 = v0 = 0
+    for b, wk in enumerate(works):
+        J, M, N = wk.ch.J, wk.ch.graph.num_nodes, wk.ch.N
+        yield b, wk, dict(loss=bp.loss[b], g_dq=bp.g_dq[j0:j0 + J], g_dT=bp.g_dT[m0:m0 + M],
+                          g_dA=bp.g_dA[m0:m0 + M], g_col=bp.g_colors[v0:v0 + N], pos=bp.vertices[v0:v0 + N])
+        j0 += J; m0 += M; v0 += N
+
+
+def test_batched_vertices_match_plain_ops(works):
+    """Fused ED + DQ skinning vs EmbeddedGraphGpu -> DqSkinningGpu (float rounding only)."""
     bp = pipeline.BatchedPipeline(works)
     bp.step()
     plain = pipeline.PlainPipeline(works).step()
     torch.cuda.synchronize()
-    j0 = m0 = v0 = 0
-    for b, (wk, res) in enumerate(zip(works, plain)):
-        loss, g_dq, gT, gA, g_col, face, _ = res
-        J, M, N = wk.ch.J, wk.ch.graph.num_nodes, wk.ch.N
-        assert _rel(bp.loss[b], loss) < 1e-3
-        assert _rel(bp.g_dq[j0:j0 + J], g_dq) < 2e-3
-        assert _rel(bp.g_dT[m0:m0 + M], gT) < 2e-3
-        assert _rel(bp.g_dA[m0:m0 + M], gA) < 2e-3
-        assert _rel(bp.g_colors[v0:v0 + N], g_col) < 2e-3
-        j0 += J; m0 += M; v0 += N
+    for (b, wk, got), res in zip(_split(bp, works), plain):
+        assert _rel(got["pos"], res[7]) < 1e-6
+
+
+def test_batched_matches_plain_at_the_same_vertices(works):
+    """With identical skinned vertices every discrete decision (tri-ids, L1
+    signs, boundary flags, nearest points) is shared, so the fused step and
+    the op-by-op chain agree up to float summation order."""
+    bp = pipeline.BatchedPipeline(works)
+    bp.step()
+    torch.cuda.synchronize()
+    pos = [got["pos"].clone() for _, _, got in _split(bp, works)]
+    plain = pipeline.PlainPipeline(works).step(positions=pos)
+    torch.cuda.synchronize()
+    p0 = 0
+    for (b, wk, got), res in zip(_split(bp, works), plain):
+        loss, g_dq, gT, gA, g_col, face, render, _ = res
+        n = wk.ext.shape[0] * wk.W * wk.H
+        assert torch.equal(bp.face[p0:p0 + n], face.reshape(-1))
+        assert torch.equal(bp.render[p0:p0 + n], render.reshape(-1, 3))
+        p0 += n
+        assert _rel(got["loss"], loss) < 1e-5
+        assert _rel(got["g_col"], g_col) < 1e-4
+        for key, ref in (("g_dq", g_dq), ("g_dT", gT), ("g_dA", gA)):
+            assert _rel(got[key], ref) < 1e-4, key
     assert float(bp.loss[:, 1].sum()) > 0 and float(bp.loss[:, 0].sum()) > 0
+
+
+def test_batched_matches_plain_end_to_end(works):
+    """Each path with its own vertices: the L1 colour term is discontinuous,
+    so ulp-level vertex differences move a few pixels' tri-ids or signs and
+    the gradients agree only to the size of those flips."""
+    bp = pipeline.BatchedPipeline(works)
+    bp.step()
+    plain = pipeline.PlainPipeline(works).step()
+    torch.cuda.synchronize()
+    for (b, wk, got), res in zip(_split(bp, works), plain):
+        loss, g_dq, gT, gA, g_col = res[:5]
+        assert _rel(got["loss"], loss) < 1e-3
+        for key, ref in (("g_dq", g_dq), ("g_dT", gT), ("g_dA", gA), ("g_col", g_col)):
+            assert _rel(got[key], ref) < 3e-2, key
 
 
 def test_graph_replay_matches_eager(works):
@@ -85,18 +125,18 @@ def test_graph_replay_matches_eager(works):
 
 def test_batched_matches_cpu_oracle_step(works):
     """The fused GPU step against the oracle's composition of the reference
-    ops (FP64 where it matters) on the same inputs."""
+    ops: the forward vertices against the oracle's ED + DQS, then the rest of
+    the step evaluated by the oracle at the GPU's vertices (FP64 adjoints)."""
     from oracle import pipeline_ref
     bp = pipeline.BatchedPipeline(works)
     bp.step()
     torch.cuda.synchronize()
-    j0 = m0 = v0 = 0
-    for b, wk in enumerate(works):
-        ref = pipeline_ref.step(wk, bp.w_color, bp.w_sil)
-        J, M, N = wk.ch.J, wk.ch.graph.num_nodes, wk.ch.N
+    for b, wk, got in _split(bp, works):
+        own = pipeline_ref.step(wk, bp.w_color, bp.w_sil)
+        assert _rel(got["pos"], torch.as_tensor(own["pos"])) < 1e-6
+        ref = pipeline_ref.step(wk, bp.w_color, bp.w_sil, positions=got["pos"].cpu().numpy())
         t = lambda a: torch.as_tensor(a)
-        assert _rel(bp.loss[b], t(ref["loss"])) < 1e-3
-        for got, key in ((bp.g_dq[j0:j0 + J], "g_dq"), (bp.g_dT[m0:m0 + M], "g_dT"), (bp.g_dA[m0:m0 + M], "g_dA"),
-                         (bp.g_colors[v0:v0 + N], "g_col")):
-            assert _rel(got, t(ref[key])) < 5e-3, key
-        j0 += J; m0 += M; v0 += N
+        assert _rel(got["loss"], t(ref["loss"])) < 1e-5
+        assert _rel(got["g_col"], t(ref["g_col"])) < 1e-4
+        for key in ("g_dq", "g_dT", "g_dA"):
+            assert _rel(got[key], t(ref[key])) < 1e-4, key
