@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the character batch is split across GPUs by assigning whole "
@@ -151,6 +153,16 @@ def main():
     pipeline.make_targets(works, pipeline.gpu_pose_renderer(), np.random.default_rng(2))
     torch.cuda.synchronize()
     log("targets rendered")
+
+    if args.copy_calibration_mb > 0:
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        src = torch.rand(n, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration_mb} MiB read + written")
+        del src, dst
 
     bp = pipeline.BatchedPipeline(works)
     stream = torch.cuda.Stream()

@@ -4,29 +4,34 @@
 //
 // Per step (all characters at once, indexed through per-character and
 // per-view descriptor tables):
-//   1 clear            zero accumulators and tile counters (fill kernel, no memset)
-//   2 node_fwd         per ED node: Euler delta rotation -> 15-float node row
-//   3 vertex_fwd       per vertex: ED inner sum (K_ed nodes) then DQ skinning
-//   4 normals          per vertex: normalised face-normal sum of the skinned mesh
-//   5 project          per (view, vertex): image position / depth and the
-//                      image-space normal direction (CameraProjection point +
-//                      vector modes)
-//   6 tri_setup_count  per (view, face): triangle record, tile counts
-//   7 scan, 8 scatter  tile lists
-//   9 fine             per 32x8 tile: LDS depth (64-bit ds_min on packed
-//                      (depth, face) keys), resolve, write face / barycentric /
-//                      colour, L1 colour loss vs the target image and, fused,
-//                      the rasterizer backward for that loss (barycentric
-//                      chain rule + target-image model-to-data term) with
-//                      wave-segmented atomics
-//  10 boundary         per (view, vertex): silhouette-boundary flags
-//  11 sil_point        model->data DT residual, squared loss, its gradient
-//  12 sil grid + sil_pixel   data->model residual via the binned nearest
-//                      boundary-point search, squared loss, gradient
-//  13 vertex_bwd       per vertex: camera-projection adjoint over the views,
-//                      DQ-skinning adjoint (joint-DQ gradients reduced in LDS),
-//                      gradient w.r.t. the ED-deformed position
-//  14 node_bwd         one wavefront per ED node: translation / Euler gradients
+//   clear            zero the tile / grid counters and the fallback accumulators
+//                    (fill kernel, no memset, so the sequence is graph-safe)
+//   node_fwd         per ED node: Euler delta rotation -> 15-float node row
+//   vertex_fwd       per vertex: ED inner sum (4 nodes) then DQ skinning
+//   normals          per vertex: normalised face-normal sum of the skinned mesh
+//   project          per (view, vertex): image position / depth and the
+//                    image-space normal direction (CameraProjection point +
+//                    vector modes)
+//   tri_count        per (view, face): triangle record, tiles touched
+//   tile_scan, tri_scan, bin_scatter   per-tile triangle lists and, per
+//                    triangle, the slots of its (triangle, tile) records
+//   fine             per 32x8 tile: LDS depth (64-bit ds_min on packed
+//                    (depth, face) keys); per pixel: face / barycentric /
+//                    colour outputs, L1 colour loss vs the target image and
+//                    its gradient (barycentric chain rule + the target-image
+//                    model-to-data term), kept in LDS; then the thread that
+//                    rasterised a triangle sums the pixels it won and writes
+//                    one (triangle, tile) gradient record -- no atomics
+//   boundary         per (view, vertex): silhouette-boundary flags
+//   sil_point        model->data DT residual, squared loss, its gradient
+//   sil grid + sil_pixel   data->model residual via the binned nearest
+//                    boundary-point search, squared loss, gradient
+//   vertex_bwd       per vertex: gather of the (triangle, tile) records of its
+//                    faces in every view, camera-projection adjoint, DQ-skinning
+//                    adjoint, colour gradient
+//   node_bwd         one wavefront per ED node: translation / Euler gradients
+//   finalize         per character: losses and joint-DQ gradients from the
+//                    per-block partial sums (fixed order: deterministic)
 #include <algorithm>
 #include <vector>
 
@@ -34,17 +39,16 @@
 #include "skin_common.hpp"
 
 namespace dcx {
-
-dcx_status exclusive_scan(const unsigned int* in, unsigned int* out, unsigned int* aux, long long n,
-                          hipStream_t s);
-
 namespace {
 
 constexpr int kB = 256;
-constexpr int KS = 4;   // skinning joints per vertex
-constexpr int KE = 4;   // ED nodes per vertex
+constexpr int KS = 4;         // skinning joints per vertex
+constexpr int KE = 4;         // ED nodes per vertex
 constexpr int kNodeRow = 15;  // Rd(9) g(3) td(3)
 constexpr int kImgFs = 2;     // image_filter_size of the model-to-data term
+constexpr int kGrad = 18;     // per (triangle, tile) record: 3 vertices x (pos 3 + colour 3)
+constexpr int kApronW = kTileW + 2 * kImgFs, kApronH = kTileH + 2 * kImgFs;
+constexpr int kXcd = 8;       // L2 domains the workgroups are dealt round-robin to
 
 struct CharD {
   int N, F, M, J;
@@ -61,11 +65,11 @@ struct ViewD {
 
 struct Dev {
   int B, V;
-  int maxN, maxF, maxM, maxS2;
+  int maxN, maxF, maxM, maxS2, maxJ, maxTiles;
+  int gxN, gxS;           // blocks per (view|char) of the vertex and DT-pixel kernels
   long long totVN, totVF, totPix, totTiles, totCells;
   const CharD* ch;
   const ViewD* vw;
-  const long long* tile0;  // [V+1] for tile -> view lookup
   // static mesh data (global arrays, local indices)
   const float* rest;      // [sumN,3]
   const int* faces;       // [sumF,3] local vertex ids
@@ -78,7 +82,7 @@ struct Dev {
   const int* n2v_vtx;     // local vertex ids
   const float* n2v_w;
   const int* vf_off;      // [sumN+1] global offsets
-  const int* vf_list;     // local face ids
+  const int* vf_list;     // local face ids, increasing per vertex
   // scratch
   float* node_tab;        // [sumM,15]
   float* vdef;            // [sumN,3] skinned vertices
@@ -86,21 +90,28 @@ struct Dev {
   f4* proj;               // [totVN]
   f2* nrm2;               // [totVN]
   TriRec* tri;            // [totVF]
+  unsigned int* tri_cnt;  // [totVF] tiles touched per (view, face)
+  unsigned int* tri_base; // [totVF+1] first record slot per (view, face)
   unsigned int* tile_cnt; // [totTiles]
   unsigned int* tile_off; // [totTiles+1]
   unsigned int* tile_fill;
-  unsigned int* scan_aux;
-  unsigned int* bins;
+  unsigned int* bins;     // [capacity] tile-major triangle lists
+  float* rec;             // [capacity,18] (triangle, tile) gradient records
+  unsigned char* rec_vis; // [capacity] 1 if the triangle won a pixel of the tile
   unsigned int capacity;
   unsigned char* isb;     // [totVN]
   unsigned int* cell_cnt; // [totCells]
   unsigned int* cell_off; // [totCells+1]
   unsigned int* cell_fill;
   unsigned int* cell_items;  // [totVN]
-  unsigned int* scan_aux2;
-  float* g_v;             // [sumN,3] position gradient accumulator
+  unsigned int* scan_aux;
+  float* g_v;             // [sumN,3] position gradient from records beyond capacity
   float* g_pts;           // [totVN,2]
   float* u;               // [sumN,3] gradient w.r.t. the ED-deformed position
+  float* loss_tile;       // [totTiles] colour loss per tile
+  float* loss_pt;         // [V*gxN] silhouette model->data loss per block
+  float* loss_px;         // [V*gxS] silhouette data->model loss per block
+  float* dq_part;         // [B*gxN, maxJ*8] joint-DQ gradient per vertex block
 };
 
 struct IO {
@@ -126,8 +137,8 @@ struct IO {
 };
 
 struct ClearList {
-  unsigned int* p[10];
-  long long n[10];
+  unsigned int* p[8];
+  long long n[8];
   int count;
 };
 
@@ -136,6 +147,83 @@ __global__ __launch_bounds__(kB) void clear_kernel(ClearList L) {
   for (int k = 0; k < L.count; ++k)
     for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < L.n[k]; i += stride) L.p[k][i] = 0u;
 }
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of any length: 1024-element block scans, a recursive scan of
+// the block totals, add-back; out[n] receives the total.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned int block_scan256(unsigned int v, unsigned int* s_wave, unsigned int& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned int inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned int o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  unsigned int base = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < kB / 64; ++w) {
+    if (w < wid) base += s_wave[w];
+    total += s_wave[w];
+  }
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(kB) void scan1024_kernel(const unsigned int* in, unsigned int* out,
+                                                      unsigned int* sums, long long n) {
+  __shared__ unsigned int s_wave[kB / 64];
+  const long long base = (long long)blockIdx.x * 1024 + threadIdx.x * 4;
+  unsigned int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { v[k] = base + k < n ? in[base + k] : 0u; s += v[k]; }
+  unsigned int total;
+  unsigned int ex = block_scan256(s, s_wave, total);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kB) void scan_add_kernel(unsigned int* out, const unsigned int* sums, long long n) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i < n) out[i] += sums[i >> 10];
+}
+
+__global__ void copy1_kernel(const unsigned int* src, unsigned int* dst) { *dst = *src; }
+
+// Scratch elements scan_big needs for n inputs (block totals of every level).
+long long scan_aux_elems(long long n) {
+  long long t = 0;
+  while (n > 1) {
+    n = (n + 1023) / 1024;
+    t += n + 1;
+  }
+  return t + 2;
+}
+
+dcx_status scan_big(const unsigned int* in, unsigned int* out, unsigned int* aux, long long n, hipStream_t s) {
+  const long long nb = (n + 1023) / 1024;
+  hipLaunchKernelGGL(scan1024_kernel, dim3((unsigned)nb), dim3(kB), 0, s, in, out, aux, n);
+  if (nb == 1) {
+    hipLaunchKernelGGL(copy1_kernel, dim3(1), dim3(1), 0, s, aux, out + n);
+    return check_launch("scan");
+  }
+  dcx_status st = scan_big(aux, aux, aux + nb + 1, nb, s);  // aux[0..nb) offsets, aux[nb] total
+  if (st) return st;
+  hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(n, kB)), dim3(kB), 0, s, out, aux, n);
+  hipLaunchKernelGGL(copy1_kernel, dim3(1), dim3(1), 0, s, aux + nb, out + n);
+  return check_launch("scan");
+}
+
+// ---------------------------------------------------------------------------
+// Forward: ED, skinning, normals, projection
+// ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kB) void node_fwd_kernel(Dev d, IO io) {
   const CharD c = d.ch[blockIdx.y];
@@ -231,6 +319,19 @@ __global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
   d.nrm2[vw.vn0 + v] = l > 0.f ? mk2(du / l, dv / l) : mk2(0.f, 0.f);
 }
 
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+
+struct TileRange { int tx0, ty0, ntx, nty; };
+__device__ __forceinline__ TileRange tile_range(const TriRec& t) {
+  TileRange r;
+  const int x0 = t.bx0_by0 & 0xffff, y0 = t.bx0_by0 >> 16, x1 = t.bx1_by1 & 0xffff, y1 = t.bx1_by1 >> 16;
+  r.tx0 = x0 / kTileW; r.ty0 = y0 / kTileH;
+  r.ntx = x1 / kTileW - r.tx0 + 1; r.nty = y1 / kTileH - r.ty0 + 1;
+  return r;
+}
+
 __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
@@ -240,18 +341,20 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
   const f4* pv = d.proj + vw.vn0;
   TriRec t;
-  TriRec* dst = d.tri + vw.vf0 + f;
+  const long long ri = vw.vf0 + f;
   if (!setup_tri(pv[fv[0]], pv[fv[1]], pv[fv[2]], f, vw.W, vw.H, t)) {
     t.bx0_by0 = 0x7fff7fff;
     t.bx1_by1 = 0;
-    *dst = t;
+    d.tri[ri] = t;
+    d.tri_cnt[ri] = 0u;
     return;
   }
-  *dst = t;
-  const int x0 = t.bx0_by0 & 0xffff, y0 = t.bx0_by0 >> 16, x1 = t.bx1_by1 & 0xffff, y1 = t.bx1_by1 >> 16;
+  d.tri[ri] = t;
+  const TileRange r = tile_range(t);
+  d.tri_cnt[ri] = (unsigned int)(r.ntx * r.nty);
   unsigned int* cnt = d.tile_cnt + vw.tile0;
-  for (int ty = y0 / kTileH; ty <= y1 / kTileH; ++ty)
-    for (int tx = x0 / kTileW; tx <= x1 / kTileW; ++tx) atomicAdd(cnt + ty * vw.tiles_x + tx, 1u);
+  for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
+    for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) atomicAdd(cnt + ty * vw.tiles_x + tx, 1u);
 }
 
 __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
@@ -263,14 +366,18 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
   const long long ri = vw.vf0 + f;
   const TriRec& t = d.tri[ri];
   if (t.bx0_by0 == 0x7fff7fff) return;
-  const int x0 = t.bx0_by0 & 0xffff, y0 = t.bx0_by0 >> 16, x1 = t.bx1_by1 & 0xffff, y1 = t.bx1_by1 >> 16;
-  for (int ty = y0 / kTileH; ty <= y1 / kTileH; ++ty)
-    for (int tx = x0 / kTileW; tx <= x1 / kTileW; ++tx) {
+  const TileRange r = tile_range(t);
+  for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
+    for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) {
       const long long tile = vw.tile0 + ty * vw.tiles_x + tx;
       const unsigned int slot = d.tile_off[tile] + atomicAdd(d.tile_fill + tile, 1u);
       if (slot < d.capacity) d.bins[slot] = (unsigned int)ri;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Fine tile kernel
+// ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, unsigned long long* s_key) {
   const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
@@ -291,7 +398,9 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
   }
 }
 
-// Barycentric VJP (same derivation as the standalone backward).
+// Barycentric VJP: gradient of (b0, b1) (third weight 1 - b0 - b1) w.r.t. the
+// three world-space vertices through the projection (same derivation as the
+// standalone rasterizer backward).
 __device__ void bary_vjp_p(const float* E, const float* K, const f3 X[3], float px, float py, const float gb_in[3],
                            f3 out[3]) {
   float h2[3], u[3], v[3], iz[3];
@@ -342,29 +451,6 @@ __device__ void bary_vjp_p(const float* E, const float* K, const f3 X[3], float 
   }
 }
 
-template <int NV>
-__device__ __forceinline__ bool seg_reduce_p(float (&v)[NV], int key) {
-  const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1, 64);
-  const int next = __shfl_down(key, 1, 64);
-  int start = (lane == 0 || prev != key) ? lane : 0;
-#pragma unroll
-  for (int dd = 1; dd < 64; dd <<= 1) {
-    const int o = __shfl_up(start, dd, 64);
-    if (lane >= dd) start = max(start, o);
-  }
-#pragma unroll
-  for (int dd = 1; dd < 64; dd <<= 1) {
-    const bool take = lane - dd >= start;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      const float o = __shfl_up(v[j], dd, 64);
-      if (take) v[j] += o;
-    }
-  }
-  return lane == 63 || next != key;
-}
-
 __device__ __forceinline__ float block_sum(float v, float* s_red) {
   v = wave_sum(v);
   __syncthreads();
@@ -375,46 +461,62 @@ __device__ __forceinline__ float block_sum(float v, float* s_red) {
   return t;
 }
 
+// Workgroup id -> linear tile id such that each L2 domain (workgroups are
+// dealt to the XCDs round-robin) gets a contiguous run of tiles, so triangle
+// records shared by neighbouring tiles are fetched into one L2.
+// Bijective on [0, n): domain x receives the q + (x < r) workgroups with
+// wg % kXcd == x (q = n / kXcd, r = n % kXcd) and maps them, in order, onto
+// the run starting at x * q + min(x, r).
+__device__ __forceinline__ long long xcd_tile(long long wg, long long n) {
+  const long long q = n / kXcd, r = n % kXcd;
+  const long long x = wg % kXcd, i = wg / kXcd;
+  return x * q + (x < r ? x : r) + i;
+}
+
 __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
   __shared__ unsigned long long s_key[kTilePix];
-  __shared__ int s_view;
+  __shared__ float s_g[kGrad][kTilePix];
+  __shared__ float s_tgt[3][kApronH * kApronW];
   __shared__ float s_red[kTilePix / 64];
-  const long long tile = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int lo = 0, hi = d.V;  // tile0[lo] <= tile < tile0[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (d.tile0[mid] <= tile) lo = mid; else hi = mid;
-    }
-    s_view = lo;
-  }
-  s_key[threadIdx.x] = kEmptyKey;
-  __syncthreads();
-  const int w = s_view;
+  const long long lin = xcd_tile(blockIdx.x, (long long)gridDim.x);
+  if (lin >= (long long)d.V * d.maxTiles) return;
+  const int w = (int)(lin / d.maxTiles);
+  const int lt = (int)(lin - (long long)w * d.maxTiles);
   const ViewD vw = d.vw[w];
+  if (lt >= vw.tiles) return;
   const CharD c = d.ch[vw.ch];
-  const int lt = (int)(tile - vw.tile0);
+  const long long tile = vw.tile0 + lt;
   const int tx0 = (lt % vw.tiles_x) * kTileW, ty0 = (lt / vw.tiles_x) * kTileH;
   const unsigned int off = d.tile_off[tile], end = d.tile_off[tile + 1];
-  if (end <= d.capacity) {
-    for (unsigned int k = off + threadIdx.x; k < end; k += kTilePix) raster_in_tile(d.tri[d.bins[k]], tx0, ty0, s_key);
-  } else {
-    const TriRec* tv = d.tri + vw.vf0;
-    for (int f = threadIdx.x; f < c.F; f += kTilePix) raster_in_tile(tv[f], tx0, ty0, s_key);
+  const bool overflow = end > d.capacity;
+  const unsigned int n_items = overflow ? (unsigned int)c.F : end - off;
+  const TriRec* tv = d.tri + vw.vf0;
+
+  s_key[threadIdx.x] = kEmptyKey;
+  // target colours of the tile plus a kImgFs apron (clamped reads; the apron is
+  // only used by pixels whose whole filter window lies inside the image)
+  for (int i = threadIdx.x; i < kApronH * kApronW; i += kTilePix) {
+    const int ax = i % kApronW, ay = i / kApronW;
+    const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
+    const f3 t = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
+    s_tgt[0][i] = t.x; s_tgt[1][i] = t.y; s_tgt[2][i] = t.z;
   }
   __syncthreads();
-  const int x = tx0 + (threadIdx.x % kTileW), y = ty0 + (threadIdx.x / kTileW);
+  for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix)
+    raster_in_tile(overflow ? tv[k] : d.tri[d.bins[off + k]], tx0, ty0, s_key);
+  __syncthreads();
+
+  // ---- per pixel: outputs, colour loss, gradient contributions -> LDS
+  const int px_i = threadIdx.x % kTileW, py_i = threadIdx.x / kTileW;
+  const int x = tx0 + px_i, y = ty0 + py_i;
   const bool inimg = x < vw.W && y < vw.H;
   const unsigned long long key = s_key[threadIdx.x];
   const int f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
-  float acc[18];
-#pragma unroll
-  for (int k = 0; k < 18; ++k) acc[k] = 0.f;
   float lsum = 0.f;
-  int i0 = 0, i1 = 0, i2 = 0;
   if (inimg) {
     const long long o = vw.pix0 + (long long)y * vw.W + x;
-    const f3 tgt = ld3(io.target + 3 * o);
+    const int ta = (py_i + kImgFs) * kApronW + (px_i + kImgFs);
+    const f3 tgt = mk3(s_tgt[0][ta], s_tgt[1][ta], s_tgt[2][ta]);
     if (f < 0) {
       io.face[o] = -1;
       io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
@@ -422,7 +524,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
     } else {
       const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
-      i0 = fv[0]; i1 = fv[1]; i2 = fv[2];
+      const int i0 = fv[0], i1 = fv[1], i2 = fv[2];
       const f4* pv = d.proj + vw.vn0;
       const float px = (float)x + 0.5f, py = (float)y + 0.5f;
       const f2 ab = native_bary(pv[i0], pv[i1], pv[i2], px, py);
@@ -440,7 +542,11 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       const f3 g = mk3(io.w_col * sgn(df.x), io.w_col * sgn(df.y), io.w_col * sgn(df.z));
       // colour gradient and dL/dweights (vertexColor, shadeless)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) { acc[9 + 3 * k] = g.x * wgt[k]; acc[10 + 3 * k] = g.y * wgt[k]; acc[11 + 3 * k] = g.z * wgt[k]; }
+      for (int k = 0; k < 3; ++k) {
+        s_g[6 * k + 3][threadIdx.x] = g.x * wgt[k];
+        s_g[6 * k + 4][threadIdx.x] = g.y * wgt[k];
+        s_g[6 * k + 5][threadIdx.x] = g.z * wgt[k];
+      }
       const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
       const float* E = io.ext + (size_t)w * 12;
       const float* K = io.intr + (size_t)w * 9;
@@ -455,7 +561,8 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         float norm = 0.f;
         for (int yy = -fs; yy <= fs; ++yy)
           for (int xx = -fs; xx <= fs; ++xx) {
-            const f3 I = ld3(io.target + 3 * (vw.pix0 + (long long)(y + yy) * vw.W + (x + xx)));
+            const int a = (py_i + kImgFs + yy) * kApronW + (px_i + kImgFs + xx);
+            const f3 I = mk3(s_tgt[0][a], s_tgt[1][a], s_tgt[2][a]);
             const float den = (float)(xx * xx + yy * yy);
             float gu = 0.f, gvv = 0.f;
             if (den != 0.f) { gu = (float)xx / den; gvv = (float)yy / den; }
@@ -485,23 +592,71 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         }
       }
 #pragma unroll
-      for (int k = 0; k < 3; ++k) { acc[3 * k] = gx[k].x; acc[3 * k + 1] = gx[k].y; acc[3 * k + 2] = gx[k].z; }
+      for (int k = 0; k < 3; ++k) {
+        s_g[6 * k][threadIdx.x] = gx[k].x;
+        s_g[6 * k + 1][threadIdx.x] = gx[k].y;
+        s_g[6 * k + 2][threadIdx.x] = gx[k].z;
+      }
     }
   }
-  const bool tail = seg_reduce_p<18>(acc, f);
-  if (f >= 0 && tail) {
-    const int idx[3] = {i0, i1, i2};
+  const float lt_sum = block_sum(lsum, s_red);  // its barriers also publish s_g
+  if (threadIdx.x == 0) d.loss_tile[tile] = lt_sum;
+  // the winning face of each pixel, in the low half of its key slot
+  int* s_face = reinterpret_cast<int*>(s_key);
+  __syncthreads();
+  s_face[threadIdx.x] = f;
+  __syncthreads();
+
+  // ---- per (triangle, tile): the thread that rasterised the triangle sums
+  // the pixels it won (row-major order) into one record
+  const int tx = tx0 / kTileW, ty = ty0 / kTileH;
+  for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
+    const unsigned int ri_local = overflow ? k : d.bins[off + k] - (unsigned int)vw.vf0;
+    const TriRec t = tv[ri_local];
+    if (t.bx0_by0 == 0x7fff7fff) continue;
+    const int bx0 = max(t.bx0_by0 & 0xffff, tx0), by0 = max(t.bx0_by0 >> 16, ty0);
+    const int bx1 = min(t.bx1_by1 & 0xffff, tx0 + kTileW - 1), by1 = min(t.bx1_by1 >> 16, ty0 + kTileH - 1);
+    if (bx0 > bx1 || by0 > by1) continue;
+    const int face = t.face_flags & kFaceMask;
+    float a[kGrad];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float* gp = d.g_v + 3 * ((size_t)c.v0 + idx[k]);
-      atomicAdd(gp, acc[3 * k]); atomicAdd(gp + 1, acc[3 * k + 1]); atomicAdd(gp + 2, acc[3 * k + 2]);
-      float* gc = io.g_col + 3 * ((size_t)c.v0 + idx[k]);
-      atomicAdd(gc, acc[9 + 3 * k]); atomicAdd(gc + 1, acc[10 + 3 * k]); atomicAdd(gc + 2, acc[11 + 3 * k]);
+    for (int q = 0; q < kGrad; ++q) a[q] = 0.f;
+    bool won = false;
+    for (int yy = by0; yy <= by1; ++yy)
+      for (int xx = bx0; xx <= bx1; ++xx) {
+        const int p = (yy - ty0) * kTileW + (xx - tx0);
+        if (s_face[p] == face) {
+          won = true;
+#pragma unroll
+          for (int q = 0; q < kGrad; ++q) a[q] += s_g[q][p];
+        }
+      }
+    const TileRange r = tile_range(t);
+    const unsigned int e = d.tri_base[vw.vf0 + ri_local] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
+    if (e < d.capacity) {
+      d.rec_vis[e] = won ? 1 : 0;
+      if (won) {
+        float* dst = d.rec + (size_t)e * kGrad;
+#pragma unroll
+        for (int q = 0; q < kGrad; ++q) dst[q] = a[q];
+      }
+    } else if (won) {
+      // no record slot (the batch overflowed its capacity): accumulate directly
+      const int* fv = d.faces + 3 * ((size_t)c.f0 + face);
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        float* gp = d.g_v + 3 * ((size_t)c.v0 + fv[kk]);
+        atomicAdd(gp, a[6 * kk]); atomicAdd(gp + 1, a[6 * kk + 1]); atomicAdd(gp + 2, a[6 * kk + 2]);
+        float* gc = io.g_col + 3 * ((size_t)c.v0 + fv[kk]);
+        atomicAdd(gc, a[6 * kk + 3]); atomicAdd(gc + 1, a[6 * kk + 4]); atomicAdd(gc + 2, a[6 * kk + 5]);
+      }
     }
   }
-  const float t = block_sum(lsum, s_red);
-  if (threadIdx.x == 0 && t != 0.f) atomicAdd(io.loss + 2 * vw.ch, t);
 }
+
+// ---------------------------------------------------------------------------
+// Silhouette
+// ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kB) void boundary_kernel_p(Dev d, IO io) {
   const int w = blockIdx.y;
@@ -571,7 +726,7 @@ __global__ __launch_bounds__(kB) void sil_point_kernel_p(Dev d, IO io) {
     d.g_pts[2 * pi + 1] = gy;
   }
   const float t = block_sum(lsum, s_red);
-  if (threadIdx.x == 0 && t != 0.f) atomicAdd(io.loss + 2 * vw.ch + 1, t);
+  if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
 }
 
 struct GridGeomP { float ox, oy, cell; };
@@ -662,10 +817,14 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
     }
   }
   const float t = block_sum(lsum, s_red);
-  if (threadIdx.x == 0 && t != 0.f) atomicAdd(io.loss + 2 * vw.ch + 1, t);
+  if (threadIdx.x == 0) d.loss_px[(size_t)w * gridDim.x + blockIdx.x] = t;
 }
 
-// dL/dv (raster + projection adjoint over views) -> DQ-skinning adjoint.
+// ---------------------------------------------------------------------------
+// Backward: vertices, ED nodes, per-character reductions
+// ---------------------------------------------------------------------------
+
+// dL/dv (gathered raster records + projection adjoint over views) -> DQ-skinning adjoint.
 __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
   __shared__ float s_acc[256 * 8];
@@ -676,13 +835,38 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   if (v < c.N) {
     const int gv = c.v0 + v;
     f3 g = ld3(d.g_v + 3 * (size_t)gv);
+    f3 gc = ld3(io.g_col + 3 * (size_t)gv);
     const f3 X = ld3(d.vdef + 3 * (size_t)gv);
+    const int k0 = d.vf_off[gv], k1 = d.vf_off[gv + 1];
     for (int k = 0; k < c.C; ++k) {
       const int w = c.view0 + k;
+      const ViewD vw = d.vw[w];
+      // raster records of the incident faces in this view
+      int prev = -1;
+      for (int q = k0; q < k1; ++q) {
+        const int fl = d.vf_list[q];
+        if (fl == prev) continue;  // a face listing this vertex twice is handled once, all slots
+        prev = fl;
+        const int* fv = d.faces + 3 * ((size_t)c.f0 + fl);
+        const long long ri = vw.vf0 + fl;
+        const unsigned int e0 = d.tri_base[ri];
+        const unsigned int e1 = min(d.tri_base[ri + 1], d.capacity);
+        for (unsigned int e = e0; e < e1; ++e) {
+          if (!d.rec_vis[e]) continue;
+          const float* r = d.rec + (size_t)e * kGrad;
+#pragma unroll
+          for (int s = 0; s < 3; ++s)
+            if (fv[s] == v) {
+              g += ld3(r + 6 * s);
+              gc += ld3(r + 6 * s + 3);
+            }
+        }
+      }
+      // silhouette: camera-projection adjoint (CameraProjectionGPUOpGrad.cu:12-79)
+      const f2 gp = ((const f2*)d.g_pts)[vw.vn0 + v];
+      if (gp.x == 0.f && gp.y == 0.f) continue;
       const float* E = io.ext + (size_t)w * 12;
       const float* K = io.intr + (size_t)w * 9;
-      const f2 gp = ((const f2*)d.g_pts)[d.vw[w].vn0 + v];
-      if (gp.x == 0.f && gp.y == 0.f) continue;
       const float xc = E[0] * X.x + E[1] * X.y + E[2] * X.z + E[3];
       const float yc = E[4] * X.x + E[5] * X.y + E[6] * X.z + E[7];
       const float zc = E[8] * X.x + E[9] * X.y + E[10] * X.z + E[11];
@@ -692,6 +876,7 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
       const float g0 = gp.x * j00, g1 = gp.x * j01 + gp.y * j11, g2 = gp.x * j02 + gp.y * j12;
       g = g + mk3(E[0] * g0 + E[4] * g1 + E[8] * g2, E[1] * g0 + E[5] * g1 + E[9] * g2, E[2] * g0 + E[6] * g1 + E[10] * g2);
     }
+    io.g_col[3 * (size_t)gv] = gc.x; io.g_col[3 * (size_t)gv + 1] = gc.y; io.g_col[3 * (size_t)gv + 2] = gc.z;
     const f3 vnr = ed_inner(d, c, gv);
     int idx[KS];
     float w[KS];
@@ -726,8 +911,8 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
     d.u[3 * (size_t)gv] = u.x; d.u[3 * (size_t)gv + 1] = u.y; d.u[3 * (size_t)gv + 2] = u.z;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < c.J * 8; i += kB)
-    if (s_acc[i] != 0.f) atomicAdd(io.g_dq + (size_t)c.j0 * 8 + i, s_acc[i]);
+  float* part = d.dq_part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (size_t)d.maxJ * 8;
+  for (int i = threadIdx.x; i < c.J * 8; i += kB) part[i] = s_acc[i];
 }
 
 __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
@@ -759,6 +944,31 @@ __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
   }
 }
 
+// One block per character: losses and joint-DQ gradients from the per-block
+// partial sums, each summed in a fixed order.
+__global__ __launch_bounds__(kB) void finalize_kernel(Dev d, IO io) {
+  __shared__ float s_red[kB / 64];
+  const int b = blockIdx.x;
+  const CharD c = d.ch[b];
+  const ViewD v0 = d.vw[c.view0], v1 = d.vw[c.view0 + c.C - 1];
+  float s = 0.f;
+  for (long long t = v0.tile0 + threadIdx.x; t < v1.tile0 + v1.tiles; t += kB) s += d.loss_tile[t];
+  const float lc = block_sum(s, s_red);
+  s = 0.f;
+  for (long long i = (long long)c.view0 * d.gxN + threadIdx.x; i < (long long)(c.view0 + c.C) * d.gxN; i += kB)
+    s += d.loss_pt[i];
+  for (long long i = (long long)c.view0 * d.gxS + threadIdx.x; i < (long long)(c.view0 + c.C) * d.gxS; i += kB)
+    s += d.loss_px[i];
+  const float ls = block_sum(s, s_red);
+  if (threadIdx.x == 0) { io.loss[2 * b] = lc; io.loss[2 * b + 1] = ls; }
+  const int nblk = (c.N + kB - 1) / kB;
+  for (int i = threadIdx.x; i < c.J * 8; i += kB) {
+    float a = 0.f;
+    for (int k = 0; k < nblk; ++k) a += d.dq_part[((size_t)b * d.gxN + k) * (size_t)d.maxJ * 8 + i];
+    io.g_dq[(size_t)c.j0 * 8 + i] = a;
+  }
+}
+
 template <class T>
 hipError_t up(T** dst, const std::vector<T>& src) {
   hipError_t e = hipMalloc(dst, sizeof(T) * std::max<size_t>(src.size(), 1));
@@ -781,9 +991,9 @@ struct dcx_pipeline {
 };
 
 static void pipe_free(dcx_pipeline* p) {
-  if (p->exec) hipGraphExecDestroy(p->exec);
-  if (p->graph) hipGraphDestroy(p->graph);
-  for (void* a : p->allocs) hipFree(a);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (void* a : p->allocs) (void)hipFree(a);
   delete p;
 }
 
@@ -797,13 +1007,13 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   std::vector<float> rest, jw, ew;
   std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list;
   std::vector<float> n2v_w;
-  std::vector<long long> tile0;
   long long vn = 0, vf = 0, pix = 0, sil = 0, tiles = 0, cells = 0;
-  int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0;
+  int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0, maxJ = 0, maxTiles = 0;
   for (int b = 0; b < n_chars; ++b) {
     const dcx_char_desc& c = chars[b];
     if (!(c.num_vertices > 0 && c.num_faces > 0 && c.num_nodes > 0 && c.num_joints > 0 && c.num_joints <= 256 &&
-          c.num_cameras > 0 && c.width > 0 && c.height > 0 && c.dt_size > 2 && c.width <= 32767 && c.height <= 32767)) {
+          c.num_cameras > 0 && c.width > 0 && c.height > 0 && c.dt_size > 2 && c.dt_size <= 16384 &&
+          c.width <= 32767 && c.height <= 32767)) {
       delete p;
       return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: character %d has invalid sizes", b);
     }
@@ -846,7 +1056,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
         n2v_w[slot] = c.node_w[KE * v + k];
       }
     for (int m = 0; m < d.M; ++m) n2v_off.push_back(base + cnt[m + 1]);
-    // vertex -> face CSR
+    // vertex -> face CSR (faces ascending per vertex)
     std::vector<int> vc(d.N + 1, 0);
     for (int i = 0; i < 3 * d.F; ++i) vc[c.faces[i] + 1]++;
     for (int v = 0; v < d.N; ++v) vc[v + 1] += vc[v];
@@ -864,31 +1074,34 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
       v.vn0 = vn; v.vf0 = vf; v.pix0 = pix; v.sil0 = sil; v.tile0 = tiles; v.cell0 = cells;
       v.gx = c.dt_size / 30 + 6; v.gy = c.dt_size / 30 + 6;
       vn += d.N; vf += d.F; pix += (long long)c.width * c.height; sil += (long long)c.dt_size * c.dt_size;
-      tile0.push_back(tiles);
       tiles += v.tiles; cells += (long long)v.gx * v.gy;
+      maxTiles = std::max(maxTiles, v.tiles);
       vw.push_back(v);
     }
     p->sumN += d.N; p->sumF += d.F; p->sumM += d.M; p->sumJ += d.J;
-    maxN = std::max(maxN, d.N); maxF = std::max(maxF, d.F); maxM = std::max(maxM, d.M);
+    maxN = std::max(maxN, d.N); maxF = std::max(maxF, d.F); maxM = std::max(maxM, d.M); maxJ = std::max(maxJ, d.J);
     maxS2 = std::max(maxS2, c.dt_size * c.dt_size);
   }
-  tile0.push_back(tiles);
   p->V = (int)vw.size();
   p->sumS2 = sil;
   Dev& d = p->d;
-  d.B = n_chars; d.V = p->V; d.maxN = maxN; d.maxF = maxF; d.maxM = maxM; d.maxS2 = maxS2;
+  d.B = n_chars; d.V = p->V; d.maxN = maxN; d.maxF = maxF; d.maxM = maxM; d.maxS2 = maxS2; d.maxJ = maxJ;
+  d.maxTiles = maxTiles;
+  d.gxN = (int)ceil_div(maxN, kB);
+  d.gxS = (int)ceil_div(maxS2, kB);
   d.totVN = vn; d.totVF = vf; d.totPix = pix; d.totTiles = tiles; d.totCells = cells;
   const long long cap = bin_capacity > 0 ? bin_capacity : std::max<long long>(4 * vf, 1 << 20);
-  if (cap >= (1ll << 32) || vf >= (1ll << 32) || (tiles + 1023) / 1024 > 4096 || (cells + 1023) / 1024 > 4096) {
+  if (cap >= (1ll << 32) || vf >= (1ll << 31) || (long long)p->V * maxTiles >= (1ll << 31)) {
     delete p;
-    return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: batch too large (bins %lld, tiles %lld)", cap, tiles);
+    return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: batch too large (records %lld, tiles %lld)", cap, tiles);
   }
   d.capacity = (unsigned int)cap;
+  const long long aux = std::max(scan_aux_elems(tiles), std::max(scan_aux_elems(vf), scan_aux_elems(cells)));
   hipError_t e = hipSuccess;
   auto track = [&](void* a) { p->allocs.push_back(a); };
   auto alloc = [&](auto** dst, size_t bytes) {
     if (e != hipSuccess) return;
-    e = hipMalloc((void**)dst, std::max<size_t>(bytes, 1));
+    e = hipMalloc((void**)dst, std::max<size_t>(bytes, 4));
     if (e == hipSuccess) track((void*)*dst);
   };
   auto upload = [&](auto** dst, const auto& v) {
@@ -896,13 +1109,13 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     e = up(dst, v);
     if (e == hipSuccess) track((void*)*dst);
   };
-  CharD* dch; ViewD* dvw; long long* dt0;
+  CharD* dch; ViewD* dvw;
   float* drest; int *dfaces, *djidx, *denode, *dnv, *dn2o, *dn2v, *dvfo, *dvfl; float *djw, *dew, *dn2w;
-  upload(&dch, ch); upload(&dvw, vw); upload(&dt0, tile0);
+  upload(&dch, ch); upload(&dvw, vw);
   upload(&drest, rest); upload(&dfaces, faces); upload(&djidx, jidx); upload(&djw, jw);
   upload(&denode, enode); upload(&dew, ew); upload(&dnv, node_vtx); upload(&dn2o, n2v_off);
   upload(&dn2v, n2v_vtx); upload(&dn2w, n2v_w); upload(&dvfo, vf_off); upload(&dvfl, vf_list);
-  d.ch = dch; d.vw = dvw; d.tile0 = dt0; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
+  d.ch = dch; d.vw = dvw; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
   d.enode = denode; d.ew = dew; d.node_vtx = dnv; d.n2v_off = dn2o; d.n2v_vtx = dn2v; d.n2v_w = dn2w;
   d.vf_off = dvfo; d.vf_list = dvfl;
   alloc(&d.node_tab, sizeof(float) * kNodeRow * (size_t)p->sumM);
@@ -911,20 +1124,27 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.proj, sizeof(f4) * (size_t)vn);
   alloc(&d.nrm2, sizeof(f2) * (size_t)vn);
   alloc(&d.tri, sizeof(TriRec) * (size_t)vf);
+  alloc(&d.tri_cnt, 4 * (size_t)vf);
+  alloc(&d.tri_base, 4 * (size_t)(vf + 1));
   alloc(&d.tile_cnt, 4 * (size_t)tiles);
   alloc(&d.tile_off, 4 * (size_t)(tiles + 1));
   alloc(&d.tile_fill, 4 * (size_t)tiles);
-  alloc(&d.scan_aux, 4 * (size_t)((tiles + 1023) / 1024 + 1));
+  alloc(&d.scan_aux, 4 * (size_t)aux);
   alloc(&d.bins, 4 * (size_t)cap);
+  alloc(&d.rec, sizeof(float) * kGrad * (size_t)cap);
+  alloc(&d.rec_vis, (size_t)cap);
   alloc(&d.isb, (size_t)vn);
   alloc(&d.cell_cnt, 4 * (size_t)cells);
   alloc(&d.cell_off, 4 * (size_t)(cells + 1));
   alloc(&d.cell_fill, 4 * (size_t)cells);
   alloc(&d.cell_items, 4 * (size_t)vn);
-  alloc(&d.scan_aux2, 4 * (size_t)((cells + 1023) / 1024 + 1));
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
   alloc(&d.u, sizeof(float) * 3 * (size_t)p->sumN);
+  alloc(&d.loss_tile, sizeof(float) * (size_t)tiles);
+  alloc(&d.loss_pt, sizeof(float) * (size_t)p->V * d.gxN);
+  alloc(&d.loss_px, sizeof(float) * (size_t)p->V * d.gxS);
+  alloc(&d.dq_part, sizeof(float) * (size_t)n_chars * d.gxN * maxJ * 8);
   if (e != hipSuccess) {
     pipe_free(p);
     return set_error(DCX_ERR_OOM, "dcx_pipeline_create: %s", hipGetErrorString(e));
@@ -947,9 +1167,10 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 
 // Launch order of one step; dcx_pipeline_profile times each entry.
 static const char* const kStageNames[] = {
-    "clear",         "node_fwd",      "vertex_fwd",   "normals",        "project",       "tri_count",
-    "tile_scan",     "bin_scatter",   "fine",         "boundary",       "sil_point",     "sil_grid_count",
-    "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "vertex_bwd",     "node_bwd"};
+    "clear",          "node_fwd",      "vertex_fwd",       "normals",   "project",    "tri_count",
+    "tile_scan",      "tri_scan",      "bin_scatter",      "fine",      "boundary",   "sil_point",
+    "sil_grid_count", "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "vertex_bwd", "node_bwd",
+    "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
@@ -968,17 +1189,15 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   auto add = [&](void* ptr, long long n) { L.p[L.count] = (unsigned int*)ptr; L.n[L.count] = n; L.count++; };
   add(d.g_v, 3ll * p->sumN);
   add(in->g_colors, 3ll * p->sumN);
-  add(in->g_dq, 8ll * p->sumJ);
-  add(in->loss, 2ll * p->B);
   add(d.tile_cnt, d.totTiles);
   add(d.tile_fill, d.totTiles);
   add(d.cell_cnt, d.totCells);
   add(d.cell_fill, d.totCells);
   int stage = 0;
-  auto mark = [&]() { if (ev) hipEventRecord(ev[stage], s); stage++; };
+  auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
+  const int B = p->B, V = p->V;
   mark();
   hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, s, L);
-  const int B = p->B, V = p->V;
   mark();
   hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
   mark();
@@ -990,29 +1209,34 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
-  dcx_status st = exclusive_scan(d.tile_cnt, d.tile_off, d.scan_aux, d.totTiles, s);
+  dcx_status st = scan_big(d.tile_cnt, d.tile_off, d.scan_aux, d.totTiles, s);
+  if (st) return st;
+  mark();
+  st = scan_big(d.tri_cnt, d.tri_base, d.scan_aux, d.totVF, s);
   if (st) return st;
   mark();
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
-  hipLaunchKernelGGL(fine_kernel, dim3((unsigned)d.totTiles), dim3(kTilePix), 0, s, d, io);
+  hipLaunchKernelGGL(fine_kernel, dim3((unsigned)((long long)V * d.maxTiles)), dim3(kTilePix), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(boundary_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(sil_point_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(sil_point_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(sil_grid_count_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
-  st = exclusive_scan(d.cell_cnt, d.cell_off, d.scan_aux2, d.totCells, s);
+  st = scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, s);
   if (st) return st;
   mark();
   hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(ceil_div(d.maxS2, kB), V), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(vertex_bwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(vertex_bwd_kernel, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
+  mark();
+  hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, s, d, io);
   mark();
   return check_launch("dcx_pipeline_step");
 }
@@ -1026,13 +1250,13 @@ extern "C" dcx_status dcx_pipeline_capture(dcx_pipeline* p, const dcx_pipeline_i
   DCX_REQUIRE(p && io, "dcx_pipeline_capture: null argument");
   DCX_REQUIRE(stream, "dcx_pipeline_capture: capture needs a non-default stream");
   hipStream_t s = as_stream(stream);
-  if (p->exec) { hipGraphExecDestroy(p->exec); p->exec = nullptr; }
-  if (p->graph) { hipGraphDestroy(p->graph); p->graph = nullptr; }
+  if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   DCX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   dcx_status st = pipe_enqueue(p, io, s);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(s, &g);
-  if (st) { if (g) hipGraphDestroy(g); return st; }
+  if (st) { if (g) (void)hipGraphDestroy(g); return st; }
   if (e != hipSuccess) return set_error(DCX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
   p->graph = g;
   DCX_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
@@ -1067,6 +1291,6 @@ extern "C" dcx_status dcx_pipeline_profile(dcx_pipeline* p, const dcx_pipeline_i
     if (hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]) != hipSuccess)
       st = set_error(DCX_ERR_HIP, "dcx_pipeline_profile: hipEventElapsedTime");
   for (int i = 0; i <= kNumStages; ++i)
-    if (ev[i]) hipEventDestroy(ev[i]);
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
   return st;
 }
