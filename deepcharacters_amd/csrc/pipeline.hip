@@ -26,9 +26,10 @@
 //   sil_point        model->data DT residual, squared loss, its gradient
 //   sil grid + sil_pixel   data->model residual via the binned nearest
 //                    boundary-point search, squared loss, gradient
-//   vertex_bwd       per vertex: gather of the (triangle, tile) records of its
-//                    faces in every view, camera-projection adjoint, DQ-skinning
-//                    adjoint, colour gradient
+//   face_sum         per face: its (triangle, tile) records summed over tiles
+//                    and views
+//   vertex_bwd       per vertex: gather of its faces' sums, camera-projection
+//                    adjoint, DQ-skinning adjoint, colour gradient
 //   node_bwd         one wavefront per ED node: translation / Euler gradients
 //   finalize         per character: losses and joint-DQ gradients from the
 //                    per-block partial sums (fixed order: deterministic)
@@ -105,6 +106,7 @@ struct Dev {
   unsigned int* cell_fill;
   unsigned int* cell_items;  // [totVN]
   unsigned int* scan_aux;
+  float* face_sum;        // [sumF,18] raster gradient per face, summed over tiles and views
   float* g_v;             // [sumN,3] position gradient from records beyond capacity
   float* g_pts;           // [totVN,2]
   float* u;               // [sumN,3] gradient w.r.t. the ED-deformed position
@@ -729,13 +731,19 @@ __global__ __launch_bounds__(kB) void sil_point_kernel_p(Dev d, IO io) {
   if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
 }
 
+// Uniform grid over a view's DT crop for the nearest-boundary-point search:
+// kRing cells per search radius R = 30 * scale (cells a hair larger than
+// R / kRing), kGridMargin cells of margin, so every point within R of a DT
+// pixel lies inside the grid.
+constexpr int kRing = 8;
+constexpr int kGridMargin = kRing + 1;
 struct GridGeomP { float ox, oy, cell; };
 __device__ __forceinline__ GridGeomP grid_geom_p(const float* cr) {
   const float sc = cr[2];
   GridGeomP g;
-  g.cell = 30.f * sc * (1.f + 1.f / 1024.f);
-  g.ox = (0.f - cr[0]) * sc + cr[3] - 2.f * g.cell;
-  g.oy = (0.f - cr[1]) * sc + cr[4] - 2.f * g.cell;
+  g.cell = 30.f * sc * (1.f + 1.f / 1024.f) / (float)kRing;
+  g.ox = (0.f - cr[0]) * sc + cr[3] - (float)kGridMargin * g.cell;
+  g.oy = (0.f - cr[1]) * sc + cr[4] - (float)kGridMargin * g.cell;
   return g;
 }
 __device__ __forceinline__ int cell_of_p(const GridGeomP& g, int gx, int gy, float x, float y) {
@@ -793,21 +801,37 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       const float* cr = io.crops + (size_t)w * 7;
       const float sc = cr[2];
       const float ix = ((float)ui - cr[0]) * sc + cr[3], iy = ((float)vi - cr[1]) * sc + cr[4];
-      float bd = 30.f * sc;
+      const float R = 30.f * sc;
+      float bd = R;
       int best = -1;
       const GridGeomP gg = grid_geom_p(cr);
       const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
       const f4* pv = d.proj + vw.vn0;
-      for (int yy = max(cy - 1, 0); yy <= min(cy + 1, vw.gy - 1); ++yy)
-        for (int xx = max(cx - 1, 0); xx <= min(cx + 1, vw.gx - 1); ++xx) {
-          const long long cell = vw.cell0 + yy * vw.gx + xx;
-          for (unsigned int k = d.cell_off[cell]; k < d.cell_off[cell + 1]; ++k) {
-            const int v = (int)d.cell_items[k];
-            const float dx = pv[v].x - ix, dy = pv[v].y - iy;
-            const float dd = sqrtf(dx * dx + dy * dy);
-            if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; }
+      // Rings of cells at Chebyshev distance k around the pixel's cell.  A
+      // point in ring k + 1 or beyond is at least k cells away along one
+      // axis, so once the best distance is below that bound (with a margin
+      // for float rounding) no later point can beat or tie it; the result is
+      // the brute-force minimum of (distance, index) with distance < R.
+      for (int k = 0; k <= kRing; ++k) {
+        for (int oy = -k; oy <= k; ++oy) {
+          const int yy = cy + oy;
+          if (yy < 0 || yy >= vw.gy) continue;
+          const int step = (oy == -k || oy == k) ? 1 : 2 * k;
+          for (int ox = -k; ox <= k; ox += (step > 0 ? step : 1)) {
+            const int xx = cx + ox;
+            if (xx < 0 || xx >= vw.gx) continue;
+            const long long cell = vw.cell0 + yy * vw.gx + xx;
+            for (unsigned int q = d.cell_off[cell]; q < d.cell_off[cell + 1]; ++q) {
+              const int v = (int)d.cell_items[q];
+              const float dx = pv[v].x - ix, dy = pv[v].y - iy;
+              const float dd = sqrtf(dx * dx + dy * dy);
+              if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; }
+            }
           }
         }
+        const float reach = (float)k * gg.cell * (1.f - 1e-4f);
+        if (bd < reach || reach >= R) break;
+      }
       if (best >= 0) {
         const float r0 = (pv[best].x - ix) / (30.f * sc), r1 = (pv[best].y - iy) / (30.f * sc);
         lsum = io.w_sil * (r0 * r0 + r1 * r1);
@@ -824,7 +848,32 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
 // Backward: vertices, ED nodes, per-character reductions
 // ---------------------------------------------------------------------------
 
-// dL/dv (gathered raster records + projection adjoint over views) -> DQ-skinning adjoint.
+// Per face of every character: its (triangle, tile) records summed over the
+// tiles of each view and over the views, in a fixed order.
+__global__ __launch_bounds__(kB) void face_sum_kernel(Dev d) {
+  const CharD c = d.ch[blockIdx.y];
+  const int f = blockIdx.x * kB + threadIdx.x;
+  if (f >= c.F) return;
+  float a[kGrad];
+#pragma unroll
+  for (int q = 0; q < kGrad; ++q) a[q] = 0.f;
+  for (int k = 0; k < c.C; ++k) {
+    const long long ri = d.vw[c.view0 + k].vf0 + f;
+    const unsigned int e0 = d.tri_base[ri];
+    const unsigned int e1 = min(d.tri_base[ri + 1], d.capacity);
+    for (unsigned int e = e0; e < e1; ++e) {
+      if (!d.rec_vis[e]) continue;
+      const float* r = d.rec + (size_t)e * kGrad;
+#pragma unroll
+      for (int q = 0; q < kGrad; ++q) a[q] += r[q];
+    }
+  }
+  float* o = d.face_sum + ((size_t)c.f0 + f) * kGrad;
+#pragma unroll
+  for (int q = 0; q < kGrad; ++q) o[q] = a[q];
+}
+
+// dL/dv (per-face raster sums + projection adjoint over views) -> DQ-skinning adjoint.
 __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
   __shared__ float s_acc[256 * 8];
@@ -837,31 +886,24 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
     f3 g = ld3(d.g_v + 3 * (size_t)gv);
     f3 gc = ld3(io.g_col + 3 * (size_t)gv);
     const f3 X = ld3(d.vdef + 3 * (size_t)gv);
-    const int k0 = d.vf_off[gv], k1 = d.vf_off[gv + 1];
+    // raster: the per-face sums of the incident faces
+    int prev = -1;
+    for (int q = d.vf_off[gv]; q < d.vf_off[gv + 1]; ++q) {
+      const int fl = d.vf_list[q];
+      if (fl == prev) continue;  // a face listing this vertex twice is handled once, all slots
+      prev = fl;
+      const int* fv = d.faces + 3 * ((size_t)c.f0 + fl);
+      const float* r = d.face_sum + ((size_t)c.f0 + fl) * kGrad;
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+        if (fv[s] == v) {
+          g += ld3(r + 6 * s);
+          gc += ld3(r + 6 * s + 3);
+        }
+    }
     for (int k = 0; k < c.C; ++k) {
       const int w = c.view0 + k;
       const ViewD vw = d.vw[w];
-      // raster records of the incident faces in this view
-      int prev = -1;
-      for (int q = k0; q < k1; ++q) {
-        const int fl = d.vf_list[q];
-        if (fl == prev) continue;  // a face listing this vertex twice is handled once, all slots
-        prev = fl;
-        const int* fv = d.faces + 3 * ((size_t)c.f0 + fl);
-        const long long ri = vw.vf0 + fl;
-        const unsigned int e0 = d.tri_base[ri];
-        const unsigned int e1 = min(d.tri_base[ri + 1], d.capacity);
-        for (unsigned int e = e0; e < e1; ++e) {
-          if (!d.rec_vis[e]) continue;
-          const float* r = d.rec + (size_t)e * kGrad;
-#pragma unroll
-          for (int s = 0; s < 3; ++s)
-            if (fv[s] == v) {
-              g += ld3(r + 6 * s);
-              gc += ld3(r + 6 * s + 3);
-            }
-        }
-      }
       // silhouette: camera-projection adjoint (CameraProjectionGPUOpGrad.cu:12-79)
       const f2 gp = ((const f2*)d.g_pts)[vw.vn0 + v];
       if (gp.x == 0.f && gp.y == 0.f) continue;
@@ -1072,7 +1114,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
       v.tiles_x = (c.width + kTileW - 1) / kTileW;
       v.tiles = v.tiles_x * ((c.height + kTileH - 1) / kTileH);
       v.vn0 = vn; v.vf0 = vf; v.pix0 = pix; v.sil0 = sil; v.tile0 = tiles; v.cell0 = cells;
-      v.gx = c.dt_size / 30 + 6; v.gy = c.dt_size / 30 + 6;
+      v.gx = c.dt_size * kRing / 30 + 2 * kGridMargin + 3; v.gy = v.gx;
       vn += d.N; vf += d.F; pix += (long long)c.width * c.height; sil += (long long)c.dt_size * c.dt_size;
       tiles += v.tiles; cells += (long long)v.gx * v.gy;
       maxTiles = std::max(maxTiles, v.tiles);
@@ -1138,6 +1180,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.cell_off, 4 * (size_t)(cells + 1));
   alloc(&d.cell_fill, 4 * (size_t)cells);
   alloc(&d.cell_items, 4 * (size_t)vn);
+  alloc(&d.face_sum, sizeof(float) * kGrad * (size_t)p->sumF);
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
   alloc(&d.u, sizeof(float) * 3 * (size_t)p->sumN);
@@ -1169,8 +1212,8 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "project",    "tri_count",
     "tile_scan",      "tri_scan",      "bin_scatter",      "fine",      "boundary",   "sil_point",
-    "sil_grid_count", "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "vertex_bwd", "node_bwd",
-    "finalize"};
+    "sil_grid_count", "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "face_sum",   "vertex_bwd",
+    "node_bwd",       "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
@@ -1231,6 +1274,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
   mark();
   hipLaunchKernelGGL(vertex_bwd_kernel, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
   mark();
