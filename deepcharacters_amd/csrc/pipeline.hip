@@ -104,7 +104,7 @@ struct Dev {
   unsigned int* cell_cnt; // [totCells]
   unsigned int* cell_off; // [totCells+1]
   unsigned int* cell_fill;
-  unsigned int* cell_items;  // [totVN]
+  f4* cell_pts;           // [totVN] (x, y, point id bits, 0) grouped by cell
   unsigned int* scan_aux;
   float* face_sum;        // [sumF,18] raster gradient per face, summed over tiles and views
   float* g_v;             // [sumN,3] position gradient from records beyond capacity
@@ -334,45 +334,110 @@ __device__ __forceinline__ TileRange tile_range(const TriRec& t) {
   return r;
 }
 
+// Per-workgroup aggregation of (triangle, tile) pairs: an LDS open-addressing
+// table keyed by the tile, so the workgroup issues one device-scope atomic per
+// distinct tile instead of one per pair (device-scope atomics are executed in
+// memory, not in an XCD's L2).  Pairs of large triangles, and pairs whose
+// tile finds no free slot within kProbe probes, go straight to memory; every
+// thread probing a given tile follows the same sequence, so all of them agree
+// on whether and where that tile sits in the table.
+constexpr int kHash = 1024;
+constexpr int kProbe = 8;
+constexpr int kBigTri = 32;  // tiles; larger triangles bypass the table
+constexpr unsigned int kHashEmpty = 0xffffffffu;
+
+__device__ __forceinline__ int hash_find(unsigned int* keys, unsigned int tile, bool insert) {
+  unsigned int h = (tile * 2654435761u) >> 22;  // 10 bits
+#pragma unroll 1
+  for (int pr = 0; pr < kProbe; ++pr, h = (h + 1) & (kHash - 1)) {
+    const unsigned int cur = insert ? atomicCAS(keys + h, kHashEmpty, tile) : keys[h];
+    if (cur == tile || (insert && cur == kHashEmpty)) return (int)h;
+    if (!insert && cur == kHashEmpty) return -1;
+  }
+  return -1;
+}
+
 __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
+  __shared__ unsigned int s_keys[kHash];
+  __shared__ unsigned int s_cnt[kHash];
+  for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
+  __syncthreads();
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
   const int f = blockIdx.x * kB + threadIdx.x;
-  if (f >= c.F) return;
-  const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
-  const f4* pv = d.proj + vw.vn0;
-  TriRec t;
-  const long long ri = vw.vf0 + f;
-  if (!setup_tri(pv[fv[0]], pv[fv[1]], pv[fv[2]], f, vw.W, vw.H, t)) {
-    t.bx0_by0 = 0x7fff7fff;
-    t.bx1_by1 = 0;
-    d.tri[ri] = t;
-    d.tri_cnt[ri] = 0u;
-    return;
-  }
-  d.tri[ri] = t;
-  const TileRange r = tile_range(t);
-  d.tri_cnt[ri] = (unsigned int)(r.ntx * r.nty);
   unsigned int* cnt = d.tile_cnt + vw.tile0;
-  for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
-    for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) atomicAdd(cnt + ty * vw.tiles_x + tx, 1u);
+  if (f < c.F) {
+    const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
+    const f4* pv = d.proj + vw.vn0;
+    TriRec t;
+    const long long ri = vw.vf0 + f;
+    if (!setup_tri(pv[fv[0]], pv[fv[1]], pv[fv[2]], f, vw.W, vw.H, t)) {
+      t.bx0_by0 = 0x7fff7fff;
+      t.bx1_by1 = 0;
+      d.tri[ri] = t;
+      d.tri_cnt[ri] = 0u;
+    } else {
+      d.tri[ri] = t;
+      const TileRange r = tile_range(t);
+      d.tri_cnt[ri] = (unsigned int)(r.ntx * r.nty);
+      const bool big = r.ntx * r.nty > kBigTri;
+      for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
+        for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) {
+          const unsigned int lt = (unsigned int)(ty * vw.tiles_x + tx);
+          const int h = big ? -1 : hash_find(s_keys, lt, true);
+          if (h >= 0) atomicAdd(s_cnt + h, 1u);
+          else atomicAdd(cnt + lt, 1u);
+        }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHash; i += kB)
+    if (s_keys[i] != kHashEmpty) atomicAdd(cnt + s_keys[i], s_cnt[i]);
 }
 
 __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
+  __shared__ unsigned int s_keys[kHash];
+  __shared__ unsigned int s_cnt[kHash];
+  __shared__ unsigned int s_base[kHash];
+  for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
+  __syncthreads();
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
   const int f = blockIdx.x * kB + threadIdx.x;
-  if (f >= c.F) return;
   const long long ri = vw.vf0 + f;
-  const TriRec& t = d.tri[ri];
-  if (t.bx0_by0 == 0x7fff7fff) return;
-  const TileRange r = tile_range(t);
+  TileRange r{0, 0, 0, 0};
+  bool big = false;
+  if (f < c.F) {
+    const TriRec& t = d.tri[ri];
+    if (t.bx0_by0 != 0x7fff7fff) {
+      r = tile_range(t);
+      big = r.ntx * r.nty > kBigTri;
+    }
+  }
+  // pass 1: per-tile counts of this workgroup's pairs
+  if (!big)
+    for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
+      for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) {
+        const int h = hash_find(s_keys, (unsigned int)(ty * vw.tiles_x + tx), true);
+        if (h >= 0) atomicAdd(s_cnt + h, 1u);
+      }
+  __syncthreads();
+  // one reservation per distinct tile
+  for (int i = threadIdx.x; i < kHash; i += kB) {
+    if (s_keys[i] != kHashEmpty) s_base[i] = atomicAdd(d.tile_fill + vw.tile0 + s_keys[i], s_cnt[i]);
+    s_cnt[i] = 0u;
+  }
+  __syncthreads();
+  // pass 2: slots
   for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
     for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) {
-      const long long tile = vw.tile0 + ty * vw.tiles_x + tx;
-      const unsigned int slot = d.tile_off[tile] + atomicAdd(d.tile_fill + tile, 1u);
+      const unsigned int lt = (unsigned int)(ty * vw.tiles_x + tx);
+      const long long tile = vw.tile0 + lt;
+      const int h = big ? -1 : hash_find(s_keys, lt, false);
+      const unsigned int rank = h >= 0 ? s_base[h] + atomicAdd(s_cnt + h, 1u) : atomicAdd(d.tile_fill + tile, 1u);
+      const unsigned int slot = d.tile_off[tile] + rank;
       if (slot < d.capacity) d.bins[slot] = (unsigned int)ri;
     }
 }
@@ -773,7 +838,7 @@ __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
   const int cl = cell_of_p(grid_geom_p(io.crops + (size_t)w * 7), vw.gx, vw.gy, p.x, p.y);
   if (cl < 0) return;
   const long long cell = vw.cell0 + cl;
-  d.cell_items[d.cell_off[cell] + atomicAdd(d.cell_fill + cell, 1u)] = (unsigned int)v;
+  d.cell_pts[d.cell_off[cell] + atomicAdd(d.cell_fill + cell, 1u)] = mk4(p.x, p.y, __int_as_float(v), 0.f);
 }
 
 // Data->model residual via the binned search + loss + gradient (atomic, as the reference).
@@ -806,7 +871,7 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       int best = -1;
       const GridGeomP gg = grid_geom_p(cr);
       const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
-      const f4* pv = d.proj + vw.vn0;
+      float bx = 0.f, by = 0.f;
       // Rings of cells at Chebyshev distance k around the pixel's cell.  A
       // point in ring k + 1 or beyond is at least k cells away along one
       // axis, so once the best distance is below that bound (with a margin
@@ -821,11 +886,13 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
             const int xx = cx + ox;
             if (xx < 0 || xx >= vw.gx) continue;
             const long long cell = vw.cell0 + yy * vw.gx + xx;
-            for (unsigned int q = d.cell_off[cell]; q < d.cell_off[cell + 1]; ++q) {
-              const int v = (int)d.cell_items[q];
-              const float dx = pv[v].x - ix, dy = pv[v].y - iy;
+            const unsigned int q1 = d.cell_off[cell + 1];
+            for (unsigned int q = d.cell_off[cell]; q < q1; ++q) {
+              const f4 it = d.cell_pts[q];
+              const int v = __float_as_int(it.z);
+              const float dx = it.x - ix, dy = it.y - iy;
               const float dd = sqrtf(dx * dx + dy * dy);
-              if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; }
+              if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; bx = it.x; by = it.y; }
             }
           }
         }
@@ -833,7 +900,7 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
         if (bd < reach || reach >= R) break;
       }
       if (best >= 0) {
-        const float r0 = (pv[best].x - ix) / (30.f * sc), r1 = (pv[best].y - iy) / (30.f * sc);
+        const float r0 = (bx - ix) / (30.f * sc), r1 = (by - iy) / (30.f * sc);
         lsum = io.w_sil * (r0 * r0 + r1 * r1);
         atomicAdd(d.g_pts + 2 * (vw.vn0 + best), 2.f * io.w_sil * r0 / (30.f * sc));
         atomicAdd(d.g_pts + 2 * (vw.vn0 + best) + 1, 2.f * io.w_sil * r1 / (30.f * sc));
@@ -1179,7 +1246,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.cell_cnt, 4 * (size_t)cells);
   alloc(&d.cell_off, 4 * (size_t)(cells + 1));
   alloc(&d.cell_fill, 4 * (size_t)cells);
-  alloc(&d.cell_items, 4 * (size_t)vn);
+  alloc(&d.cell_pts, sizeof(f4) * (size_t)vn);
   alloc(&d.face_sum, sizeof(float) * kGrad * (size_t)p->sumF);
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
