@@ -540,10 +540,38 @@ __device__ __forceinline__ long long xcd_tile(long long wg, long long n) {
   return x * q + (x < r ? x : r) + i;
 }
 
+constexpr int kFast = 256;          // tiles with at most this many triangles take the pixel-parallel path
+constexpr int kGPad = kTilePix + 1;  // row stride of the per-pixel gradient table (bank spread)
+
+// Sums of the per-pixel gradient table over the pixels each (triangle, tile)
+// pair won, written as one record per pair (or, without a record slot,
+// accumulated into the fallback buffers).
+__device__ __forceinline__ void emit_record(const Dev& d, const IO& io, const CharD& c, unsigned int e, int face,
+                                            int q, float a, bool won) {
+  if (e < d.capacity) {
+    if (q == 0) d.rec_vis[e] = won ? 1 : 0;
+    if (won) d.rec[(size_t)e * kGrad + q] = a;
+  } else if (won) {
+    const int kk = q / 6, comp = q - 6 * kk;
+    const int vtx = c.v0 + d.faces[3 * ((size_t)c.f0 + face) + kk];
+    if (comp < 3) atomicAdd(d.g_v + 3 * (size_t)vtx + comp, a);
+    else atomicAdd(io.g_col + 3 * (size_t)vtx + comp - 3, a);
+  }
+}
+
 __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
+  // u1: triangle records of the tile (raster) / per-pixel gradient table (backward)
+  __shared__ union U1 { TriRec tri[kFast]; float g[kGrad * kGPad]; } u1;
+  // u2: target colours with apron (per-pixel phase) / counting-sort scratch (records)
+  __shared__ union U2 {
+    float tgt[3 * kApronH * kApronW];
+    struct { unsigned int wcnt[kTilePix / 64][kFast]; unsigned short order[kTilePix]; } srt;
+  } u2;
   __shared__ unsigned long long s_key[kTilePix];
-  __shared__ float s_g[kGrad][kTilePix];
-  __shared__ float s_tgt[3][kApronH * kApronW];
+  __shared__ unsigned int s_start[kFast + 1];
+  __shared__ unsigned int s_e[kFast];
+  __shared__ int s_fj[kFast];
+  __shared__ unsigned int s_wave[kTilePix / 64];
   __shared__ float s_red[kTilePix / 64];
   const long long lin = xcd_tile(blockIdx.x, (long long)gridDim.x);
   if (lin >= (long long)d.V * d.maxTiles) return;
@@ -554,36 +582,73 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
   const CharD c = d.ch[vw.ch];
   const long long tile = vw.tile0 + lt;
   const int tx0 = (lt % vw.tiles_x) * kTileW, ty0 = (lt / vw.tiles_x) * kTileH;
+  const int tx = tx0 / kTileW, ty = ty0 / kTileH;
   const unsigned int off = d.tile_off[tile], end = d.tile_off[tile + 1];
   const bool overflow = end > d.capacity;
   const unsigned int n_items = overflow ? (unsigned int)c.F : end - off;
+  const bool fast = !overflow && n_items <= (unsigned int)kFast;  // uniform over the block
   const TriRec* tv = d.tri + vw.vf0;
+  const int px_i = threadIdx.x % kTileW, py_i = threadIdx.x / kTileW;
+  const int x = tx0 + px_i, y = ty0 + py_i;
+  const bool inimg = x < vw.W && y < vw.H;
 
-  s_key[threadIdx.x] = kEmptyKey;
   // target colours of the tile plus a kImgFs apron (clamped reads; the apron is
   // only used by pixels whose whole filter window lies inside the image)
   for (int i = threadIdx.x; i < kApronH * kApronW; i += kTilePix) {
     const int ax = i % kApronW, ay = i / kApronW;
     const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
     const f3 t = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
-    s_tgt[0][i] = t.x; s_tgt[1][i] = t.y; s_tgt[2][i] = t.z;
+    u2.tgt[i] = t.x; u2.tgt[kApronH * kApronW + i] = t.y; u2.tgt[2 * kApronH * kApronW + i] = t.z;
   }
-  __syncthreads();
-  for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix)
-    raster_in_tile(overflow ? tv[k] : d.tri[d.bins[off + k]], tx0, ty0, s_key);
-  __syncthreads();
+
+  int f = -1;   // winning face of this pixel
+  int bj = -1;  // fast path: index of the winning triangle in the tile list
+  if (fast) {
+    if (threadIdx.x < n_items) {
+      const unsigned int ri = d.bins[off + threadIdx.x];
+      const TriRec t = d.tri[ri];
+      u1.tri[threadIdx.x] = t;
+      s_fj[threadIdx.x] = t.face_flags & kFaceMask;
+      const TileRange r = tile_range(t);
+      s_e[threadIdx.x] = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
+    }
+    __syncthreads();
+    // pixel-parallel coverage: the minimum packed (depth, face) key over the
+    // tile's triangles, with the same exact edge functions as the
+    // triangle-parallel path
+    if (inimg) {
+      unsigned long long best = kEmptyKey;
+      for (int j = 0; j < (int)n_items; ++j) {
+        const TriRec& t = u1.tri[j];
+        if (x < (t.bx0_by0 & 0xffff) || x > (t.bx1_by1 & 0xffff) || y < (t.bx0_by0 >> 16) || y > (t.bx1_by1 >> 16))
+          continue;
+        EdgeWalk ew;
+        ew.init(t, x, y);
+        if (!ew.inside(ew.e)) continue;
+        const unsigned long long k = make_key(interp_iz(ew.e[0], ew.e[1], ew.e[2], t.iz), t.face_flags & kFaceMask);
+        if (k < best) { best = k; bj = j; }
+      }
+      if (bj >= 0) f = (int)(best & 0xffffffffull);
+    }
+    __syncthreads();  // u1.tri is overwritten by the gradient table below
+  } else {
+    s_key[threadIdx.x] = kEmptyKey;
+    __syncthreads();
+    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix)
+      raster_in_tile(overflow ? tv[k] : d.tri[d.bins[off + k]], tx0, ty0, s_key);
+    __syncthreads();
+    const unsigned long long key = s_key[threadIdx.x];
+    f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
+  }
 
   // ---- per pixel: outputs, colour loss, gradient contributions -> LDS
-  const int px_i = threadIdx.x % kTileW, py_i = threadIdx.x / kTileW;
-  const int x = tx0 + px_i, y = ty0 + py_i;
-  const bool inimg = x < vw.W && y < vw.H;
-  const unsigned long long key = s_key[threadIdx.x];
-  const int f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
+  float* s_g = u1.g;
+  const int pix = threadIdx.x;
   float lsum = 0.f;
   if (inimg) {
     const long long o = vw.pix0 + (long long)y * vw.W + x;
     const int ta = (py_i + kImgFs) * kApronW + (px_i + kImgFs);
-    const f3 tgt = mk3(s_tgt[0][ta], s_tgt[1][ta], s_tgt[2][ta]);
+    const f3 tgt = mk3(u2.tgt[ta], u2.tgt[kApronH * kApronW + ta], u2.tgt[2 * kApronH * kApronW + ta]);
     if (f < 0) {
       io.face[o] = -1;
       io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
@@ -610,9 +675,9 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       // colour gradient and dL/dweights (vertexColor, shadeless)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        s_g[6 * k + 3][threadIdx.x] = g.x * wgt[k];
-        s_g[6 * k + 4][threadIdx.x] = g.y * wgt[k];
-        s_g[6 * k + 5][threadIdx.x] = g.z * wgt[k];
+        s_g[(6 * k + 3) * kGPad + pix] = g.x * wgt[k];
+        s_g[(6 * k + 4) * kGPad + pix] = g.y * wgt[k];
+        s_g[(6 * k + 5) * kGPad + pix] = g.z * wgt[k];
       }
       const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
       const float* E = io.ext + (size_t)w * 12;
@@ -629,7 +694,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         for (int yy = -fs; yy <= fs; ++yy)
           for (int xx = -fs; xx <= fs; ++xx) {
             const int a = (py_i + kImgFs + yy) * kApronW + (px_i + kImgFs + xx);
-            const f3 I = mk3(s_tgt[0][a], s_tgt[1][a], s_tgt[2][a]);
+            const f3 I = mk3(u2.tgt[a], u2.tgt[kApronH * kApronW + a], u2.tgt[2 * kApronH * kApronW + a]);
             const float den = (float)(xx * xx + yy * yy);
             float gu = 0.f, gvv = 0.f;
             if (den != 0.f) { gu = (float)xx / den; gvv = (float)yy / den; }
@@ -660,23 +725,66 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        s_g[6 * k][threadIdx.x] = gx[k].x;
-        s_g[6 * k + 1][threadIdx.x] = gx[k].y;
-        s_g[6 * k + 2][threadIdx.x] = gx[k].z;
+        s_g[(6 * k) * kGPad + pix] = gx[k].x;
+        s_g[(6 * k + 1) * kGPad + pix] = gx[k].y;
+        s_g[(6 * k + 2) * kGPad + pix] = gx[k].z;
       }
     }
   }
-  const float lt_sum = block_sum(lsum, s_red);  // its barriers also publish s_g
+  const float lt_sum = block_sum(lsum, s_red);  // its barriers also publish s_g and retire u2.tgt
   if (threadIdx.x == 0) d.loss_tile[tile] = lt_sum;
-  // the winning face of each pixel, in the low half of its key slot
+
+  if (fast) {
+    // ---- counting sort of the covered pixels by winning triangle, pixel order
+    // kept within each triangle: ranks within the wavefront from ballots,
+    // then per-wavefront counts and a scan over the triangles
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < (kTilePix / 64) * kFast; i += kTilePix) (&u2.srt.wcnt[0][0])[i] = 0u;
+    __syncthreads();
+    int rank = 0;
+    unsigned long long act = __ballot(bj >= 0);
+    while (act) {
+      const int leader = __ffsll((long long)act) - 1;
+      const int jv = __shfl(bj, leader, 64);
+      const unsigned long long m = __ballot(bj == jv);
+      if (bj == jv) rank = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == leader) u2.srt.wcnt[wv][jv] = (unsigned int)__popcll(m);
+      act &= ~m;
+    }
+    __syncthreads();
+    unsigned int cnt = 0;
+    if (threadIdx.x < kFast) {
+#pragma unroll
+      for (int q = 0; q < kTilePix / 64; ++q) {
+        const unsigned int cq = u2.srt.wcnt[q][threadIdx.x];
+        u2.srt.wcnt[q][threadIdx.x] = cnt;  // prefix over wavefronts
+        cnt += cq;
+      }
+    }
+    unsigned int total;
+    const unsigned int ex = block_scan256(cnt, s_wave, total);
+    if (threadIdx.x < kFast) s_start[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_start[kFast] = total;
+    __syncthreads();
+    if (bj >= 0) u2.srt.order[s_start[bj] + u2.srt.wcnt[wv][bj] + rank] = (unsigned short)pix;
+    __syncthreads();
+    // ---- one (triangle, component) pair per work item
+    for (int i = threadIdx.x; i < (int)n_items * kGrad; i += kTilePix) {
+      const int j = i / kGrad, q = i - j * kGrad;
+      const unsigned int st = s_start[j], en = s_start[j + 1];
+      float a = 0.f;
+      for (unsigned int r = st; r < en; ++r) a += s_g[q * kGPad + u2.srt.order[r]];
+      emit_record(d, io, c, s_e[j], s_fj[j], q, a, en > st);
+    }
+    return;
+  }
+
+  // ---- slow path (long or overflowed tile lists): the thread that rasterised
+  // a triangle sums the pixels it won (row-major order)
   int* s_face = reinterpret_cast<int*>(s_key);
   __syncthreads();
   s_face[threadIdx.x] = f;
   __syncthreads();
-
-  // ---- per (triangle, tile): the thread that rasterised the triangle sums
-  // the pixels it won (row-major order) into one record
-  const int tx = tx0 / kTileW, ty = ty0 / kTileH;
   for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
     const unsigned int ri_local = overflow ? k : d.bins[off + k] - (unsigned int)vw.vf0;
     const TriRec t = tv[ri_local];
@@ -695,29 +803,13 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         if (s_face[p] == face) {
           won = true;
 #pragma unroll
-          for (int q = 0; q < kGrad; ++q) a[q] += s_g[q][p];
+          for (int q = 0; q < kGrad; ++q) a[q] += s_g[q * kGPad + p];
         }
       }
     const TileRange r = tile_range(t);
     const unsigned int e = d.tri_base[vw.vf0 + ri_local] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
-    if (e < d.capacity) {
-      d.rec_vis[e] = won ? 1 : 0;
-      if (won) {
-        float* dst = d.rec + (size_t)e * kGrad;
 #pragma unroll
-        for (int q = 0; q < kGrad; ++q) dst[q] = a[q];
-      }
-    } else if (won) {
-      // no record slot (the batch overflowed its capacity): accumulate directly
-      const int* fv = d.faces + 3 * ((size_t)c.f0 + face);
-#pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
-        float* gp = d.g_v + 3 * ((size_t)c.v0 + fv[kk]);
-        atomicAdd(gp, a[6 * kk]); atomicAdd(gp + 1, a[6 * kk + 1]); atomicAdd(gp + 2, a[6 * kk + 2]);
-        float* gc = io.g_col + 3 * ((size_t)c.v0 + fv[kk]);
-        atomicAdd(gc, a[6 * kk + 3]); atomicAdd(gc + 1, a[6 * kk + 4]); atomicAdd(gc + 2, a[6 * kk + 5]);
-      }
-    }
+    for (int q = 0; q < kGrad; ++q) emit_record(d, io, c, e, face, q, a[q], won);
   }
 }
 
