@@ -30,9 +30,10 @@
 //                    and views
 //   vertex_bwd       per vertex: gather of its faces' sums, camera-projection
 //                    adjoint, DQ-skinning adjoint, colour gradient
+//   joint_reduce     one wavefront per joint: DQ gradient over its (vertex, slot)
+//                    entries (static CSR, fixed order: deterministic)
 //   node_bwd         one wavefront per ED node: translation / Euler gradients
-//   finalize         per character: losses and joint-DQ gradients from the
-//                    per-block partial sums (fixed order: deterministic)
+//   finalize         per character: losses from the per-block partial sums
 #include <algorithm>
 #include <vector>
 
@@ -84,6 +85,8 @@ struct Dev {
   const float* n2v_w;
   const int* vf_off;      // [sumN+1] global offsets
   const int* vf_list;     // local face ids, increasing per vertex
+  const int* jv_off;      // [sumJ+1] global offsets: joint -> (vertex, slot) entries
+  const int* jv_ent;      // local vertex * 4 + slot, increasing
   // scratch
   float* node_tab;        // [sumM,15]
   float* vdef;            // [sumN,3] skinned vertices
@@ -113,7 +116,8 @@ struct Dev {
   float* loss_tile;       // [totTiles] colour loss per tile
   float* loss_pt;         // [V*gxN] silhouette model->data loss per block
   float* loss_px;         // [V*gxS] silhouette data->model loss per block
-  float* dq_part;         // [B*gxN, maxJ*8] joint-DQ gradient per vertex block
+  float* vq;              // [sumN,8] gradient w.r.t. the vertex's blended, normalised DQ
+  float* vws;             // [sumN,4] skinning weight x blend sign per slot
 };
 
 struct IO {
@@ -1035,9 +1039,8 @@ __global__ __launch_bounds__(kB) void face_sum_kernel(Dev d) {
 // dL/dv (per-face raster sums + projection adjoint over views) -> DQ-skinning adjoint.
 __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
-  __shared__ float s_acc[256 * 8];
   const CharD c = d.ch[blockIdx.y];
-  for (int i = threadIdx.x; i < c.J * 8; i += kB) { s_tab[i] = io.dq[(size_t)c.j0 * 8 + i]; s_acc[i] = 0.f; }
+  for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
   __syncthreads();
   const int v = blockIdx.x * kB + threadIdx.x;
   if (v < c.N) {
@@ -1102,18 +1105,43 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
       gbr = (gr - r * proj) * bl.scale;
       gbd = gd * bl.scale;
     }
-    for (int k = 0; k < KS; ++k) {
-      const float ws = w[k] * bl.sgn[k];
-      float* a = s_acc + idx[k] * 8;
-      atomicAdd(a, gbr.x * ws); atomicAdd(a + 1, gbr.y * ws); atomicAdd(a + 2, gbr.z * ws); atomicAdd(a + 3, gbr.w * ws);
-      atomicAdd(a + 4, gbd.x * ws); atomicAdd(a + 5, gbd.y * ws); atomicAdd(a + 6, gbd.z * ws); atomicAdd(a + 7, gbd.w * ws);
-    }
+    // the joint DQ gradients sum w_k * s_k * (gbr, gbd) over vertices: kept per
+    // vertex and reduced per joint in a fixed order by joint_reduce_kernel
+    float* q8 = d.vq + 8 * (size_t)gv;
+    q8[0] = gbr.x; q8[1] = gbr.y; q8[2] = gbr.z; q8[3] = gbr.w;
+    q8[4] = gbd.x; q8[5] = gbd.y; q8[6] = gbd.z; q8[7] = gbd.w;
+    for (int k = 0; k < KS; ++k) d.vws[KS * (size_t)gv + k] = w[k] * bl.sgn[k];
     const f3 u = mulT(R, g);
     d.u[3 * (size_t)gv] = u.x; d.u[3 * (size_t)gv + 1] = u.y; d.u[3 * (size_t)gv + 2] = u.z;
   }
-  __syncthreads();
-  float* part = d.dq_part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (size_t)d.maxJ * 8;
-  for (int i = threadIdx.x; i < c.J * 8; i += kB) part[i] = s_acc[i];
+}
+
+// One wavefront per (character, joint): the joint's DQ gradient over its
+// (vertex, slot) entries, lane-strided then reduced across the wavefront.
+__global__ __launch_bounds__(kB) void joint_reduce_kernel(Dev d, IO io, int totJ) {
+  const int j = (blockIdx.x * kB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (j >= totJ) return;
+  int b = 0;
+  while (b + 1 < d.B && d.ch[b + 1].j0 <= j) ++b;
+  const int v0 = d.ch[b].v0;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int e = d.jv_off[j] + lane; e < d.jv_off[j + 1]; e += 64) {
+    const int ent = d.jv_ent[e];
+    const size_t gv = (size_t)v0 + (ent >> 2);
+    const float ws = d.vws[KS * gv + (ent & 3)];
+    const float* q8 = d.vq + 8 * gv;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] += ws * q8[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = wave_sum(acc[q]);
+  if (lane < 8) {
+    float r = acc[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) r = lane == q ? acc[q] : r;
+    io.g_dq[8 * (size_t)j + lane] = r;
+  }
 }
 
 __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
@@ -1145,8 +1173,8 @@ __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
   }
 }
 
-// One block per character: losses and joint-DQ gradients from the per-block
-// partial sums, each summed in a fixed order.
+// One block per character: the losses from the per-block partial sums, in a
+// fixed order.
 __global__ __launch_bounds__(kB) void finalize_kernel(Dev d, IO io) {
   __shared__ float s_red[kB / 64];
   const int b = blockIdx.x;
@@ -1162,12 +1190,6 @@ __global__ __launch_bounds__(kB) void finalize_kernel(Dev d, IO io) {
     s += d.loss_px[i];
   const float ls = block_sum(s, s_red);
   if (threadIdx.x == 0) { io.loss[2 * b] = lc; io.loss[2 * b + 1] = ls; }
-  const int nblk = (c.N + kB - 1) / kB;
-  for (int i = threadIdx.x; i < c.J * 8; i += kB) {
-    float a = 0.f;
-    for (int k = 0; k < nblk; ++k) a += d.dq_part[((size_t)b * d.gxN + k) * (size_t)d.maxJ * 8 + i];
-    io.g_dq[(size_t)c.j0 * 8 + i] = a;
-  }
 }
 
 template <class T>
@@ -1206,7 +1228,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   std::vector<CharD> ch(n_chars);
   std::vector<ViewD> vw;
   std::vector<float> rest, jw, ew;
-  std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list;
+  std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list, jv_off(1, 0), jv_ent;
   std::vector<float> n2v_w;
   long long vn = 0, vf = 0, pix = 0, sil = 0, tiles = 0, cells = 0;
   int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0, maxJ = 0, maxTiles = 0;
@@ -1267,6 +1289,15 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     for (int f = 0; f < d.F; ++f)
       for (int k = 0; k < 3; ++k) vf_list[vbase + vfill[c.faces[3 * f + k]]++] = f;
     for (int v = 0; v < d.N; ++v) vf_off.push_back(vbase + vc[v + 1]);
+    // joint -> (vertex, slot) CSR, entries in increasing (vertex, slot) order
+    std::vector<int> jc(d.J + 1, 0);
+    for (int i = 0; i < KS * d.N; ++i) jc[c.joint_idx[i] + 1]++;
+    for (int j = 0; j < d.J; ++j) jc[j + 1] += jc[j];
+    std::vector<int> jfill(jc.begin(), jc.end() - 1);
+    const int jbase = (int)jv_ent.size();
+    jv_ent.resize(jbase + KS * d.N);
+    for (int i = 0; i < KS * d.N; ++i) jv_ent[jbase + jfill[c.joint_idx[i]]++] = i;  // i = v * 4 + slot
+    for (int j = 0; j < d.J; ++j) jv_off.push_back(jbase + jc[j + 1]);
     for (int k = 0; k < d.C; ++k) {
       ViewD v{};
       v.ch = b; v.W = c.width; v.H = c.height; v.S = c.dt_size;
@@ -1311,14 +1342,16 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     if (e == hipSuccess) track((void*)*dst);
   };
   CharD* dch; ViewD* dvw;
-  float* drest; int *dfaces, *djidx, *denode, *dnv, *dn2o, *dn2v, *dvfo, *dvfl; float *djw, *dew, *dn2w;
+  float* drest; int *dfaces, *djidx, *denode, *dnv, *dn2o, *dn2v, *dvfo, *dvfl, *djvo, *djve;
+  float *djw, *dew, *dn2w;
   upload(&dch, ch); upload(&dvw, vw);
   upload(&drest, rest); upload(&dfaces, faces); upload(&djidx, jidx); upload(&djw, jw);
   upload(&denode, enode); upload(&dew, ew); upload(&dnv, node_vtx); upload(&dn2o, n2v_off);
   upload(&dn2v, n2v_vtx); upload(&dn2w, n2v_w); upload(&dvfo, vf_off); upload(&dvfl, vf_list);
+  upload(&djvo, jv_off); upload(&djve, jv_ent);
   d.ch = dch; d.vw = dvw; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
   d.enode = denode; d.ew = dew; d.node_vtx = dnv; d.n2v_off = dn2o; d.n2v_vtx = dn2v; d.n2v_w = dn2w;
-  d.vf_off = dvfo; d.vf_list = dvfl;
+  d.vf_off = dvfo; d.vf_list = dvfl; d.jv_off = djvo; d.jv_ent = djve;
   alloc(&d.node_tab, sizeof(float) * kNodeRow * (size_t)p->sumM);
   alloc(&d.vdef, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.vnrm, sizeof(float) * 3 * (size_t)p->sumN);
@@ -1346,7 +1379,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.loss_tile, sizeof(float) * (size_t)tiles);
   alloc(&d.loss_pt, sizeof(float) * (size_t)p->V * d.gxN);
   alloc(&d.loss_px, sizeof(float) * (size_t)p->V * d.gxS);
-  alloc(&d.dq_part, sizeof(float) * (size_t)n_chars * d.gxN * maxJ * 8);
+  alloc(&d.vq, sizeof(float) * 8 * (size_t)p->sumN);
+  alloc(&d.vws, sizeof(float) * KS * (size_t)p->sumN);
   if (e != hipSuccess) {
     pipe_free(p);
     return set_error(DCX_ERR_OOM, "dcx_pipeline_create: %s", hipGetErrorString(e));
@@ -1372,7 +1406,7 @@ static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "project",    "tri_count",
     "tile_scan",      "tri_scan",      "bin_scatter",      "fine",      "boundary",   "sil_point",
     "sil_grid_count", "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "face_sum",   "vertex_bwd",
-    "node_bwd",       "finalize"};
+    "joint_reduce",   "node_bwd",      "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
@@ -1437,6 +1471,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
   mark();
   hipLaunchKernelGGL(vertex_bwd_kernel, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
+  mark();
+  hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 64, kB)), dim3(kB), 0, s, d, io, p->sumJ);
   mark();
   hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
   mark();
