@@ -68,7 +68,7 @@ struct ViewD {
 struct Dev {
   int B, V;
   int maxN, maxF, maxM, maxS2, maxJ, maxTiles;
-  int gxN, gxS;           // blocks per (view|char) of the vertex and DT-pixel kernels
+  int gxN, gxS;           // blocks per (view|char) of the vertex kernels / DT-pixel tiles per view
   long long totVN, totVF, totPix, totTiles, totCells;
   const CharD* ch;
   const ViewD* vw;
@@ -937,63 +937,121 @@ __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
   d.cell_pts[d.cell_off[cell] + atomicAdd(d.cell_fill + cell, 1u)] = mk4(p.x, p.y, __int_as_float(v), 0.f);
 }
 
-// Data->model residual via the binned search + loss + gradient (atomic, as the reference).
+constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
+constexpr int kMaxRegionPts = 2048;
+constexpr int kMaxRegionRows = 64;
+
+__device__ __forceinline__ bool dt_boundary(const unsigned char* img, int S, int ui, int vi) {
+  if (img[(size_t)vi * S + ui] != 127) return false;
+  int counter = 0;
+  for (int j = -2; j <= 2; ++j)
+    for (int i = -2; i <= 2; ++i) {
+      const int yy = vi + j, xx = ui + i;
+      if (yy >= 0 && yy < S && xx >= 0 && xx < S && img[(size_t)yy * S + xx] == 127) counter++;
+    }
+  return counter >= 4;
+}
+
+__device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, float& bd, int& best, float& bx,
+                                             float& by) {
+  if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; bx = x; by = y; }
+}
+
+// Data->model residual (MultiViewSilhouetteLossGPUOp.cu:142-240) + loss +
+// gradient (atomic, as the reference).  One workgroup per 16x16 tile of DT
+// pixels: the boundary points of every grid cell within R of the tile (each
+// cell row of the region is one contiguous range of the grid) are copied to
+// LDS and each boundary pixel scans that list, keeping the minimum of
+// (distance, index) with distance < R -- the brute-force result, since every
+// point within R of the pixel is in the list.  Regions with more than
+// kMaxRegionPts points fall back to a ring search over the grid.
 __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
+  __shared__ f4 s_pts[kMaxRegionPts];
+  __shared__ unsigned int s_rstart[kMaxRegionRows + 1];
+  __shared__ unsigned int s_ra[kMaxRegionRows];
+  __shared__ unsigned int s_wave[kB / 64];
   __shared__ float s_red[kB / 64];
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const int S = vw.S;
-  const int p = blockIdx.x * kB + threadIdx.x;
+  const int tpr = (S + kDtTile - 1) / kDtTile;
+  const int tile = blockIdx.x;
+  const bool tile_ok = tile < tpr * tpr;
+  const int ui = (tile % tpr) * kDtTile + (int)(threadIdx.x % kDtTile);
+  const int vi = (tile / tpr) * kDtTile + (int)(threadIdx.x / kDtTile);
+  const unsigned char* img = io.dt + vw.sil0;
+  const bool bnd = tile_ok && ui < S && vi < S && dt_boundary(img, S, ui, vi);
   float lsum = 0.f;
-  if (p < S * S) {
-    const int ui = p % S, vi = p / S;
-    const unsigned char* img = io.dt + vw.sil0;
-    bool bnd = img[(size_t)vi * S + ui] == 127;
-    if (bnd) {
-      int counter = 0;
-      for (int j = -2; j <= 2; ++j)
-        for (int i = -2; i <= 2; ++i) {
-          const int yy = vi + j, xx = ui + i;
-          if (yy >= 0 && yy < S && xx >= 0 && xx < S && img[(size_t)yy * S + xx] == 127) counter++;
-        }
-      bnd = counter >= 4;
+  if (__syncthreads_or(bnd)) {
+    const float* cr = io.crops + (size_t)w * 7;
+    const float sc = cr[2];
+    const float R = 30.f * sc;
+    const GridGeomP gg = grid_geom_p(cr);
+    const int tu0 = (tile % tpr) * kDtTile, tv0 = (tile / tpr) * kDtTile;
+    const float xlo = ((float)tu0 - cr[0]) * sc + cr[3] - R, xhi = ((float)(tu0 + kDtTile - 1) - cr[0]) * sc + cr[3] + R;
+    const float ylo = ((float)tv0 - cr[1]) * sc + cr[4] - R, yhi = ((float)(tv0 + kDtTile - 1) - cr[1]) * sc + cr[4] + R;
+    const int cx0 = max(0, (int)floorf((xlo - gg.ox) / gg.cell) - 1);
+    const int cx1 = min(vw.gx - 1, (int)floorf((xhi - gg.ox) / gg.cell) + 1);
+    const int cy0 = max(0, (int)floorf((ylo - gg.oy) / gg.cell) - 1);
+    const int cy1 = min(vw.gy - 1, (int)floorf((yhi - gg.oy) / gg.cell) + 1);
+    const int nrows = cy1 - cy0 + 1;
+    unsigned int len = 0;
+    if ((int)threadIdx.x < nrows && threadIdx.x < kMaxRegionRows && cx0 <= cx1) {
+      const long long rc = vw.cell0 + (long long)(cy0 + (int)threadIdx.x) * vw.gx;
+      const unsigned int a = d.cell_off[rc + cx0];
+      len = d.cell_off[rc + cx1 + 1] - a;
+      s_ra[threadIdx.x] = a;
     }
+    unsigned int total;
+    const unsigned int ex = block_scan256(len, s_wave, total);
+    if (threadIdx.x < kMaxRegionRows) s_rstart[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_rstart[kMaxRegionRows] = total;
+    __syncthreads();
+    const bool in_lds = nrows > 0 && nrows <= kMaxRegionRows && cx0 <= cx1 && total <= (unsigned int)kMaxRegionPts;
+    if (in_lds) {
+      for (unsigned int i = threadIdx.x; i < total; i += kB) {
+        int r = 0;
+        while (r + 1 < nrows && s_rstart[r + 1] <= i) ++r;
+        s_pts[i] = d.cell_pts[s_ra[r] + (i - s_rstart[r])];
+      }
+    }
+    __syncthreads();
     if (bnd) {
-      const float* cr = io.crops + (size_t)w * 7;
-      const float sc = cr[2];
       const float ix = ((float)ui - cr[0]) * sc + cr[3], iy = ((float)vi - cr[1]) * sc + cr[4];
-      const float R = 30.f * sc;
-      float bd = R;
+      float bd = R, bx = 0.f, by = 0.f;
       int best = -1;
-      const GridGeomP gg = grid_geom_p(cr);
-      const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
-      float bx = 0.f, by = 0.f;
-      // Rings of cells at Chebyshev distance k around the pixel's cell.  A
-      // point in ring k + 1 or beyond is at least k cells away along one
-      // axis, so once the best distance is below that bound (with a margin
-      // for float rounding) no later point can beat or tie it; the result is
-      // the brute-force minimum of (distance, index) with distance < R.
-      for (int k = 0; k <= kRing; ++k) {
-        for (int oy = -k; oy <= k; ++oy) {
-          const int yy = cy + oy;
-          if (yy < 0 || yy >= vw.gy) continue;
-          const int step = (oy == -k || oy == k) ? 1 : 2 * k;
-          for (int ox = -k; ox <= k; ox += (step > 0 ? step : 1)) {
-            const int xx = cx + ox;
-            if (xx < 0 || xx >= vw.gx) continue;
-            const long long cell = vw.cell0 + yy * vw.gx + xx;
-            const unsigned int q1 = d.cell_off[cell + 1];
-            for (unsigned int q = d.cell_off[cell]; q < q1; ++q) {
-              const f4 it = d.cell_pts[q];
-              const int v = __float_as_int(it.z);
-              const float dx = it.x - ix, dy = it.y - iy;
-              const float dd = sqrtf(dx * dx + dy * dy);
-              if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; bx = it.x; by = it.y; }
+      if (in_lds) {
+        for (unsigned int i = 0; i < total; ++i) {
+          const f4 it = s_pts[i];
+          const float dx = it.x - ix, dy = it.y - iy;
+          nearest_take(sqrtf(dx * dx + dy * dy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+        }
+      } else {
+        // rings of cells at Chebyshev distance k around the pixel's cell; a
+        // point in ring k + 1 or beyond is at least k cells away along one
+        // axis, so the search stops once the best distance is below that
+        // bound (with a margin for float rounding)
+        const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
+        for (int k = 0; k <= kRing; ++k) {
+          for (int oy = -k; oy <= k; ++oy) {
+            const int yy = cy + oy;
+            if (yy < 0 || yy >= vw.gy) continue;
+            const int step = (oy == -k || oy == k) ? 1 : 2 * k;
+            for (int ox = -k; ox <= k; ox += (step > 0 ? step : 1)) {
+              const int xx = cx + ox;
+              if (xx < 0 || xx >= vw.gx) continue;
+              const long long cell = vw.cell0 + yy * vw.gx + xx;
+              const unsigned int q1 = d.cell_off[cell + 1];
+              for (unsigned int q = d.cell_off[cell]; q < q1; ++q) {
+                const f4 it = d.cell_pts[q];
+                const float dx = it.x - ix, dy = it.y - iy;
+                nearest_take(sqrtf(dx * dx + dy * dy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+              }
             }
           }
+          const float reach = (float)k * gg.cell * (1.f - 1e-4f);
+          if (bd < reach || reach >= R) break;
         }
-        const float reach = (float)k * gg.cell * (1.f - 1e-4f);
-        if (bd < reach || reach >= R) break;
       }
       if (best >= 0) {
         const float r0 = (bx - ix) / (30.f * sc), r1 = (by - iy) / (30.f * sc);
@@ -1320,7 +1378,12 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   d.B = n_chars; d.V = p->V; d.maxN = maxN; d.maxF = maxF; d.maxM = maxM; d.maxS2 = maxS2; d.maxJ = maxJ;
   d.maxTiles = maxTiles;
   d.gxN = (int)ceil_div(maxN, kB);
-  d.gxS = (int)ceil_div(maxS2, kB);
+  {
+    int maxS = 0;
+    for (const ViewD& v : vw) maxS = std::max(maxS, v.S);
+    const int tpr = (maxS + kDtTile - 1) / kDtTile;
+    d.gxS = tpr * tpr;  // DT-pixel tiles per view
+  }
   d.totVN = vn; d.totVF = vf; d.totPix = pix; d.totTiles = tiles; d.totCells = cells;
   const long long cap = bin_capacity > 0 ? bin_capacity : std::max<long long>(4 * vf, 1 << 20);
   if (cap >= (1ll << 32) || vf >= (1ll << 31) || (long long)p->V * maxTiles >= (1ll << 31)) {
