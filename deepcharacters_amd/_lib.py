@@ -42,6 +42,10 @@ SIGNATURES: dict[str, str] = {
     "dcx_pipeline_capture": "ppS",
     "dcx_pipeline_replay": "pS",
     "dcx_pipeline_profile": "pppiS",
+    "dcx_skeleton_create": "iipppppipppp",
+    "dcx_skeleton_destroy": "p",
+    "dcx_fk": "pippppS",
+    "dcx_dof_influence": "piippipp",
 }
 
 # Functions whose return type is not dcx_status.

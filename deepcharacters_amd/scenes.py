@@ -401,3 +401,70 @@ def capsule_body(target_verts: int, K: int = 4, seed: int = 0, ed_nodes: int = 5
     ch.texcoords = uv_atlas(f)
     ch.meta["parts"] = [p[0] for p in _BODY_PARTS]
     return ch
+
+
+# ----------------------------------------------------------------------------
+# Kinematic skeleton (DOFs) for the capsule body
+# ----------------------------------------------------------------------------
+
+_BODY_PARENTS = [-1, 0, 1, 2, 2, 4, 5, 2, 7, 8, 0, 10, 11, 0, 13, 14, 12, 15]
+JOINT_REVOLUTE, JOINT_PRISMATIC = 0, 1
+
+
+def body_skeleton(centres: np.ndarray | None = None) -> dict:
+    """57-DOF kinematic tree over the 18 capsule-body joints.
+
+    Root: three prismatic (x, y, z) then three revolute (x, y, z) joints at the
+    pelvis; every other body joint: a ball of three revolute joints (x, y, z)
+    offset from its parent body joint.  One DOF per kinematic joint (57).
+    Skinning joint i is the last kinematic joint of body joint i with bind
+    pose T(-centre_i), so zero DOFs give identity transforms.
+    Returns dict(parent, type, dof, offset, axis, skin_joint, skin_scale,
+    skin_bind, num_dofs, names).
+    """
+    c = np.asarray(_BODY_JOINTS if centres is None else centres, np.float64)
+    eye = np.eye(3)
+    parent, typ, dof, off, axis, names = [], [], [], [], [], []
+    last = {}
+
+    def add(p, t, o, a, name):
+        parent.append(p)
+        typ.append(t)
+        dof.append(len(dof))
+        off.append(o)
+        axis.append(a)
+        names.append(name)
+        return len(parent) - 1
+
+    j = -1
+    for k in range(3):
+        j = add(j, JOINT_PRISMATIC, c[0] if k == 0 else np.zeros(3), eye[k], f"root_t{'xyz'[k]}")
+    for k in range(3):
+        j = add(j, JOINT_REVOLUTE, np.zeros(3), eye[k], f"root_r{'xyz'[k]}")
+    last[0] = j
+    for b in range(1, len(c)):
+        p = last[_BODY_PARENTS[b]]
+        j = p
+        for k in range(3):
+            j = add(j, JOINT_REVOLUTE, (c[b] - c[_BODY_PARENTS[b]]) if k == 0 else np.zeros(3), eye[k],
+                    f"j{b}_r{'xyz'[k]}")
+        last[b] = j
+    bind = np.zeros((len(c), 12))
+    for b in range(len(c)):
+        bind[b] = [1, 0, 0, -c[b, 0], 0, 1, 0, -c[b, 1], 0, 0, 1, -c[b, 2]]
+    return dict(parent=np.array(parent, np.int32), type=np.array(typ, np.int32), dof=np.array(dof, np.int32),
+                offset=np.array(off), axis=np.array(axis), skin_joint=np.array([last[b] for b in range(len(c))],
+                                                                              np.int32),
+                skin_scale=np.ones(len(c)), skin_bind=bind, num_dofs=len(dof), names=names)
+
+
+def random_dofs(rng: np.random.Generator, skel: dict, B: int, max_angle: float = 0.3,
+                trans_sigma: float = 0.02) -> np.ndarray:
+    """[B,D] DOFs: revolute angles U(-max_angle, max_angle), prismatic N(0, sigma)."""
+    D = int(skel["num_dofs"])
+    out = np.zeros((B, D))
+    for j, (t, d) in enumerate(zip(skel["type"], skel["dof"])):
+        if d < 0:
+            continue
+        out[:, d] = rng.uniform(-max_angle, max_angle, B) if t == JOINT_REVOLUTE else rng.normal(0, trans_sigma, B)
+    return out

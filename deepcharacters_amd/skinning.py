@@ -12,6 +12,10 @@ from __future__ import annotations
 
 import torch
 
+import ctypes
+
+import numpy as np
+
 from . import _lib
 from ._torch import dev, opt, ptr, stream_ptr
 
@@ -147,3 +151,92 @@ class DQSkinFn(torch.autograd.Function):
         if g_rest is not None:
             g_rest = g_rest.sum(0)
         return g_dq, g_w, g_rest, None
+
+
+# ----------------------------------------------------------------------------
+# Skeleton forward kinematics: the DOF-input form of DqSkinningGpu
+# ----------------------------------------------------------------------------
+
+class Skeleton:
+    """Device-resident kinematic tree (dcx_skeleton); `desc` as returned by
+    scenes.body_skeleton (parent, type, dof, offset, axis, skin_joint,
+    skin_scale, skin_bind, num_dofs)."""
+
+    def __init__(self, desc: dict):
+        self.desc = desc
+        self.J = len(desc["parent"])
+        self.D = int(desc["num_dofs"])
+        self.Js = len(desc["skin_joint"])
+        a32 = lambda a, t=np.float32: np.ascontiguousarray(a, dtype=t)
+        self._keep = [a32(desc["parent"], np.int32), a32(desc["type"], np.int32), a32(desc["dof"], np.int32),
+                      a32(desc["offset"]), a32(desc["axis"]), a32(desc["skin_joint"], np.int32),
+                      a32(desc.get("skin_scale", np.ones(self.Js))), a32(desc["skin_bind"])]
+        p = [a.ctypes.data_as(ctypes.c_void_p) for a in self._keep]
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        st = lib.dcx_skeleton_create(self.J, self.D, p[0], p[1], p[2], p[3], p[4], self.Js, p[5], p[6], p[7],
+                                     ctypes.byref(h))
+        if st != 0:
+            raise _lib.DcxError("dcx_skeleton_create", st, lib.dcx_last_error().decode())
+        self.handle = h
+
+    def influence(self, joint_idx: np.ndarray, weights: np.ndarray, max_entries: int | None = None) -> np.ndarray:
+        """The DqSkinningGpuGrad DOF influence table [D, E, 4] for a static skinning."""
+        ji = np.ascontiguousarray(joint_idx, np.int32)
+        w = np.ascontiguousarray(weights, np.float32)
+        N, K = ji.shape
+        need = ctypes.c_int(0)
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _lib.call("dcx_dof_influence", self.handle, N, K, vp(ji), vp(w), 1, None, ctypes.byref(need))
+        E = max(1, need.value if max_entries is None else max_entries)
+        table = np.zeros((self.D, E, 4), np.float32)
+        _lib.call("dcx_dof_influence", self.handle, N, K, vp(ji), vp(w), E, vp(table), ctypes.byref(need))
+        return table
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().dcx_skeleton_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def skeleton_fk(skeleton: Skeleton, dofs: torch.Tensor):
+    """dofs [B,D] -> (dual_quaternions [B,Js,8], global_joint_position [B,J,3],
+    global_joint_axis [B,J,3])."""
+    d = dev(dofs, "dofs")
+    B = d.shape[0]
+    if d.shape[1] != skeleton.D:
+        raise ValueError(f"dofs has {d.shape[1]} columns, the skeleton {skeleton.D}")
+    dq = torch.empty((B, skeleton.Js, 8), device=d.device)
+    jp = torch.empty((B, skeleton.J, 3), device=d.device)
+    ja = torch.empty((B, skeleton.J, 3), device=d.device)
+    _lib.call("dcx_fk", skeleton.handle, B, ptr(d), ptr(dq), ptr(jp), ptr(ja), stream_ptr())
+    return dq, jp, ja
+
+
+def dq_skinning_gpu_dofs(skeleton: Skeleton, dofs: torch.Tensor, skinning_weights: torch.Tensor,
+                         joint_idx: torch.Tensor, rest_pos: torch.Tensor, rest_nrm: torch.Tensor | None = None):
+    """DqSkinningGpu (DQSkinningGPUOp.cpp:5-15): (dofs, skinning_weights) ->
+    (skinned_vertex_positions, skinned_vertex_normals, global_joint_position,
+    global_joint_axis, dual_quaternions, output_skinning_weights), with the
+    forward kinematics on the GPU."""
+    dq, jp, ja = skeleton_fk(skeleton, dofs)
+    pos, nrm, dq_out, w_out = dq_skinning_gpu(dq, skinning_weights, joint_idx, rest_pos, rest_nrm)
+    return pos, nrm, jp, ja, dq_out, w_out
+
+
+def dq_skinning_gpu_grad_dofs(skinned_vertex_positions_grad, skinned_vertex_positions, global_joint_position,
+                              global_joint_axis, dual_quaternions, skinning_weights, joint_idx, rest_pos,
+                              influence: torch.Tensor, ref_compat: bool = True):
+    """DqSkinningGpuGrad (DQSkinningGPUOpGrad.cpp:5-15): (dofs_grad [B,D],
+    skinning_weights_grad [B,N,K]).  dofs_grad is the reference's rigid-joint
+    approximation over the influence table; the weight gradient follows the
+    reference formula by default (ref_compat) or the exact adjoint."""
+    g_dofs = dof_grad(skinned_vertex_positions_grad, skinned_vertex_positions, global_joint_position,
+                      global_joint_axis, influence)
+    _, g_w, _ = dq_skinning_backward(skinned_vertex_positions_grad, dual_quaternions, skinning_weights, joint_idx,
+                                     rest_pos, need_dq=False, ref_compat=ref_compat)
+    return g_dofs, g_w

@@ -332,6 +332,44 @@ dcx_status dcx_mesh_boundary(const int32_t* face, const float* points_image, int
                              int H, int W, uint8_t* is_boundary, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Skeleton forward kinematics (the host FK of DqSkinningGpu)                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct dcx_skeleton dcx_skeleton;
+
+#define DCX_JOINT_REVOLUTE 0  /* rotation about the joint axis by the DOF value */
+#define DCX_JOINT_PRISMATIC 1 /* translation along the joint axis by the DOF value */
+
+/* Kinematic tree, HOST arrays, joints ordered parents-first: parent[J] (-1 =
+ * root), type[J], dof[J] (driving DOF, -1 = fixed), offset[J,3] (translation
+ * from the parent frame), axis[J,3] (unit, joint frame).  Skinning joint i is
+ * carried by joint skin_joint[i] with uniform scale skin_scale[i] (NULL = 1)
+ * and bind-pose inverse skin_bind[i] (3x4 row-major):
+ *   T_i = G_joint * diag(scale) * T_bind.
+ * Replaces the Skeletool skeleton + character setup that DqSkinningGpu parses
+ * from its character file (DQSkinningGPUOp.cpp:30-60). */
+dcx_status dcx_skeleton_create(int num_joints, int num_dofs, const int32_t* parent, const int32_t* type,
+                               const int32_t* dof, const float* offset, const float* axis, int num_skin_joints,
+                               const int32_t* skin_joint, const float* skin_scale, const float* skin_bind,
+                               dcx_skeleton** out);
+dcx_status dcx_skeleton_destroy(dcx_skeleton* skeleton);
+
+/* dofs [B,D] -> skinning-joint dual quaternions [B,Js,8], global joint
+ * positions [B,J,3] and axes [B,J,3]: the per-batch host FK of
+ * DqSkinningGpu (DQSkinningGPUOp.cpp:257-347, DualQuaternion.h:58-69) and
+ * its global_joint_position / global_joint_axis outputs. */
+dcx_status dcx_fk(const dcx_skeleton* skeleton, int B, const float* dofs, float* dual_quaternions, float* joint_pos,
+                  float* joint_axis, void* stream);
+
+/* HOST: the DOF influence table of DqSkinningGpuGrad
+ * (DQSkinningGPUOpGrad.cpp:90-217): rows (vertex, joint type, joint, weight),
+ * max_entries per DOF, -1 padded, from the static skinning (joint_idx /
+ * weights [N,K], skinning-joint ids).  *entries_needed receives the largest
+ * row count; table may be NULL to query it. */
+dcx_status dcx_dof_influence(const dcx_skeleton* skeleton, int N, int K, const int32_t* joint_idx,
+                             const float* weights, int max_entries, float* table, int* entries_needed);
+
+/* ------------------------------------------------------------------------ */
 /* Batched character pipeline (fused forward + backward, graph-capturable)  */
 /* ------------------------------------------------------------------------ */
 

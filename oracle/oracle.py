@@ -299,3 +299,45 @@ def boundary_flags(proj_uv, covered):
     out = np.zeros((B, C, N), np.uint8)
     _call("orc_boundary_flags", _c(proj_uv, np.float32), _c(covered, np.uint8), B, C, N, H, W, out)
     return out.astype(bool)
+
+
+# ----------------------------------------------------------------------------
+# Skeleton forward kinematics and the DOF influence table
+# ----------------------------------------------------------------------------
+
+def fk(skel, dofs, dtype=np.float64):
+    """(dual quaternions [B,Js,8], joint positions [B,J,3], joint axes [B,J,3]).
+    skel: dict with parent, type, dof, offset, axis, skin_joint, skin_scale, skin_bind."""
+    dofs = np.atleast_2d(np.asarray(dofs))
+    B, D = dofs.shape
+    J = len(skel["parent"])
+    Js = len(skel["skin_joint"])
+    dq = np.zeros((B, Js, 8), dtype)
+    jp = np.zeros((B, J, 3), dtype)
+    ja = np.zeros((B, J, 3), dtype)
+    scale = skel.get("skin_scale")
+    if scale is None:
+        scale = np.ones(Js)
+    _call(f"orc_fk_{_suf(dtype)}", B, J, D, _c(skel["parent"], np.int32), _c(skel["type"], np.int32),
+          _c(skel["dof"], np.int32), _c(skel["offset"], dtype), _c(skel["axis"], dtype), Js,
+          _c(skel["skin_joint"], np.int32), _c(scale, dtype), _c(skel["skin_bind"], dtype), _c(dofs, dtype), dq, jp,
+          ja)
+    return dq, jp, ja
+
+
+def dof_influence(skel, joint_idx, weights, max_entries=None):
+    """The DqSkinningGpuGrad influence table [D, E, 4] (-1 padded)."""
+    ji = _c(joint_idx, np.int32)
+    N, K = ji.shape
+    w = _c(weights, np.float32)
+    J = len(skel["parent"])
+    D = int(max(np.max(skel["dof"]) + 1, skel.get("num_dofs", 0)))
+    lib().orc_dof_influence.restype = ctypes.c_int
+    args = [N, K, ji, w, len(skel["skin_joint"]), _c(skel["skin_joint"], np.int32), J, _c(skel["parent"], np.int32),
+            _c(skel["type"], np.int32), _c(skel["dof"], np.int32), D]
+    conv = lambda a: ctypes.c_int(a) if isinstance(a, int) else _p(a)
+    most = lib().orc_dof_influence(*[conv(a) for a in args], ctypes.c_int(1), None)
+    E = max(1, most if max_entries is None else max_entries)
+    table = np.zeros((D, E, 4), np.float32)
+    lib().orc_dof_influence(*[conv(a) for a in args], ctypes.c_int(E), _p(table))
+    return table

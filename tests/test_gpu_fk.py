@@ -1,0 +1,71 @@
+"""GPU forward kinematics (DqSkinningGpu's host FK moved on device) and the
+DOF-input DqSkinningGpu / DqSkinningGpuGrad pair against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from deepcharacters_amd import scenes, skinning
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _align(dq, ref):
+    """Per joint, the DQ sign closest to the reference (q and -q are the same rotation)."""
+    s = np.sign(np.sum(dq[..., :4] * ref[..., :4], axis=-1, keepdims=True))
+    return dq * np.where(s == 0, 1, s)
+
+
+def test_fk_matches_oracle():
+    desc = scenes.body_skeleton()
+    sk = skinning.Skeleton(desc)
+    dofs = scenes.random_dofs(np.random.default_rng(0), desc, 5, 1.2, 0.1)
+    dq, jp, ja = skinning.skeleton_fk(sk, _t(dofs))
+    rdq, rjp, rja = orc.fk(desc, dofs.astype(np.float32).astype(np.float64))
+    assert _rel(_align(dq.cpu().numpy(), rdq), rdq) < 1e-5
+    assert _rel(jp.cpu().numpy(), rjp) < 1e-5 and _rel(ja.cpu().numpy(), rja) < 1e-5
+
+
+def test_dof_skinning_pair_matches_oracle():
+    desc = scenes.body_skeleton()
+    sk = skinning.Skeleton(desc)
+    ch = scenes.capsule_body(4000, seed=2, ed_nodes=0)
+    rng = np.random.default_rng(1)
+    B = 3
+    dofs = scenes.random_dofs(rng, desc, B, 0.5, 0.03).astype(np.float32)
+    w = np.broadcast_to(ch.joint_w, (B,) + ch.joint_w.shape).astype(np.float32)
+    pos, nrm, jp, ja, dq, w_out = skinning.dq_skinning_gpu_dofs(sk, _t(dofs), _t(w), _t(ch.joint_idx, torch.int32),
+                                                                _t(ch.verts), _t(ch.normals))
+    rdq, rjp, rja = orc.fk(desc, dofs.astype(np.float64))
+    rpos, rnrm = orc.dqs_fwd(rdq, w, ch.joint_idx, ch.verts, ch.normals)
+    assert _rel(pos.cpu().numpy(), rpos) < 1e-5 and _rel(nrm.cpu().numpy(), rnrm) < 1e-5
+    assert torch.equal(w_out, _t(w))
+    # the influence table built by the library equals the oracle's, row for row
+    tab = sk.influence(ch.joint_idx, ch.joint_w)
+    assert np.array_equal(tab, orc.dof_influence(desc, ch.joint_idx, ch.joint_w))
+    g = rng.normal(size=(B, ch.N, 3)).astype(np.float32)
+    g_dofs, g_w = skinning.dq_skinning_gpu_grad_dofs(_t(g), pos, jp, ja, dq, _t(w), _t(ch.joint_idx, torch.int32),
+                                                     _t(ch.verts), _t(tab))
+    ref_gd = orc.dof_grad(g, pos.cpu().numpy(), jp.cpu().numpy(), ja.cpu().numpy(), tab)
+    assert _rel(g_dofs.cpu().numpy(), ref_gd) < 1e-4
+    _, ref_gw, _ = orc.dqs_bwd(g, dq.cpu().numpy(), w, ch.joint_idx, ch.verts, ref_compat=True)
+    assert _rel(g_w.cpu().numpy(), ref_gw) < 1e-4
+
+
+def test_fk_rejects_bad_trees():
+    desc = scenes.body_skeleton()
+    bad = dict(desc)
+    bad["parent"] = desc["parent"].copy()
+    bad["parent"][3] = 10   # a parent after its child
+    from deepcharacters_amd import _lib
+    with pytest.raises(_lib.DcxError):
+        skinning.Skeleton(bad)

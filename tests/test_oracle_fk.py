@@ -1,0 +1,147 @@
+"""Pins the forward-kinematics oracle (the host FK of DqSkinningGpu,
+DQSkinningGPUOp.cpp:257-347) and the DOF influence table
+(DQSkinningGPUOpGrad.cpp:90-217): identity at zero DOFs, rigid rotation about
+a joint, root translation, quaternion branches, chain invariants, influence
+structure, and the rigid DOF gradient against FP64 finite differences of
+FK + DQ skinning for rigidly attached vertices."""
+import numpy as np
+import pytest
+
+from deepcharacters_amd import scenes
+from oracle import oracle as orc
+
+
+def rot(axis, ang):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def dq_apply(dq, p):
+    q, d = dq[:4], dq[4:]
+    R = scenes.quat_to_mat(q[None])[0]
+    # t = 2 * d * conj(q)
+    t = 2 * scenes.quat_mul(d[None], (q * np.array([-1, -1, -1, 1]))[None])[0][:3]
+    return p @ R.T + t
+
+
+def test_zero_dofs_give_identity_and_rest_joint_positions():
+    sk = scenes.body_skeleton()
+    dq, jp, ja = orc.fk(sk, np.zeros((1, sk["num_dofs"])))
+    np.testing.assert_allclose(dq[0, :, :4], np.tile([0, 0, 0, 1.0], (18, 1)), atol=1e-15)
+    np.testing.assert_allclose(dq[0, :, 4:], 0, atol=1e-15)
+    np.testing.assert_allclose(jp[0, sk["skin_joint"]], scenes._BODY_JOINTS, atol=1e-15)
+    np.testing.assert_allclose(np.linalg.norm(ja[0], axis=1), 1, atol=1e-15)
+
+
+@pytest.mark.parametrize("k,ang", [(0, 0.7), (1, -1.1), (2, 2.5)])
+def test_single_revolute_dof_rotates_the_subtree_about_the_joint(k, ang):
+    sk = scenes.body_skeleton()
+    names = sk["names"]
+    dofs = np.zeros((1, sk["num_dofs"]))
+    dofs[0, sk["dof"][names.index(f"j5_r{'xyz'[k]}")]] = ang   # left elbow
+    dq, jp, _ = orc.fk(sk, dofs)
+    c5 = scenes._BODY_JOINTS[5]
+    pts = np.random.default_rng(0).normal(size=(20, 3))
+    R = rot(np.eye(3)[k], ang)
+    for b in (5, 6):   # elbow and wrist move rigidly about the elbow
+        np.testing.assert_allclose(dq_apply(dq[0, b], pts), (pts - c5) @ R.T + c5, atol=1e-12)
+    for b in (0, 1, 2, 4, 7, 10):   # the rest of the body does not move
+        np.testing.assert_allclose(dq_apply(dq[0, b], pts), pts, atol=1e-12)
+    np.testing.assert_allclose(jp[0, sk["skin_joint"][6]], (scenes._BODY_JOINTS[6] - c5) @ R.T + c5, atol=1e-12)
+
+
+def test_root_translation_and_rotation():
+    sk = scenes.body_skeleton()
+    dofs = np.zeros((1, sk["num_dofs"]))
+    dofs[0, :3] = [0.1, -0.2, 0.3]
+    dofs[0, 4] = 0.9   # root rotation about y
+    dq, _, _ = orc.fk(sk, dofs)
+    pts = np.random.default_rng(1).normal(size=(10, 3))
+    c0 = scenes._BODY_JOINTS[0]
+    want = (pts - c0) @ rot([0, 1, 0], 0.9).T + c0 + [0.1, -0.2, 0.3]
+    for b in range(18):
+        np.testing.assert_allclose(dq_apply(dq[0, b], pts), want, atol=1e-12)
+
+
+def test_quaternion_extraction_all_branches():
+    """Bind rotations chosen so that each of the four branches (w, x, y, z
+    largest) is taken; the DQ rotation must reproduce the matrix."""
+    rng = np.random.default_rng(2)
+    for q in ([0.1, 0.2, 0.3, 0.9], [0.95, 0.1, -0.2, 0.05], [0.1, -0.9, 0.3, 0.2], [0.2, 0.1, 0.97, -0.05]):
+        q = np.array(q) / np.linalg.norm(q)
+        R = scenes.quat_to_mat(q[None])[0]
+        sk = dict(parent=np.array([-1], np.int32), type=np.array([0], np.int32), dof=np.array([0], np.int32),
+                  offset=np.zeros((1, 3)), axis=np.array([[0, 0, 1.0]]), skin_joint=np.array([0], np.int32),
+                  skin_scale=np.ones(1), skin_bind=np.concatenate([R, rng.normal(size=(3, 1))], 1).reshape(1, 12),
+                  num_dofs=1)
+        dq, _, _ = orc.fk(sk, np.zeros((1, 1)))
+        got = scenes.quat_to_mat(dq[0, :, :4])[0]
+        np.testing.assert_allclose(got, R, atol=1e-12)
+        assert abs(np.linalg.norm(dq[0, 0, :4]) - 1) < 1e-12
+
+
+def test_chain_invariants_random_pose():
+    """Under any pose: bone lengths are preserved and axes stay unit."""
+    sk = scenes.body_skeleton()
+    rng = np.random.default_rng(3)
+    dofs = scenes.random_dofs(rng, sk, 4, 1.0, 0.1)
+    _, jp, ja = orc.fk(sk, dofs)
+    sj = sk["skin_joint"]
+    rest = scenes._BODY_JOINTS
+    for b, p in enumerate(scenes._BODY_PARENTS):
+        if p < 0:
+            continue
+        L = np.linalg.norm(rest[b] - rest[p])
+        np.testing.assert_allclose(np.linalg.norm(jp[:, sj[b]] - jp[:, sj[p]], axis=1), L, atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(ja, axis=2), 1, atol=1e-12)
+
+
+def test_influence_table_structure():
+    sk = scenes.body_skeleton()
+    ch = scenes.capsule_body(1500, seed=0, ed_nodes=0)
+    tab = orc.dof_influence(sk, ch.joint_idx, ch.joint_w)
+    D, E, _ = tab.shape
+    N, K = ch.joint_idx.shape
+    rows = lambda d: tab[d][tab[d, :, 0] >= 0]
+    # every skinning joint hangs below the root: root DOFs see every (vertex, slot)
+    for d in range(6):
+        r = rows(d)
+        assert len(r) == N * K
+        assert np.array_equal(r[:, 0], np.repeat(np.arange(N), K))
+        assert set(r[:, 1].tolist()) == ({1.0} if d < 3 else {0.0})
+    # the left-toe ball's DOFs see exactly the slots skinned to the left toe
+    toe = [sk["dof"][i] for i, n in enumerate(sk["names"]) if n.startswith("j16_")]
+    want = np.argwhere(ch.joint_idx == 16)
+    for d in toe:
+        r = rows(d)
+        assert np.array_equal(r[:, 0], want[:, 0]) and np.allclose(r[:, 3], ch.joint_w[want[:, 0], want[:, 1]])
+
+
+def test_rigid_dof_gradient_matches_fd_for_rigidly_attached_vertices():
+    """One-hot skinning: the reference's rigid DOF gradient is the exact
+    derivative of FK + DQ skinning."""
+    sk = scenes.body_skeleton()
+    ch = scenes.capsule_body(800, seed=1, ed_nodes=0)
+    ji = ch.joint_idx.copy()
+    w = np.zeros_like(ch.joint_w)
+    w[:, 0] = 1.0
+    rng = np.random.default_rng(4)
+    dofs = scenes.random_dofs(rng, sk, 1, 0.6, 0.05)
+    g = rng.normal(size=(1, ch.N, 3))
+
+    def energy(dv):
+        dq, _, _ = orc.fk(sk, dv)
+        pos, _ = orc.dqs_fwd(dq, w[None], ji, ch.verts)
+        return float(np.sum(pos * g))
+
+    dq, jp, ja = orc.fk(sk, dofs)
+    pos, _ = orc.dqs_fwd(dq, w[None], ji, ch.verts)
+    tab = orc.dof_influence(sk, ji, w)
+    gd = orc.dof_grad(g, pos, jp, ja, tab)
+    h = 1e-6
+    for d in list(range(0, 57, 5)) + [56]:
+        e = np.zeros_like(dofs)
+        e[0, d] = h
+        fd = (energy(dofs + e) - energy(dofs - e)) / (2 * h)
+        assert abs(gd[0, d] - fd) <= 1e-6 * max(1.0, abs(fd)), (d, gd[0, d], fd)
