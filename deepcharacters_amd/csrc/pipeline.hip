@@ -544,7 +544,8 @@ __device__ __forceinline__ long long xcd_tile(long long wg, long long n) {
   return x * q + (x < r ? x : r) + i;
 }
 
-constexpr int kFast = 256;          // tiles with at most this many triangles take the pixel-parallel path
+constexpr int kFast = 256;          // tiles with at most this many triangles stage their list in LDS
+constexpr int kFaceHash = 512;      // open-addressing table face -> index in the tile list
 constexpr int kGPad = kTilePix + 1;  // row stride of the per-pixel gradient table (bank spread)
 
 // Sums of the per-pixel gradient table over the pixels each (triangle, tile)
@@ -572,6 +573,8 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
     struct { unsigned int wcnt[kTilePix / 64][kFast]; unsigned short order[kTilePix]; } srt;
   } u2;
   __shared__ unsigned long long s_key[kTilePix];
+  __shared__ int s_hkey[kFaceHash];             // face -> list index (fast path)
+  __shared__ unsigned short s_hval[kFaceHash];
   __shared__ unsigned int s_start[kFast + 1];
   __shared__ unsigned int s_e[kFast];
   __shared__ int s_fj[kFast];
@@ -608,31 +611,33 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
   int f = -1;   // winning face of this pixel
   int bj = -1;  // fast path: index of the winning triangle in the tile list
   if (fast) {
+    // stage the tile's triangle records in LDS and index their faces
+    s_key[threadIdx.x] = kEmptyKey;
+    for (int i = threadIdx.x; i < kFaceHash; i += kTilePix) s_hkey[i] = -1;
+    __syncthreads();
     if (threadIdx.x < n_items) {
       const unsigned int ri = d.bins[off + threadIdx.x];
       const TriRec t = d.tri[ri];
       u1.tri[threadIdx.x] = t;
-      s_fj[threadIdx.x] = t.face_flags & kFaceMask;
+      const int face = t.face_flags & kFaceMask;
+      s_fj[threadIdx.x] = face;
       const TileRange r = tile_range(t);
       s_e[threadIdx.x] = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
+      // faces are distinct within a tile list; at most kFast <= kFaceHash / 2
+      // entries, so linear probing always finds a free slot
+      unsigned int h = ((unsigned int)face * 2654435761u) >> 23;
+      for (int pr = 0; pr < kFaceHash; ++pr, h = (h + 1) & (kFaceHash - 1))
+        if (atomicCAS(s_hkey + h, -1, face) == -1) { s_hval[h] = (unsigned short)threadIdx.x; break; }
     }
     __syncthreads();
-    // pixel-parallel coverage: the minimum packed (depth, face) key over the
-    // tile's triangles, with the same exact edge functions as the
-    // triangle-parallel path
-    if (inimg) {
-      unsigned long long best = kEmptyKey;
-      for (int j = 0; j < (int)n_items; ++j) {
-        const TriRec& t = u1.tri[j];
-        if (x < (t.bx0_by0 & 0xffff) || x > (t.bx1_by1 & 0xffff) || y < (t.bx0_by0 >> 16) || y > (t.bx1_by1 >> 16))
-          continue;
-        EdgeWalk ew;
-        ew.init(t, x, y);
-        if (!ew.inside(ew.e)) continue;
-        const unsigned long long k = make_key(interp_iz(ew.e[0], ew.e[1], ew.e[2], t.iz), t.face_flags & kFaceMask);
-        if (k < best) { best = k; bj = j; }
-      }
-      if (bj >= 0) f = (int)(best & 0xffffffffull);
+    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) raster_in_tile(u1.tri[k], tx0, ty0, s_key);
+    __syncthreads();
+    const unsigned long long key = s_key[threadIdx.x];
+    if (inimg && key != kEmptyKey) {
+      f = (int)(key & 0xffffffffull);
+      unsigned int h = ((unsigned int)f * 2654435761u) >> 23;
+      for (int pr = 0; pr < kFaceHash; ++pr, h = (h + 1) & (kFaceHash - 1))
+        if (s_hkey[h] == f) { bj = s_hval[h]; break; }
     }
     __syncthreads();  // u1.tri is overwritten by the gradient table below
   } else {
