@@ -469,6 +469,15 @@ dcx_status render_fwd(const dcx_topo* t, const dcx_raster_cfg* c, int B, const f
   return check_launch("dcx_render_fwd(binned)");
 }
 
+dcx_status vertex_normals(const dcx_topo* t, int B, int C, const float* vpos, float* fnrm, float* vnrm,
+                          float* vn_out, hipStream_t s) {
+  hipLaunchKernelGGL(face_normal_kernel, dim3(ceil_div((long long)B * t->F, kB)), dim3(kB), 0, s, vpos, t->faces,
+                     t->N, t->F, B, fnrm);
+  hipLaunchKernelGGL(vertex_normal_kernel, dim3(ceil_div((long long)B * t->N, kB)), dim3(kB), 0, s, fnrm,
+                     t->vf_off, t->vf_list, t->N, t->F, B, C, vnrm, vn_out);
+  return check_launch("vertex_normals");
+}
+
 }  // namespace dcx
 
 using namespace dcx;
@@ -480,7 +489,7 @@ extern "C" dcx_status dcx_topo_create(const int32_t* faces, int F, int N, const 
 
 extern "C" dcx_status dcx_topo_destroy(dcx_topo* t) {
   if (!t) return DCX_OK;
-  hipFree(t->faces); hipFree(t->texcoords); hipFree(t->vf_off); hipFree(t->vf_list);
+  hipFree(t->faces); hipFree(t->texcoords); hipFree(t->vf_off); hipFree(t->vf_list); hipFree(t->texel_tab);
   delete t;
   return DCX_OK;
 }

@@ -9,6 +9,8 @@ struct dcx_topo {
   float* texcoords = nullptr; // [F,6] or null
   int* vf_off = nullptr;      // [N+1] vertex -> face CSR offsets
   int* vf_list = nullptr;     // [3F] face ids, increasing per vertex
+  float4* texel_tab = nullptr; // [tab_h, tab_w] (face, a, b, c) of the UV atlas, built on request
+  int tab_h = 0, tab_w = 0;
 };
 
 namespace dcx {
@@ -52,6 +54,10 @@ struct WsLayout {
 };
 
 size_t render_ws_fixed(const dcx_topo* t, const dcx_raster_cfg* c, int B, WsLayout* L);
+// Area-weighted, unnormalised vertex normals of B meshes (vnrm [B,N,3]; face
+// normal scratch fnrm [B,F,3]); also broadcast into vn_out [B,C,N,3] if given.
+dcx_status vertex_normals(const dcx_topo* t, int B, int C, const float* vpos, float* fnrm, float* vnrm,
+                          float* vn_out, hipStream_t s);
 dcx_status validate_cfg(const dcx_topo* t, const dcx_raster_cfg* c, int B);
 
 }  // namespace dcx

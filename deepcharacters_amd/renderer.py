@@ -112,6 +112,37 @@ def render_forward(topo, cfg, vertex_pos, vertex_color, texture, sh_coeff, extri
     return bary, face, render, vnorm
 
 
+NORMAL_MAP_CODES = {"normal": 1, "position": 2, "face": 3}
+
+
+def build_texel_table(topo: Topology, tex_h: int, tex_w: int):
+    """Texel -> (face, barycentrics) table of the UV atlas, kept in the topology."""
+    if not topo.has_texcoords:
+        raise ValueError("compute_normal_map needs texture coordinates")
+    _lib.call("dcx_topo_texel_table", topo.handle, tex_h, tex_w, stream_ptr())
+
+
+def render_normal_map(topo, cfg, mode: str, vertex_pos, tex_h: int, tex_w: int):
+    """CudaRendererGpu for compute_normal_map in {'normal', 'position', 'face'}:
+    (barycentric, face, render, vertex_normal, normal_map [B,TH,TW,3]); no
+    rasterisation, buffers cleared (CUDABasedRasterization.cu:567-575)."""
+    vp = dev(vertex_pos, "vertex_pos")
+    B, N = vp.shape[0], vp.shape[1]
+    C, H, W = cfg.num_cameras, cfg.height, cfg.width
+    d = vp.device
+    build_texel_table(topo, tex_h, tex_w)
+    bary = torch.empty((B, C, H, W, 2), device=d)
+    face = torch.empty((B, C, H, W), device=d, dtype=torch.int32)
+    render = torch.empty((B, C, H, W, 3), device=d)
+    vnorm = torch.empty((B, C, N, 3), device=d)
+    nmap = torch.empty((B, tex_h, tex_w, 3), device=d)
+    nbytes = _lib.load().dcx_normal_map_workspace_size(topo.handle, B)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=d)
+    _lib.call("dcx_render_normal_map", topo.handle, ctypes.byref(cfg), B, NORMAL_MAP_CODES[mode], ptr(vp), tex_h,
+              tex_w, ptr(bary), ptr(face), ptr(render), ptr(vnorm), ptr(nmap), ptr(ws), nbytes, stream_ptr())
+    return bary, face, render, vnorm, nmap
+
+
 def render_backward(topo, cfg, render_grad, vertex_pos, vertex_color, texture, sh_coeff, target_image,
                     vertex_normal, barycentric, face, extrinsics, intrinsics):
     """(vertex_pos_grad, vertex_color_grad, texture_grad, sh_coeff_grad) — CudaRendererGradGpu."""
@@ -193,8 +224,6 @@ class CudaRendererGpu:
             raise ValueError(f"INVALID SHADING MODE {shadingMode_attr!r}")
         if compute_normal_map_attr not in NORMAL_MAP_MODES:
             raise ValueError(f"INVALID Normal MODE {compute_normal_map_attr!r}")
-        if compute_normal_map_attr != "none":
-            raise NotImplementedError("compute_normal_map modes other than 'none' are not built yet")
         shading = shadingMode_attr
         if albedoMode_attr not in ("vertexColor", "textured"):
             shading = "shadeless"  # CudaRenderer.cpp:71-74
@@ -219,8 +248,16 @@ class CudaRendererGpu:
         tgt = targetImage_input
         ext = torch.as_tensor(extrinsics_input, dtype=torch.float32, device=d)
         intr = torch.as_tensor(intrinsics_input, dtype=torch.float32, device=d)
-        self.cudaRendererOperator = _RenderFn.apply(vertexPos_input, vertexColor_input, tex, sh, tgt, ext, intr,
-                                                    self.topology, self.cfg, albedoMode_attr)
+        if compute_normal_map_attr != "none":
+            # normal-map modes skip rasterisation; like the reference's registered
+            # gradient, nothing flows back through the UV-space map
+            TH, TW = tex.shape[1], tex.shape[2]
+            with torch.no_grad():
+                outs = render_normal_map(self.topology, self.cfg, compute_normal_map_attr, vertexPos_input, TH, TW)
+            self.cudaRendererOperator = outs
+        else:
+            self.cudaRendererOperator = _RenderFn.apply(vertexPos_input, vertexColor_input, tex, sh, tgt, ext, intr,
+                                                        self.topology, self.cfg, albedoMode_attr)
 
     def getBaryCentricBufferTF(self):
         return self.cudaRendererOperator[0]
@@ -233,6 +270,12 @@ class CudaRendererGpu:
 
     def getRenderBufferTF(self):
         return self.cudaRendererOperator[2]
+
+    def getNormalMap(self):
+        """normal_map [B,TH,TW,3] (only when compute_normal_map is not 'none')."""
+        if self.compute_normal_map_attr == "none":
+            return None
+        return self.cudaRendererOperator[4]
 
     def getModelMaskTF(self):
         face = self.cudaRendererOperator[1]

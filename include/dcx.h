@@ -202,6 +202,30 @@ dcx_status dcx_render_fwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B
                           float* render, float* vertex_normal, void* workspace,
                           size_t workspace_bytes, void* stream);
 
+/* compute_normal_map attribute values other than 'none' */
+#define DCX_NORMAL_MAP_NORMAL 1   /* 'normal': interpolated unit vertex normals, renormalised */
+#define DCX_NORMAL_MAP_POSITION 2 /* 'position': interpolated vertex positions */
+#define DCX_NORMAL_MAP_FACE 3     /* 'face': the face id in all three channels */
+
+/* Builds (once per size; the topology keeps the last one) the texel ->
+ * (face, barycentrics) table of the UV atlas for a tex_h x tex_w texture:
+ * the host table of CUDABasedRasterization.cpp:270-327, built on the GPU.
+ * Synchronises the stream.  Not safe concurrently with renders of the same
+ * topology. */
+dcx_status dcx_topo_texel_table(dcx_topo* topo, int tex_h, int tex_w, void* stream);
+size_t dcx_normal_map_workspace_size(const dcx_topo* topo, int B);
+
+/*
+ * Forward of CudaRendererGpu for compute_normal_map = 'normal' | 'position' |
+ * 'face' (CUDABasedRasterization.cu:458-549,567-575): no rasterisation (the
+ * barycentric / face / render buffers are cleared: 0 / -1 / 0), vertex
+ * normals as dcx_render_fwd, and normal_map [B,TH,TW,3] from the texel table.
+ */
+dcx_status dcx_render_normal_map(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B, int mode,
+                                 const float* vertex_pos, int tex_h, int tex_w, float* barycentric,
+                                 int32_t* face, float* render, float* vertex_normal, float* normal_map,
+                                 void* workspace, size_t workspace_bytes, void* stream);
+
 /*
  * Backward of the rasterizer: CudaRendererGradGpu (CudaRendererGrad.cpp:6-34,
  * 241-274 -> CUDABasedRasterizationGrad.cu:114-514) for albedo vertexColor,

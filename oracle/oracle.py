@@ -341,3 +341,27 @@ def dof_influence(skel, joint_idx, weights, max_entries=None):
     table = np.zeros((D, E, 4), np.float32)
     lib().orc_dof_influence(*[conv(a) for a in args], ctypes.c_int(E), _p(table))
     return table
+
+
+# ----------------------------------------------------------------------------
+# compute_normal_map modes
+# ----------------------------------------------------------------------------
+
+NORMAL_MAP_MODES = {"normal": 1, "position": 2, "face": 3}
+
+
+def texel_table(texcoords, TH, TW):
+    """[TH, TW, 4] (face, a, b, c) of the UV atlas; texcoords [F, 6]."""
+    tc = _c(texcoords, np.float32).reshape(-1, 6)
+    out = np.zeros((TH, TW, 4), np.float32)
+    _call("orc_texel_table", tc.shape[0], tc, TW, TH, out)
+    return out
+
+
+def normal_map(table, faces, vpos, vnrm, mode):
+    """[TH, TW, 3] for one mesh (vpos, vnrm [N,3] float32)."""
+    TH, TW = table.shape[0], table.shape[1]
+    out = np.zeros((TH, TW, 3), np.float32)
+    _call("orc_normal_map", _c(table, np.float32), TW, TH, _c(faces, np.int32).reshape(-1, 3),
+          _c(vpos, np.float32), _c(vnrm, np.float32), NORMAL_MAP_MODES[mode], out)
+    return out
