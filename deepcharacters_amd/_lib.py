@@ -48,17 +48,23 @@ SIGNATURES: dict[str, str] = {
     "dcx_dof_influence": "piippipp",
     "dcx_topo_texel_table": "piiS",
     "dcx_render_normal_map": "ppiipiippppppzS",
+    "dcx_edt": "piiippzS",
+    "dcx_dt_encode": "piiippzS",
+    "dcx_dtloss_fwd": "ppppiiiipppzS",
+    "dcx_dtloss_bwd": "ppppiiiippS",
 }
 
 # Functions whose return type is not dcx_status.
 _RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_size": ctypes.c_size_t,
              "dcx_mvsil_workspace_size": ctypes.c_size_t, "dcx_pipeline_num_stages": ctypes.c_int,
-             "dcx_pipeline_stage_name": ctypes.c_char_p, "dcx_normal_map_workspace_size": ctypes.c_size_t}
+             "dcx_pipeline_stage_name": ctypes.c_char_p, "dcx_normal_map_workspace_size": ctypes.c_size_t,
+             "dcx_edt_workspace_size": ctypes.c_size_t, "dcx_dtloss_workspace_size": ctypes.c_size_t}
 _EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
                    "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int],
                    "dcx_mvsil_workspace_size": [ctypes.c_int] * 5, "dcx_pipeline_num_stages": [],
                    "dcx_pipeline_stage_name": [ctypes.c_int],
-                   "dcx_normal_map_workspace_size": [ctypes.c_void_p, ctypes.c_int]}
+                   "dcx_normal_map_workspace_size": [ctypes.c_void_p, ctypes.c_int],
+                   "dcx_edt_workspace_size": [ctypes.c_int] * 3, "dcx_dtloss_workspace_size": [ctypes.c_int] * 2}
 
 _CODES = {
     "p": ctypes.c_void_p,

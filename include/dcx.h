@@ -356,6 +356,38 @@ dcx_status dcx_mesh_boundary(const int32_t* face, const float* points_image, int
                              int H, int W, uint8_t* is_boundary, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Distance transforms and the native DT silhouette loss                     */
+/* ------------------------------------------------------------------------ */
+
+/* No counterpart op in the reference, which loads DT images computed offline
+ * (CreateTFRecordDataset.py:65-67).  mask [V,H,W] uint8 (non-zero =
+ * foreground).  dcx_edt: dist [V,H,W] = exact Euclidean distance from each
+ * pixel centre to the nearest foreground pixel centre (0 on foreground, +inf
+ * if the view has none).  dcx_dt_encode: the uint8 DT image the silhouette op
+ * reads: 127 on the contour band (foreground pixels with a background or
+ * out-of-image 4-neighbour, background pixels with a foreground 4-neighbour),
+ * clamp(rint(d_in) - 1, 0, 126) inside, 127 + clamp(rint(d_out / 2), 1, 128)
+ * outside.  Both need dcx_edt_workspace_size bytes of workspace. */
+size_t dcx_edt_workspace_size(int V, int H, int W);
+dcx_status dcx_edt(const uint8_t* mask, int V, int H, int W, float* dist, void* workspace, size_t workspace_bytes,
+                   void* stream);
+dcx_status dcx_dt_encode(const uint8_t* mask, int V, int H, int W, uint8_t* dt_out, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
+/* Native DT silhouette loss per view: L[v] = sum over points (boundary
+ * points when is_boundary is given) of D(q)^2, D the bilinear interpolation
+ * of dist [V,H,W] (pixel i's centre at i + 0.5) at q = (p - crop_min) / scale
+ * + border_offset (crops [V,7] as the silhouette op's); points whose 2x2
+ * footprint leaves the image contribute 0.  points [V,N,2]; residual [V,N]
+ * (may be NULL).  bwd: points_grad = g_loss[v] * 2 D grad D / scale. */
+size_t dcx_dtloss_workspace_size(int V, int N);
+dcx_status dcx_dtloss_fwd(const float* points, const uint8_t* is_boundary, const float* dist, const float* crops,
+                          int V, int N, int H, int W, float* residual, float* loss, void* workspace,
+                          size_t workspace_bytes, void* stream);
+dcx_status dcx_dtloss_bwd(const float* points, const uint8_t* is_boundary, const float* dist, const float* crops,
+                          int V, int N, int H, int W, const float* g_loss, float* points_grad, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Skeleton forward kinematics (the host FK of DqSkinningGpu)                */
 /* ------------------------------------------------------------------------ */
 
