@@ -25,7 +25,7 @@ SIGNATURES: dict[str, str] = {
     "dcx_topo_create": "piipp",
     "dcx_topo_destroy": "p",
     "dcx_render_fwd": "ppipppiippppppppzS",
-    "dcx_render_bwd": "ppippppiipppppppppppS",
+    "dcx_render_bwd": "ppippppiippppppppppppzS",
     "dcx_edgraph_create": "piippppipppp",
     "dcx_edgraph_destroy": "p",
     "dcx_ed_fwd": "pippppppppppppzS",
@@ -58,13 +58,15 @@ SIGNATURES: dict[str, str] = {
 _RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_size": ctypes.c_size_t,
              "dcx_mvsil_workspace_size": ctypes.c_size_t, "dcx_pipeline_num_stages": ctypes.c_int,
              "dcx_pipeline_stage_name": ctypes.c_char_p, "dcx_normal_map_workspace_size": ctypes.c_size_t,
-             "dcx_edt_workspace_size": ctypes.c_size_t, "dcx_dtloss_workspace_size": ctypes.c_size_t}
+             "dcx_edt_workspace_size": ctypes.c_size_t, "dcx_dtloss_workspace_size": ctypes.c_size_t,
+             "dcx_render_bwd_workspace_size": ctypes.c_size_t}
 _EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
                    "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int],
                    "dcx_mvsil_workspace_size": [ctypes.c_int] * 5, "dcx_pipeline_num_stages": [],
                    "dcx_pipeline_stage_name": [ctypes.c_int],
                    "dcx_normal_map_workspace_size": [ctypes.c_void_p, ctypes.c_int],
-                   "dcx_edt_workspace_size": [ctypes.c_int] * 3, "dcx_dtloss_workspace_size": [ctypes.c_int] * 2}
+                   "dcx_edt_workspace_size": [ctypes.c_int] * 3, "dcx_dtloss_workspace_size": [ctypes.c_int] * 2,
+                   "dcx_render_bwd_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]}
 
 _CODES = {
     "p": ctypes.c_void_p,

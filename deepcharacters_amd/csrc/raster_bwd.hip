@@ -9,6 +9,14 @@
 // last lane of each run issues the 18 global atomics.  SH gradients (27
 // floats per camera, hit by every pixel of that camera) are reduced per
 // block in LDS first.
+//
+// Shaded mode also differentiates the vertex normals (sums of the adjacent
+// unnormalised face normals, CUDABasedRasterizationGrad.cu:441-493): the
+// pixel kernel reduces dL/d(vertex normal) per vertex into a [B,N,3]
+// workspace (9 more floats in the same segmented scan), then one lane per
+// vertex walks its faces and applies the face-normal Jacobians
+// (RendererUtil.h:492-540) -- a deterministic gather in place of the
+// reference's per-pixel loop over the faces around each corner.
 #include "raster_common.hpp"
 #include "raster_internal.hpp"
 
@@ -37,6 +45,7 @@ struct BwdArgs {
   float* g_vcol;
   float* g_tex;
   float* g_sh;
+  float* g_vn;            // [B,N,3] shaded only
 };
 
 // imageGradient of RendererUtil.h:566-620 at a (possibly fractional) point:
@@ -115,7 +124,7 @@ __device__ void bary_vjp(const float* E, const float* K, const f3 X[3], float px
 // Segmented (by key) inclusive scan over a wavefront: afterwards the last lane
 // of each run of equal keys holds the run's sum.  Every lane must call it.
 template <int NV>
-__device__ __forceinline__ bool seg_reduce(float (&v)[NV], int key, int key2) {
+__device__ __forceinline__ bool seg_reduce(float* v, int key, int key2) {
   const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(key, 1, 64), prev2 = __shfl_up(key2, 1, 64);
   const int next = __shfl_down(key, 1, 64), next2 = __shfl_down(key2, 1, 64);
@@ -150,9 +159,9 @@ __global__ __launch_bounds__(kB) void render_bwd_kernel(BwdArgs a) {
   if (a.shaded && a.g_sh && threadIdx.x < 27) s_sh[threadIdx.x] = 0.f;
   if (a.shaded && a.g_sh) __syncthreads();
   const int f = valid ? a.face[i] : -1;
-  float acc[18];
+  float acc[27];  // 0-8 position, 9-17 colour, 18-26 vertex-normal gradients of the 3 corners
 #pragma unroll
-  for (int k = 0; k < 18; ++k) acc[k] = 0.f;
+  for (int k = 0; k < 27; ++k) acc[k] = 0.f;
   const int b = view / a.C;
   int i0 = 0, i1 = 0, i2 = 0;
   float shv[27];
@@ -257,7 +266,10 @@ __global__ __launch_bounds__(kB) void render_bwd_kernel(BwdArgs a) {
       const f3 gun = mk3((sg * gn.x - nn.x * dg) / pl, (sg * gn.y - nn.y * dg) / pl, (sg * gn.z - nn.z * dg) / pl);
       const float* vn = a.vnormal + (size_t)view * a.N * 3;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) gw[k] += dot(gun, ld3(vn + 3 * idx[k]));
+      for (int k = 0; k < 3; ++k) {
+        gw[k] += dot(gun, ld3(vn + 3 * idx[k]));
+        acc[18 + 3 * k] = w[k] * gun.x; acc[19 + 3 * k] = w[k] * gun.y; acc[20 + 3 * k] = w[k] * gun.z;
+      }
     }
     f3 X[3] = {ld3(vp + 3 * i0), ld3(vp + 3 * i1), ld3(vp + 3 * i2)};
     f3 gx[3];
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(kB) void render_bwd_kernel(BwdArgs a) {
     for (int k = 0; k < 3; ++k) { acc[3 * k] = gx[k].x; acc[3 * k + 1] = gx[k].y; acc[3 * k + 2] = gx[k].z; }
   }
   // Combine runs of equal (view, face) within the wavefront.
-  const bool tail = seg_reduce<18>(acc, f, view);
+  const bool tail = a.shaded ? seg_reduce<27>(acc, f, view) : seg_reduce<18>(acc, f, view);
   if (f >= 0 && tail) {
     float* gp = a.g_vpos + (size_t)b * a.N * 3;
     const int idx[3] = {i0, i1, i2};
@@ -307,6 +319,15 @@ __global__ __launch_bounds__(kB) void render_bwd_kernel(BwdArgs a) {
         atomicAdd(gc + 3 * idx[k] + 2, acc[11 + 3 * k]);
       }
     }
+    if (a.shaded) {
+      float* gn = a.g_vn + (size_t)b * a.N * 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        atomicAdd(gn + 3 * idx[k], acc[18 + 3 * k]);
+        atomicAdd(gn + 3 * idx[k] + 1, acc[19 + 3 * k]);
+        atomicAdd(gn + 3 * idx[k] + 2, acc[20 + 3 * k]);
+      }
+    }
   }
   if (a.shaded && a.g_sh) {
     if (block_one_view) {
@@ -323,10 +344,46 @@ __global__ __launch_bounds__(kB) void render_bwd_kernel(BwdArgs a) {
   }
 }
 
+// One lane per (mesh, vertex): n_v = sum over the faces around v (one entry
+// per corner) of N_f = (v1 - v0) x (v2 - v0).  With G_f = sum of dL/dn over
+// the three corners of f, dL/dv0 = (e1 - e2) x G_f, dL/dv1 = e2 x G_f,
+// dL/dv2 = G_f x e1 (e1 = v1 - v0, e2 = v2 - v0); v takes the terms of the
+// corners it occupies, once per face (a face listing v twice appears twice,
+// consecutively, in v's list).
+__global__ __launch_bounds__(kB) void normal_grad_kernel(const float* __restrict__ vpos, const int* __restrict__ faces,
+                                                         const int* __restrict__ off, const int* __restrict__ lst,
+                                                         const float* __restrict__ g_vn, int N, int B,
+                                                         float* __restrict__ g_vpos) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)B * N) return;
+  const int b = (int)(i / N), v = (int)(i - (long long)b * N);
+  const float* vp = vpos + (size_t)b * N * 3;
+  const float* gv = g_vn + (size_t)b * N * 3;
+  f3 s = mk3(0.f, 0.f, 0.f);
+  for (int k = off[v]; k < off[v + 1]; ++k) {
+    const int fi = lst[k];
+    if (k > off[v] && lst[k - 1] == fi) continue;
+    const int i0 = faces[3 * fi], i1 = faces[3 * fi + 1], i2 = faces[3 * fi + 2];
+    const f3 G = ld3(gv + 3 * i0) + ld3(gv + 3 * i1) + ld3(gv + 3 * i2);
+    const f3 p0 = ld3(vp + 3 * i0);
+    const f3 e1 = ld3(vp + 3 * i1) - p0, e2 = ld3(vp + 3 * i2) - p0;
+    if (i0 == v) s = s + cross(e1 - e2, G);
+    if (i1 == v) s = s + cross(e2, G);
+    if (i2 == v) s = s + cross(G, e1);
+  }
+  float* o = g_vpos + 3 * i;
+  o[0] += s.x; o[1] += s.y; o[2] += s.z;
+}
+
 }  // namespace
 }  // namespace dcx
 
 using namespace dcx;
+
+extern "C" size_t dcx_render_bwd_workspace_size(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B) {
+  if (!topo || !cfg || B <= 0) return 0;
+  return cfg->shading_mode == DCX_SHADING_SHADED ? sizeof(float) * 3 * (size_t)B * topo->N : 0;
+}
 
 extern "C" dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,
                                      const float* render_grad, const float* vertex_pos,
@@ -336,7 +393,8 @@ extern "C" dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg*
                                      const int32_t* face, const float* extrinsics,
                                      const float* intrinsics, float* vertex_pos_grad,
                                      float* vertex_color_grad, float* texture_grad,
-                                     float* sh_coeff_grad, void* stream) {
+                                     float* sh_coeff_grad, void* workspace, size_t workspace_bytes,
+                                     void* stream) {
   dcx_status st = validate_cfg(topo, cfg, B);
   if (st) return st;
   const int alb = cfg->albedo_mode;
@@ -351,6 +409,8 @@ extern "C" dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg*
               "dcx_render_bwd: textured mode needs texture + texcoords");
   DCX_REQUIRE(!(shaded || alb == ALB_TEXTURED) || vertex_normal, "dcx_render_bwd: vertex_normal required");
   DCX_REQUIRE(!shaded || sh_coeff, "dcx_render_bwd: shaded mode needs sh_coeff");
+  DCX_REQUIRE(!shaded || (workspace && workspace_bytes >= dcx_render_bwd_workspace_size(topo, cfg, B)),
+              "dcx_render_bwd: shaded mode needs a workspace of dcx_render_bwd_workspace_size() bytes");
   hipStream_t s = as_stream(stream);
   const int N = topo->N, C = cfg->num_cameras;
   DCX_HIP(hipMemsetAsync(vertex_pos_grad, 0, sizeof(float) * 3 * (size_t)B * N, s));
@@ -358,6 +418,7 @@ extern "C" dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg*
   if (texture_grad && tex_h > 0 && tex_w > 0)
     DCX_HIP(hipMemsetAsync(texture_grad, 0, sizeof(float) * 3 * (size_t)B * tex_h * tex_w, s));
   if (sh_coeff_grad) DCX_HIP(hipMemsetAsync(sh_coeff_grad, 0, sizeof(float) * 27 * (size_t)B * C, s));
+  if (shaded) DCX_HIP(hipMemsetAsync(workspace, 0, sizeof(float) * 3 * (size_t)B * N, s));
   BwdArgs a{};
   a.B = B; a.C = C; a.N = N; a.F = topo->F; a.H = cfg->height; a.W = cfg->width;
   a.albedo = alb; a.shaded = shaded ? 1 : 0; a.TH = tex_h; a.TW = tex_w;
@@ -368,7 +429,11 @@ extern "C" dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg*
   a.ext = extrinsics; a.intr = intrinsics;
   a.g_vpos = vertex_pos_grad; a.g_vcol = vertex_color_grad; a.g_tex = texture_grad;
   a.g_sh = shaded ? sh_coeff_grad : nullptr;
+  a.g_vn = shaded ? (float*)workspace : nullptr;
   const long long total = (long long)B * C * cfg->height * cfg->width;
   hipLaunchKernelGGL(render_bwd_kernel, dim3(ceil_div(total, kB)), dim3(kB), 0, s, a);
+  if (shaded)
+    hipLaunchKernelGGL(normal_grad_kernel, dim3(ceil_div((long long)B * N, kB)), dim3(kB), 0, s, vertex_pos,
+                       topo->faces, topo->vf_off, topo->vf_list, (const float*)workspace, N, B, vertex_pos_grad);
   return check_launch("dcx_render_bwd");
 }

@@ -165,10 +165,12 @@ def render_backward(topo, cfg, render_grad, vertex_pos, vertex_color, texture, s
     gsh = torch.empty((B, C, 27), device=d)
     if vc is None and cfg.albedo_mode != ALBEDO_MODES["textured"]:
         raise ValueError("vertex_color is required")
+    nbytes = _lib.load().dcx_render_bwd_workspace_size(topo.handle, ctypes.byref(cfg), B)
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=d)
     _lib.call("dcx_render_bwd", topo.handle, ctypes.byref(cfg), B, ptr(g), ptr(vp), ptr(vc), ptr(tx), TH, TW,
               ptr(sh), ptr(tgt), ptr(vn), ptr(dev(barycentric, "barycentric")),
               ptr(dev(face, "face", torch.int32)), ptr(ext), ptr(intr), ptr(gvp), ptr(gvc), ptr(gtx),
-              ptr(gsh), stream_ptr())
+              ptr(gsh), ptr(ws), nbytes, stream_ptr())
     if gtx is None:
         gtx = torch.zeros((B, 1, 1, 3), device=d)
     return gvp, gvc, gtx, gsh

@@ -234,12 +234,16 @@ dcx_status dcx_render_normal_map(const dcx_topo* topo, const dcx_raster_cfg* cfg
  * are the reference's: colour, texture (nearest texel) and SH gradients, the
  * position gradient through the barycentrics (and, shaded, through the
  * interpolated normal), plus the target-image "model-to-data" term when
- * target_image is given.  The shaded-mode derivative of the vertex normals
- * w.r.t. positions (adjacent-face loop) is not included.
+ * target_image is given.  Shaded mode also includes the derivative of the
+ * vertex normals w.r.t. the positions (the adjacent-face term of
+ * CUDABasedRasterizationGrad.cu:441-493), computed by a per-vertex gather.
  *   render_grad [B,C,H,W,3]; vertex_normal, barycentric, face: forward outputs
  *   outputs (overwritten): vertex_pos_grad [B,N,3], vertex_color_grad [B,N,3],
  *   texture_grad [B,TH,TW,3] (may be NULL), sh_coeff_grad [B,C,27] (may be NULL)
+ *   workspace: dcx_render_bwd_workspace_size() bytes (0, so NULL allowed, when
+ *   shadeless)
  */
+size_t dcx_render_bwd_workspace_size(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B);
 dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,
                           const float* render_grad, const float* vertex_pos,
                           const float* vertex_color, const float* texture, int tex_h, int tex_w,
@@ -247,7 +251,7 @@ dcx_status dcx_render_bwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B
                           const float* vertex_normal, const float* barycentric,
                           const int32_t* face, const float* extrinsics, const float* intrinsics,
                           float* vertex_pos_grad, float* vertex_color_grad, float* texture_grad,
-                          float* sh_coeff_grad, void* stream);
+                          float* sh_coeff_grad, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Embedded-graph deformation (EmbeddedGraphGpu / EmbeddedGraphGpuGrad)      */

@@ -18,7 +18,7 @@
 //   * resolve: barycentrics (a, b) of vertices 0 and 1 (third = 1-a-b), the
 //     albedo modes and SH shading of CUDABasedRasterization.cu:307-446.
 // Backward (CUDABasedRasterizationGrad.cu:114-437, vertexColor / textured /
-// foregroundMask albedo, shadeless or shaded without the adjacent-face normal
+// foregroundMask albedo, shadeless or shaded (including the adjacent-face normal
 // term): colour, SH, texture and position gradients including the
 // target-image "model-to-data" term.
 //
@@ -611,6 +611,9 @@ void raster_bwd(const BwdCfg& cf, const int32_t* faces, const T* tc, const T* g_
   std::memset(g_vcol, 0, sizeof(T) * (size_t)cf.B * cf.N * 3);
   if (g_tex) std::memset(g_tex, 0, sizeof(T) * (size_t)cf.B * cf.TH * cf.TW * 3);
   if (g_sh) std::memset(g_sh, 0, sizeof(T) * (size_t)cf.B * cf.C * 27);
+  // shaded: gradient w.r.t. the (unnormalised) vertex normals, passed on to
+  // the positions through the face normals after the pixel loop
+  std::vector<T> g_vn(cf.shading == SHADE_SHADED ? (size_t)cf.B * cf.N * 3 : 0, T(0));
   for (int b = 0; b < cf.B; ++b)
     for (int c = 0; c < cf.C; ++c) {
       const T* E = ext + (size_t)b * cf.C * 12 + 12 * c;
@@ -724,7 +727,10 @@ void raster_bwd(const BwdCfg& cf, const int32_t* faces, const T* tc, const T* g_
             T gun[3];
             for (int d = 0; d < 3; ++d) gun[d] = (sgn * gn[d] - nn[d] * dotgn) / pl;
             for (int i = 0; i < 3; ++i)
-              for (int d = 0; d < 3; ++d) gw[i] += gun[d] * vn[3 * fv[i] + d];
+              for (int d = 0; d < 3; ++d) {
+                gw[i] += gun[d] * vn[3 * fv[i] + d];
+                g_vn[((size_t)b * cf.N + fv[i]) * 3 + d] += w[i] * gun[d];
+              }
           }
           const T* Xs[3] = {vp + 3 * fv[0], vp + 3 * fv[1], vp + 3 * fv[2]};
           T gx[3][3];
@@ -754,6 +760,28 @@ void raster_bwd(const BwdCfg& cf, const int32_t* faces, const T* tc, const T* g_
             }
           }
         }
+    }
+  // vertex normal n_v = sum over the faces of v of N_f = (v1 - v0) x (v2 - v0):
+  // dL/dN_f = sum of dL/dn over the face's three corners, and
+  // dL/dv0 = (e1 - e2) x g, dL/dv1 = e2 x g, dL/dv2 = g x e1
+  if (cf.shading == SHADE_SHADED)
+    for (int b = 0; b < cf.B; ++b) {
+      const T* vp = vpos + (size_t)b * cf.N * 3;
+      const T* gv = g_vn.data() + (size_t)b * cf.N * 3;
+      T* gp = g_vpos + (size_t)b * cf.N * 3;
+      for (int f = 0; f < cf.F; ++f) {
+        const int32_t* fv = faces + 3 * f;
+        V3<T> gN = v3(T(0), T(0), T(0));
+        for (int i = 0; i < 3; ++i) gN = v3(gN.x + gv[3 * fv[i]], gN.y + gv[3 * fv[i] + 1], gN.z + gv[3 * fv[i] + 2]);
+        const V3<T> p0 = v3(vp[3 * fv[0]], vp[3 * fv[0] + 1], vp[3 * fv[0] + 2]);
+        const V3<T> e1 = v3(vp[3 * fv[1]] - p0.x, vp[3 * fv[1] + 1] - p0.y, vp[3 * fv[1] + 2] - p0.z);
+        const V3<T> e2 = v3(vp[3 * fv[2]] - p0.x, vp[3 * fv[2] + 1] - p0.y, vp[3 * fv[2] + 2] - p0.z);
+        const V3<T> d0 = cross(v3(e1.x - e2.x, e1.y - e2.y, e1.z - e2.z), gN), d1 = cross(e2, gN), d2 = cross(gN, e1);
+        const V3<T> dd[3] = {d0, d1, d2};
+        for (int i = 0; i < 3; ++i) {
+          gp[3 * fv[i]] += dd[i].x; gp[3 * fv[i] + 1] += dd[i].y; gp[3 * fv[i] + 2] += dd[i].z;
+        }
+      }
     }
 }
 

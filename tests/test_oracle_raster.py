@@ -264,6 +264,45 @@ def test_backward_matches_masked_finite_differences(albedo):
         assert abs(np.sum(g * (r2 - render)) - gvc.reshape(-1)[i]) < 1e-9
 
 
+def _random_sh(rng, C):
+    sh = np.zeros((1, C, 27))
+    for d in range(3):
+        sh[0, :, 9 * d] = 0.8
+        sh[0, :, 9 * d + 1:9 * d + 9] = rng.normal(scale=0.25, size=(C, 8))
+    return sh
+
+
+def test_shaded_backward_includes_vertex_normal_term():
+    """Shaded mode: the render depends on the positions through the
+    barycentrics AND through the vertex normals (sums of the adjacent face
+    normals, CUDABasedRasterizationGrad.cu:441-493).  FP64 central differences
+    over visibility-stable pixels pin both terms; with the adjacent-face term
+    left out the analytic gradient misses these differences by 1e-2 and more
+    relative, far outside the 1e-5 tolerance."""
+    ch, ext, intr = _scene_small(2)
+    H, W = 36, 40
+    V = ch.verts[None].copy()
+    col = ch.colors[None]
+    rng = np.random.default_rng(5)
+    kw = dict(vcol=col, albedo="vertexColor", shading="shaded", sh=_random_sh(rng, 2))
+    face, bary, render, vn = orc.raster_fwd(ch.faces, V, ext, intr, H, W, dtype=np.float64, **kw)
+    g = rng.normal(size=render.shape)
+    h = 1e-6
+    cand = np.unique(ch.faces[np.unique(face[face >= 0])].reshape(-1))
+    for v in rng.choice(cand, 12, replace=False):
+        for d in range(3):
+            i = 3 * int(v) + d
+            Vp = V.copy(); Vp.reshape(-1)[i] += h
+            Vm = V.copy(); Vm.reshape(-1)[i] -= h
+            fp, _, rp, _ = orc.raster_fwd(ch.faces, Vp, ext, intr, H, W, dtype=np.float64, **kw)
+            fm, _, rm, _ = orc.raster_fwd(ch.faces, Vm, ext, intr, H, W, dtype=np.float64, **kw)
+            gm = g * ((fp == face) & (fm == face))[..., None]
+            gv, _, _, _ = orc.raster_bwd(ch.faces, gm, V, ext, intr, face, bary, vn, **kw)
+            fd = np.sum(gm * (rp - rm)) / (2 * h)
+            an = gv.reshape(-1)[i]
+            assert abs(fd - an) <= 1e-5 * max(1.0, abs(fd)), (int(v), d, fd, an)
+
+
 def test_model_to_data_term_matches_projected_ramp_derivative():
     """With a target image linear in (x, y) and filter size 1, the reference's
     image-gradient kernel returns the exact ramp slope, so the model-to-data
