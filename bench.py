@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", default="auto", choices=["auto", "16", "32", "64"],
+                    help="raster tile width (tiles of w x 256/w pixels); auto = time each and keep the fastest")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -166,7 +168,14 @@ def main():
 
     bp = pipeline.BatchedPipeline(works)
     stream = torch.cuda.Stream()
+    tuned = None
     with torch.cuda.stream(stream):
+        if args.tile == "auto":
+            tuned = bp.autotune(reps=3)
+            log("autotune: " + ", ".join(f"{w}x{256 // w}={ms:.3f} ms" for w, ms in tuned["ms"].items())
+                + f" -> {tuned['tile_w']}x{256 // tuned['tile_w']}")
+        else:
+            bp.set_tile(int(args.tile))
         bp.step()
         stream.synchronize()
         bp.capture(stream)
@@ -230,7 +239,10 @@ def main():
                    "characters": len(works), "views_per_character": int(works[0].ext.shape[0]),
                    "resolution": [works[0].H, works[0].W], "vertices": int(sum(w.ch.N for w in works)),
                    "faces": int(sum(w.ch.F for w in works)),
-                   "ed_nodes": int(sum(w.ch.graph.num_nodes for w in works))},
+                   "ed_nodes": int(sum(w.ch.graph.num_nodes for w in works)),
+                   "tile": f"{bp.tile_width}x{256 // bp.tile_width}",
+                   "autotune_ms_per_tile": ({f"{w}x{256 // w}": ms for w, ms in tuned["ms"].items()}
+                                            if tuned else None)},
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "hbm", "kernel": "fine_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -30,8 +30,11 @@
 //                    and views
 //   vertex_bwd       per vertex: gather of its faces' sums, camera-projection
 //                    adjoint, DQ-skinning adjoint, colour gradient
-//   joint_reduce     one wavefront per joint: DQ gradient over its (vertex, slot)
-//                    entries (static CSR, fixed order: deterministic)
+//   joint_partial    one wavefront per chunk of <= 512 (vertex, slot) entries
+//                    of one joint (static CSR cut into chunks): DQ gradient
+//                    partial
+//   joint_reduce     per (joint, component): its chunk partials in chunk
+//                    order (fixed order throughout: deterministic)
 //   node_bwd         one wavefront per ED node: translation / Euler gradients
 //   finalize         per character: losses from the per-block partial sums
 #include <algorithm>
@@ -45,11 +48,11 @@ namespace {
 
 constexpr int kB = 256;
 constexpr int KS = 4;         // skinning joints per vertex
+constexpr int kJChunk = 512;  // (vertex, slot) entries per joint-reduction chunk
 constexpr int KE = 4;         // ED nodes per vertex
 constexpr int kNodeRow = 15;  // Rd(9) g(3) td(3)
 constexpr int kImgFs = 2;     // image_filter_size of the model-to-data term
 constexpr int kGrad = 18;     // per (triangle, tile) record: 3 vertices x (pos 3 + colour 3)
-constexpr int kApronW = kTileW + 2 * kImgFs, kApronH = kTileH + 2 * kImgFs;
 constexpr int kXcd = 8;       // L2 domains the workgroups are dealt round-robin to
 
 struct CharD {
@@ -68,6 +71,7 @@ struct ViewD {
 struct Dev {
   int B, V;
   int maxN, maxF, maxM, maxS2, maxJ, maxTiles;
+  int tws, ths;           // tile shape: (1 << tws) x (1 << ths) pixels, 256 in all
   int gxN, gxS;           // blocks per (view|char) of the vertex kernels / DT-pixel tiles per view
   long long totVN, totVF, totPix, totTiles, totCells;
   const CharD* ch;
@@ -87,6 +91,8 @@ struct Dev {
   const int* vf_list;     // local face ids, increasing per vertex
   const int* jv_off;      // [sumJ+1] global offsets: joint -> (vertex, slot) entries
   const int* jv_ent;      // local vertex * 4 + slot, increasing
+  const int4* jchunk;     // [nChunk] (first entry, end entry, character vertex base, joint)
+  const int* jc_off;      // [sumJ+1] joint -> its chunks
   // scratch
   float* node_tab;        // [sumM,15]
   float* vdef;            // [sumN,3] skinned vertices
@@ -117,6 +123,7 @@ struct Dev {
   float* loss_pt;         // [V*gxN] silhouette model->data loss per block
   float* loss_px;         // [V*gxS] silhouette data->model loss per block
   float* vq;              // [sumN,8] gradient w.r.t. the vertex's blended, normalised DQ
+  float* jpart;           // [nChunk,8] per-chunk joint DQ gradient partials
   float* vws;             // [sumN,4] skinning weight x blend sign per slot
 };
 
@@ -330,11 +337,11 @@ __global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
 // ---------------------------------------------------------------------------
 
 struct TileRange { int tx0, ty0, ntx, nty; };
-__device__ __forceinline__ TileRange tile_range(const TriRec& t) {
+__device__ __forceinline__ TileRange tile_range(const TriRec& t, int tws, int ths) {
   TileRange r;
   const int x0 = t.bx0_by0 & 0xffff, y0 = t.bx0_by0 >> 16, x1 = t.bx1_by1 & 0xffff, y1 = t.bx1_by1 >> 16;
-  r.tx0 = x0 / kTileW; r.ty0 = y0 / kTileH;
-  r.ntx = x1 / kTileW - r.tx0 + 1; r.nty = y1 / kTileH - r.ty0 + 1;
+  r.tx0 = x0 >> tws; r.ty0 = y0 >> ths;
+  r.ntx = (x1 >> tws) - r.tx0 + 1; r.nty = (y1 >> ths) - r.ty0 + 1;
   return r;
 }
 
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
       d.tri_cnt[ri] = 0u;
     } else {
       d.tri[ri] = t;
-      const TileRange r = tile_range(t);
+      const TileRange r = tile_range(t, d.tws, d.ths);
       d.tri_cnt[ri] = (unsigned int)(r.ntx * r.nty);
       const bool big = r.ntx * r.nty > kBigTri;
       for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
@@ -416,7 +423,7 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
   if (f < c.F) {
     const TriRec& t = d.tri[ri];
     if (t.bx0_by0 != 0x7fff7fff) {
-      r = tile_range(t);
+      r = tile_range(t, d.tws, d.ths);
       big = r.ntx * r.nty > kBigTri;
     }
   }
@@ -450,9 +457,10 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
 // Fine tile kernel
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, unsigned long long* s_key) {
+__device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, int tws, int ths,
+                                               unsigned long long* s_key) {
   const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
-  const int x1 = min(t.bx1_by1 & 0xffff, tx0 + kTileW - 1), y1 = min(t.bx1_by1 >> 16, ty0 + kTileH - 1);
+  const int x1 = min(t.bx1_by1 & 0xffff, tx0 + (1 << tws) - 1), y1 = min(t.bx1_by1 >> 16, ty0 + (1 << ths) - 1);
   if (x0 > x1 || y0 > y1) return;
   EdgeWalk w;
   w.init(t, x0, y0);
@@ -462,7 +470,7 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
     long long e[3] = {row[0], row[1], row[2]};
     for (int x = x0; x <= x1; ++x) {
       if (w.inside(e))
-        atomicMin(s_key + (y - ty0) * kTileW + (x - tx0), make_key(interp_iz(e[0], e[1], e[2], t.iz), face));
+        atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)), make_key(interp_iz(e[0], e[1], e[2], t.iz), face));
       e[0] += w.dx[0]; e[1] += w.dx[1]; e[2] += w.dx[2];
     }
     row[0] += w.dy[0]; row[1] += w.dy[1]; row[2] += w.dy[2];
@@ -564,12 +572,16 @@ __device__ __forceinline__ void emit_record(const Dev& d, const IO& io, const Ch
   }
 }
 
+// TWS: log2 of the tile width (tiles are (1 << TWS) x (1 << (8 - TWS)) pixels).
+template <int TWS>
 __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
+  constexpr int tws = TWS, ths = 8 - TWS;
+  constexpr int aw = (1 << tws) + 2 * kImgFs, an = aw * ((1 << ths) + 2 * kImgFs);  // apron row length, size
   // u1: triangle records of the tile (raster) / per-pixel gradient table (backward)
   __shared__ union U1 { TriRec tri[kFast]; float g[kGrad * kGPad]; } u1;
   // u2: target colours with apron (per-pixel phase) / counting-sort scratch (records)
   __shared__ union U2 {
-    float tgt[3 * kApronH * kApronW];
+    float tgt[3 * an];
     struct { unsigned int wcnt[kTilePix / 64][kFast]; unsigned short order[kTilePix]; } srt;
   } u2;
   __shared__ unsigned long long s_key[kTilePix];
@@ -588,24 +600,24 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
   if (lt >= vw.tiles) return;
   const CharD c = d.ch[vw.ch];
   const long long tile = vw.tile0 + lt;
-  const int tx0 = (lt % vw.tiles_x) * kTileW, ty0 = (lt / vw.tiles_x) * kTileH;
-  const int tx = tx0 / kTileW, ty = ty0 / kTileH;
+  const int tx = lt % vw.tiles_x, ty = lt / vw.tiles_x;
+  const int tx0 = tx << tws, ty0 = ty << ths;
   const unsigned int off = d.tile_off[tile], end = d.tile_off[tile + 1];
   const bool overflow = end > d.capacity;
   const unsigned int n_items = overflow ? (unsigned int)c.F : end - off;
   const bool fast = !overflow && n_items <= (unsigned int)kFast;  // uniform over the block
   const TriRec* tv = d.tri + vw.vf0;
-  const int px_i = threadIdx.x % kTileW, py_i = threadIdx.x / kTileW;
+  const int px_i = threadIdx.x & ((1 << tws) - 1), py_i = threadIdx.x >> tws;
   const int x = tx0 + px_i, y = ty0 + py_i;
   const bool inimg = x < vw.W && y < vw.H;
 
   // target colours of the tile plus a kImgFs apron (clamped reads; the apron is
   // only used by pixels whose whole filter window lies inside the image)
-  for (int i = threadIdx.x; i < kApronH * kApronW; i += kTilePix) {
-    const int ax = i % kApronW, ay = i / kApronW;
+  for (int i = threadIdx.x; i < an; i += kTilePix) {
+    const int ay = i / aw, ax = i - ay * aw;
     const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
     const f3 t = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
-    u2.tgt[i] = t.x; u2.tgt[kApronH * kApronW + i] = t.y; u2.tgt[2 * kApronH * kApronW + i] = t.z;
+    u2.tgt[i] = t.x; u2.tgt[an + i] = t.y; u2.tgt[2 * an + i] = t.z;
   }
 
   int f = -1;   // winning face of this pixel
@@ -621,7 +633,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       u1.tri[threadIdx.x] = t;
       const int face = t.face_flags & kFaceMask;
       s_fj[threadIdx.x] = face;
-      const TileRange r = tile_range(t);
+      const TileRange r = tile_range(t, tws, ths);
       s_e[threadIdx.x] = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
       // faces are distinct within a tile list; at most kFast <= kFaceHash / 2
       // entries, so linear probing always finds a free slot
@@ -630,7 +642,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         if (atomicCAS(s_hkey + h, -1, face) == -1) { s_hval[h] = (unsigned short)threadIdx.x; break; }
     }
     __syncthreads();
-    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) raster_in_tile(u1.tri[k], tx0, ty0, s_key);
+    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) raster_in_tile(u1.tri[k], tx0, ty0, tws, ths, s_key);
     __syncthreads();
     const unsigned long long key = s_key[threadIdx.x];
     if (inimg && key != kEmptyKey) {
@@ -644,7 +656,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
     s_key[threadIdx.x] = kEmptyKey;
     __syncthreads();
     for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix)
-      raster_in_tile(overflow ? tv[k] : d.tri[d.bins[off + k]], tx0, ty0, s_key);
+      raster_in_tile(overflow ? tv[k] : d.tri[d.bins[off + k]], tx0, ty0, tws, ths, s_key);
     __syncthreads();
     const unsigned long long key = s_key[threadIdx.x];
     f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
@@ -656,8 +668,8 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
   float lsum = 0.f;
   if (inimg) {
     const long long o = vw.pix0 + (long long)y * vw.W + x;
-    const int ta = (py_i + kImgFs) * kApronW + (px_i + kImgFs);
-    const f3 tgt = mk3(u2.tgt[ta], u2.tgt[kApronH * kApronW + ta], u2.tgt[2 * kApronH * kApronW + ta]);
+    const int ta = (py_i + kImgFs) * aw + (px_i + kImgFs);
+    const f3 tgt = mk3(u2.tgt[ta], u2.tgt[an + ta], u2.tgt[2 * an + ta]);
     if (f < 0) {
       io.face[o] = -1;
       io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
@@ -702,8 +714,8 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         float norm = 0.f;
         for (int yy = -fs; yy <= fs; ++yy)
           for (int xx = -fs; xx <= fs; ++xx) {
-            const int a = (py_i + kImgFs + yy) * kApronW + (px_i + kImgFs + xx);
-            const f3 I = mk3(u2.tgt[a], u2.tgt[kApronH * kApronW + a], u2.tgt[2 * kApronH * kApronW + a]);
+            const int a = (py_i + kImgFs + yy) * aw + (px_i + kImgFs + xx);
+            const f3 I = mk3(u2.tgt[a], u2.tgt[an + a], u2.tgt[2 * an + a]);
             const float den = (float)(xx * xx + yy * yy);
             float gu = 0.f, gvv = 0.f;
             if (den != 0.f) { gu = (float)xx / den; gvv = (float)yy / den; }
@@ -799,7 +811,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
     const TriRec t = tv[ri_local];
     if (t.bx0_by0 == 0x7fff7fff) continue;
     const int bx0 = max(t.bx0_by0 & 0xffff, tx0), by0 = max(t.bx0_by0 >> 16, ty0);
-    const int bx1 = min(t.bx1_by1 & 0xffff, tx0 + kTileW - 1), by1 = min(t.bx1_by1 >> 16, ty0 + kTileH - 1);
+    const int bx1 = min(t.bx1_by1 & 0xffff, tx0 + (1 << tws) - 1), by1 = min(t.bx1_by1 >> 16, ty0 + (1 << ths) - 1);
     if (bx0 > bx1 || by0 > by1) continue;
     const int face = t.face_flags & kFaceMask;
     float a[kGrad];
@@ -808,14 +820,14 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
     bool won = false;
     for (int yy = by0; yy <= by1; ++yy)
       for (int xx = bx0; xx <= bx1; ++xx) {
-        const int p = (yy - ty0) * kTileW + (xx - tx0);
+        const int p = ((yy - ty0) << tws) | (xx - tx0);
         if (s_face[p] == face) {
           won = true;
 #pragma unroll
           for (int q = 0; q < kGrad; ++q) a[q] += s_g[q * kGPad + p];
         }
       }
-    const TileRange r = tile_range(t);
+    const TileRange r = tile_range(t, d.tws, d.ths);
     const unsigned int e = d.tri_base[vw.vf0 + ri_local] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
 #pragma unroll
     for (int q = 0; q < kGrad; ++q) emit_record(d, io, c, e, face, q, a[q], won);
@@ -945,6 +957,7 @@ __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
 constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
 constexpr int kMaxRegionPts = 2048;
 constexpr int kMaxRegionRows = 64;
+constexpr int kMaxRegionCells = 2048;  // entries of the per-cell LDS offset table (rows x (cols + 1))
 
 __device__ __forceinline__ bool dt_boundary(const unsigned char* img, int S, int ui, int vi) {
   if (img[(size_t)vi * S + ui] != 127) return false;
@@ -964,14 +977,18 @@ __device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, 
 
 // Data->model residual (MultiViewSilhouetteLossGPUOp.cu:142-240) + loss +
 // gradient (atomic, as the reference).  One workgroup per 16x16 tile of DT
-// pixels: the boundary points of every grid cell within R of the tile (each
-// cell row of the region is one contiguous range of the grid) are copied to
-// LDS and each boundary pixel scans that list, keeping the minimum of
-// (distance, index) with distance < R -- the brute-force result, since every
-// point within R of the pixel is in the list.  Regions with more than
-// kMaxRegionPts points fall back to a ring search over the grid.
+// pixels.  The grid cells within R of the tile (each cell row of that region
+// is one contiguous range of the grid) are copied to LDS with a per-cell
+// offset table; each boundary pixel then searches rings of cells around its
+// own cell, nearest first, keeping the minimum of (distance, index) with
+// distance < R.  A point in ring k + 1 or beyond is at least k cells away
+// along one axis, so the search stops once the best distance is below that
+// bound (with a margin for float rounding); cells outside the region hold no
+// point within R of the tile.  The result is the brute-force one.  Regions
+// larger than the LDS tables run the same search on the global grid.
 __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
   __shared__ f4 s_pts[kMaxRegionPts];
+  __shared__ unsigned int s_coff[kMaxRegionCells];
   __shared__ unsigned int s_rstart[kMaxRegionRows + 1];
   __shared__ unsigned int s_ra[kMaxRegionRows];
   __shared__ unsigned int s_wave[kB / 64];
@@ -999,9 +1016,10 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
     const int cx1 = min(vw.gx - 1, (int)floorf((xhi - gg.ox) / gg.cell) + 1);
     const int cy0 = max(0, (int)floorf((ylo - gg.oy) / gg.cell) - 1);
     const int cy1 = min(vw.gy - 1, (int)floorf((yhi - gg.oy) / gg.cell) + 1);
-    const int nrows = cy1 - cy0 + 1;
+    const int nrows = cy1 - cy0 + 1, ncols = cx1 - cx0 + 1;
+    const bool shape_ok = nrows > 0 && ncols > 0 && nrows <= kMaxRegionRows && nrows * (ncols + 1) <= kMaxRegionCells;
     unsigned int len = 0;
-    if ((int)threadIdx.x < nrows && threadIdx.x < kMaxRegionRows && cx0 <= cx1) {
+    if (shape_ok && (int)threadIdx.x < nrows) {
       const long long rc = vw.cell0 + (long long)(cy0 + (int)threadIdx.x) * vw.gx;
       const unsigned int a = d.cell_off[rc + cx0];
       len = d.cell_off[rc + cx1 + 1] - a;
@@ -1012,8 +1030,14 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
     if (threadIdx.x < kMaxRegionRows) s_rstart[threadIdx.x] = ex;
     if (threadIdx.x == 0) s_rstart[kMaxRegionRows] = total;
     __syncthreads();
-    const bool in_lds = nrows > 0 && nrows <= kMaxRegionRows && cx0 <= cx1 && total <= (unsigned int)kMaxRegionPts;
+    const bool in_lds = shape_ok && total <= (unsigned int)kMaxRegionPts;
     if (in_lds) {
+      // LDS offset of cell (cy0 + r, cx0 + k): row start + the cell's offset within its row
+      for (int i = threadIdx.x; i < nrows * (ncols + 1); i += kB) {
+        const int r = i / (ncols + 1), k = i - r * (ncols + 1);
+        const long long rc = vw.cell0 + (long long)(cy0 + r) * vw.gx;
+        s_coff[i] = s_rstart[r] + (d.cell_off[rc + cx0 + k] - s_ra[r]);
+      }
       for (unsigned int i = threadIdx.x; i < total; i += kB) {
         int r = 0;
         while (r + 1 < nrows && s_rstart[r + 1] <= i) ++r;
@@ -1025,26 +1049,25 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       const float ix = ((float)ui - cr[0]) * sc + cr[3], iy = ((float)vi - cr[1]) * sc + cr[4];
       float bd = R, bx = 0.f, by = 0.f;
       int best = -1;
-      if (in_lds) {
-        for (unsigned int i = 0; i < total; ++i) {
-          const f4 it = s_pts[i];
-          const float dx = it.x - ix, dy = it.y - iy;
-          nearest_take(sqrtf(dx * dx + dy * dy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
-        }
-      } else {
-        // rings of cells at Chebyshev distance k around the pixel's cell; a
-        // point in ring k + 1 or beyond is at least k cells away along one
-        // axis, so the search stops once the best distance is below that
-        // bound (with a margin for float rounding)
-        const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
-        for (int k = 0; k <= kRing; ++k) {
-          for (int oy = -k; oy <= k; ++oy) {
-            const int yy = cy + oy;
-            if (yy < 0 || yy >= vw.gy) continue;
-            const int step = (oy == -k || oy == k) ? 1 : 2 * k;
-            for (int ox = -k; ox <= k; ox += (step > 0 ? step : 1)) {
-              const int xx = cx + ox;
-              if (xx < 0 || xx >= vw.gx) continue;
+      const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
+      for (int k = 0; k <= kRing; ++k) {
+        for (int oy = -k; oy <= k; ++oy) {
+          const int yy = cy + oy;
+          if (yy < 0 || yy >= vw.gy) continue;
+          const int step = (oy == -k || oy == k) ? 1 : 2 * k;
+          for (int ox = -k; ox <= k; ox += step) {
+            const int xx = cx + ox;
+            if (xx < 0 || xx >= vw.gx) continue;
+            if (in_lds) {
+              if (xx < cx0 || xx > cx1 || yy < cy0 || yy > cy1) continue;
+              const int ci = (yy - cy0) * (ncols + 1) + (xx - cx0);
+              const unsigned int q1 = s_coff[ci + 1];
+              for (unsigned int q = s_coff[ci]; q < q1; ++q) {
+                const f4 it = s_pts[q];
+                const float dx = it.x - ix, dy = it.y - iy;
+                nearest_take(sqrtf(dx * dx + dy * dy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+              }
+            } else {
               const long long cell = vw.cell0 + yy * vw.gx + xx;
               const unsigned int q1 = d.cell_off[cell + 1];
               for (unsigned int q = d.cell_off[cell]; q < q1; ++q) {
@@ -1054,9 +1077,9 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
               }
             }
           }
-          const float reach = (float)k * gg.cell * (1.f - 1e-4f);
-          if (bd < reach || reach >= R) break;
         }
+        const float reach = (float)k * gg.cell * (1.f - 1e-4f);
+        if (bd < reach || reach >= R) break;
       }
       if (best >= 0) {
         const float r0 = (bx - ix) / (30.f * sc), r1 = (by - iy) / (30.f * sc);
@@ -1179,32 +1202,45 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   }
 }
 
-// One wavefront per (character, joint): the joint's DQ gradient over its
-// (vertex, slot) entries, lane-strided then reduced across the wavefront.
-__global__ __launch_bounds__(kB) void joint_reduce_kernel(Dev d, IO io, int totJ) {
-  const int j = (blockIdx.x * kB + threadIdx.x) >> 6;
+// Joint DQ gradients in two deterministic passes.  The (vertex, slot) entries
+// of every joint are cut into chunks of at most kJChunk (a static table), so
+// the work spreads over thousands of wavefronts however skewed the joints'
+// vertex counts are.  Pass 1: one wavefront per chunk, lane-strided then
+// reduced across the wavefront.
+__global__ __launch_bounds__(kB) void joint_partial_kernel(Dev d, int nChunk) {
+  const int ck = (blockIdx.x * kB + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (j >= totJ) return;
-  int b = 0;
-  while (b + 1 < d.B && d.ch[b + 1].j0 <= j) ++b;
-  const int v0 = d.ch[b].v0;
+  if (ck >= nChunk) return;
+  const int4 r = d.jchunk[ck];
   float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int e = d.jv_off[j] + lane; e < d.jv_off[j + 1]; e += 64) {
+  for (int e = r.x + lane; e < r.y; e += 64) {
     const int ent = d.jv_ent[e];
-    const size_t gv = (size_t)v0 + (ent >> 2);
+    const size_t gv = (size_t)r.z + (ent >> 2);
     const float ws = d.vws[KS * gv + (ent & 3)];
-    const float* q8 = d.vq + 8 * gv;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] += ws * q8[q];
+    const f4* q8 = (const f4*)(d.vq + 8 * gv);
+    const f4 qa = q8[0], qb = q8[1];
+    acc[0] += ws * qa.x; acc[1] += ws * qa.y; acc[2] += ws * qa.z; acc[3] += ws * qa.w;
+    acc[4] += ws * qb.x; acc[5] += ws * qb.y; acc[6] += ws * qb.z; acc[7] += ws * qb.w;
   }
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = wave_sum(acc[q]);
   if (lane < 8) {
-    float r = acc[0];
+    float v = acc[0];
 #pragma unroll
-    for (int q = 1; q < 8; ++q) r = lane == q ? acc[q] : r;
-    io.g_dq[8 * (size_t)j + lane] = r;
+    for (int q = 1; q < 8; ++q) v = lane == q ? acc[q] : v;
+    d.jpart[8 * (size_t)ck + lane] = v;
   }
+}
+
+// Pass 2: one lane per (joint, component), the joint's chunk partials summed
+// in chunk order (a joint without entries gets 0).
+__global__ __launch_bounds__(kB) void joint_reduce_kernel(Dev d, IO io, int totJ) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= 8 * totJ) return;
+  const int j = i >> 3, q = i & 7;
+  float sum = 0.f;
+  for (int k = d.jc_off[j]; k < d.jc_off[j + 1]; ++k) sum += d.jpart[8 * (size_t)k + q];
+  io.g_dq[i] = sum;
 }
 
 __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
@@ -1255,6 +1291,25 @@ __global__ __launch_bounds__(kB) void finalize_kernel(Dev d, IO io) {
   if (threadIdx.x == 0) { io.loss[2 * b] = lc; io.loss[2 * b + 1] = ls; }
 }
 
+// Tile shapes: (1 << tws) x (1 << (8 - tws)) pixels, tws in [4, 6] (16x16,
+// 32x8, 64x4), one lane per pixel in the fine kernel.
+constexpr int kMinTileShift = 4, kMaxTileShift = 6, kDefaultTileShift = 5;
+
+// Per-view tile counts and offsets for one tile shape; returns the total.
+long long layout_tiles(std::vector<ViewD>& vw, int tws, int& max_tiles) {
+  const int tw = 1 << tws, th = 1 << (8 - tws);
+  long long tot = 0;
+  max_tiles = 0;
+  for (ViewD& v : vw) {
+    v.tiles_x = (v.W + tw - 1) / tw;
+    v.tiles = v.tiles_x * ((v.H + th - 1) / th);
+    v.tile0 = tot;
+    tot += v.tiles;
+    max_tiles = std::max(max_tiles, v.tiles);
+  }
+  return tot;
+}
+
 template <class T>
 hipError_t up(T** dst, const std::vector<T>& src) {
   hipError_t e = hipMalloc(dst, sizeof(T) * std::max<size_t>(src.size(), 1));
@@ -1269,7 +1324,10 @@ using namespace dcx;
 
 struct dcx_pipeline {
   Dev d{};
-  int B = 0, V = 0, sumN = 0, sumF = 0, sumM = 0, sumJ = 0;
+  int B = 0, V = 0, sumN = 0, sumF = 0, sumM = 0, sumJ = 0, nChunk = 0;
+  int maxTilesAll = 0;           // over the tile shapes, for the tile arrays
+  long long totTilesAll = 0;
+  std::vector<ViewD> vw_host;    // view table of the current tile shape
   long long sumS2 = 0;
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
@@ -1292,6 +1350,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   std::vector<ViewD> vw;
   std::vector<float> rest, jw, ew;
   std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list, jv_off(1, 0), jv_ent;
+  std::vector<int> jc_off(1, 0);
+  std::vector<int4> jchunk;
   std::vector<float> n2v_w;
   long long vn = 0, vf = 0, pix = 0, sil = 0, tiles = 0, cells = 0;
   int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0, maxJ = 0, maxTiles = 0;
@@ -1360,17 +1420,19 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     const int jbase = (int)jv_ent.size();
     jv_ent.resize(jbase + KS * d.N);
     for (int i = 0; i < KS * d.N; ++i) jv_ent[jbase + jfill[c.joint_idx[i]]++] = i;  // i = v * 4 + slot
-    for (int j = 0; j < d.J; ++j) jv_off.push_back(jbase + jc[j + 1]);
+    for (int j = 0; j < d.J; ++j) {
+      jv_off.push_back(jbase + jc[j + 1]);
+      for (int e = jbase + jc[j]; e < jbase + jc[j + 1]; e += kJChunk)
+        jchunk.push_back(make_int4(e, std::min(e + kJChunk, jbase + jc[j + 1]), p->sumN, p->sumJ + j));
+      jc_off.push_back((int)jchunk.size());
+    }
     for (int k = 0; k < d.C; ++k) {
       ViewD v{};
       v.ch = b; v.W = c.width; v.H = c.height; v.S = c.dt_size;
-      v.tiles_x = (c.width + kTileW - 1) / kTileW;
-      v.tiles = v.tiles_x * ((c.height + kTileH - 1) / kTileH);
-      v.vn0 = vn; v.vf0 = vf; v.pix0 = pix; v.sil0 = sil; v.tile0 = tiles; v.cell0 = cells;
+      v.vn0 = vn; v.vf0 = vf; v.pix0 = pix; v.sil0 = sil; v.cell0 = cells;
       v.gx = c.dt_size * kRing / 30 + 2 * kGridMargin + 3; v.gy = v.gx;
       vn += d.N; vf += d.F; pix += (long long)c.width * c.height; sil += (long long)c.dt_size * c.dt_size;
-      tiles += v.tiles; cells += (long long)v.gx * v.gy;
-      maxTiles = std::max(maxTiles, v.tiles);
+      cells += (long long)v.gx * v.gy;
       vw.push_back(v);
     }
     p->sumN += d.N; p->sumF += d.F; p->sumM += d.M; p->sumJ += d.J;
@@ -1379,9 +1441,26 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   }
   p->V = (int)vw.size();
   p->sumS2 = sil;
+  // tile arrays are sized for the largest count over the tile shapes
+  for (int tws = kMinTileShift; tws <= kMaxTileShift; ++tws) {
+    int mt = 0;
+    const long long t = layout_tiles(vw, tws, mt);
+    tiles = std::max(tiles, t);
+    maxTiles = std::max(maxTiles, mt);
+  }
+  p->maxTilesAll = maxTiles;
+  p->totTilesAll = tiles;
+  {
+    int mt = 0;
+    const long long t = layout_tiles(vw, kDefaultTileShift, mt);
+    p->d.tws = kDefaultTileShift;
+    p->d.ths = 8 - kDefaultTileShift;
+    p->d.maxTiles = mt;
+    p->d.totTiles = t;
+  }
+  p->vw_host = vw;
   Dev& d = p->d;
   d.B = n_chars; d.V = p->V; d.maxN = maxN; d.maxF = maxF; d.maxM = maxM; d.maxS2 = maxS2; d.maxJ = maxJ;
-  d.maxTiles = maxTiles;
   d.gxN = (int)ceil_div(maxN, kB);
   {
     int maxS = 0;
@@ -1389,7 +1468,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     const int tpr = (maxS + kDtTile - 1) / kDtTile;
     d.gxS = tpr * tpr;  // DT-pixel tiles per view
   }
-  d.totVN = vn; d.totVF = vf; d.totPix = pix; d.totTiles = tiles; d.totCells = cells;
+  d.totVN = vn; d.totVF = vf; d.totPix = pix; d.totCells = cells;
   const long long cap = bin_capacity > 0 ? bin_capacity : std::max<long long>(4 * vf, 1 << 20);
   if (cap >= (1ll << 32) || vf >= (1ll << 31) || (long long)p->V * maxTiles >= (1ll << 31)) {
     delete p;
@@ -1417,6 +1496,10 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   upload(&denode, enode); upload(&dew, ew); upload(&dnv, node_vtx); upload(&dn2o, n2v_off);
   upload(&dn2v, n2v_vtx); upload(&dn2w, n2v_w); upload(&dvfo, vf_off); upload(&dvfl, vf_list);
   upload(&djvo, jv_off); upload(&djve, jv_ent);
+  int4* djck; int* djco;
+  upload(&djck, jchunk); upload(&djco, jc_off);
+  d.jchunk = djck; d.jc_off = djco;
+  p->nChunk = (int)jchunk.size();
   d.ch = dch; d.vw = dvw; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
   d.enode = denode; d.ew = dew; d.node_vtx = dnv; d.n2v_off = dn2o; d.n2v_vtx = dn2v; d.n2v_w = dn2w;
   d.vf_off = dvfo; d.vf_list = dvfl; d.jv_off = djvo; d.jv_ent = djve;
@@ -1449,6 +1532,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.loss_px, sizeof(float) * (size_t)p->V * d.gxS);
   alloc(&d.vq, sizeof(float) * 8 * (size_t)p->sumN);
   alloc(&d.vws, sizeof(float) * KS * (size_t)p->sumN);
+  alloc(&d.jpart, sizeof(float) * 8 * (size_t)p->nChunk);
   if (e != hipSuccess) {
     pipe_free(p);
     return set_error(DCX_ERR_OOM, "dcx_pipeline_create: %s", hipGetErrorString(e));
@@ -1474,7 +1558,7 @@ static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "project",    "tri_count",
     "tile_scan",      "tri_scan",      "bin_scatter",      "fine",      "boundary",   "sil_point",
     "sil_grid_count", "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "face_sum",   "vertex_bwd",
-    "joint_reduce",   "node_bwd",      "finalize"};
+    "joint_partial",  "joint_reduce",  "node_bwd",         "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
@@ -1521,7 +1605,12 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
-  hipLaunchKernelGGL(fine_kernel, dim3((unsigned)((long long)V * d.maxTiles)), dim3(kTilePix), 0, s, d, io);
+  {
+    const dim3 grid((unsigned)((long long)V * d.maxTiles));
+    if (d.tws == 4) hipLaunchKernelGGL(fine_kernel<4>, grid, dim3(kTilePix), 0, s, d, io);
+    else if (d.tws == 6) hipLaunchKernelGGL(fine_kernel<6>, grid, dim3(kTilePix), 0, s, d, io);
+    else hipLaunchKernelGGL(fine_kernel<5>, grid, dim3(kTilePix), 0, s, d, io);
+  }
   mark();
   hipLaunchKernelGGL(boundary_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
@@ -1540,7 +1629,9 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(vertex_bwd_kernel, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 64, kB)), dim3(kB), 0, s, d, io, p->sumJ);
+  hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, s, d, p->nChunk);
+  mark();
+  hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, s, d, io, p->sumJ);
   mark();
   hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
   mark();
@@ -1601,4 +1692,107 @@ extern "C" dcx_status dcx_pipeline_profile(dcx_pipeline* p, const dcx_pipeline_i
   for (int i = 0; i <= kNumStages; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
   return st;
+}
+
+static void drop_graph(dcx_pipeline* p) {
+  if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+}
+
+extern "C" dcx_status dcx_pipeline_set_tile(dcx_pipeline* p, int tile_w) {
+  DCX_REQUIRE(p, "dcx_pipeline_set_tile: null pipeline");
+  int tws = -1;
+  for (int k = kMinTileShift; k <= kMaxTileShift; ++k)
+    if (tile_w == (1 << k)) tws = k;
+  DCX_REQUIRE(tws >= 0, "dcx_pipeline_set_tile: tile width %d is not 16, 32 or 64", tile_w);
+  if (tws == p->d.tws) return DCX_OK;
+  int mt = 0;
+  const long long tot = layout_tiles(p->vw_host, tws, mt);
+  DCX_HIP(hipMemcpy((void*)p->d.vw, p->vw_host.data(), sizeof(ViewD) * p->vw_host.size(), hipMemcpyHostToDevice));
+  p->d.tws = tws;
+  p->d.ths = 8 - tws;
+  p->d.maxTiles = mt;
+  p->d.totTiles = tot;
+  drop_graph(p);  // kernel arguments changed: a captured step is stale
+  return DCX_OK;
+}
+
+extern "C" int dcx_pipeline_tile_width(const dcx_pipeline* p) { return p ? 1 << p->d.tws : 0; }
+
+// Number of (triangle, tile) pairs of the last step (the bins' fill).
+static dcx_status pairs_of_last_step(dcx_pipeline* p, hipStream_t s, unsigned long long* pairs) {
+  unsigned int v = 0;
+  DCX_HIP(hipMemcpyAsync(&v, p->d.tile_off + p->d.totTiles, sizeof(v), hipMemcpyDeviceToHost, s));
+  DCX_HIP(hipStreamSynchronize(s));
+  *pairs = v;
+  return DCX_OK;
+}
+
+// Record storage for `cap` (triangle, tile) pairs.
+static dcx_status grow_bins(dcx_pipeline* p, long long cap) {
+  DCX_REQUIRE(cap < (1ll << 32), "dcx_pipeline_autotune: %lld records exceed the 32-bit record index", cap);
+  Dev& d = p->d;
+  unsigned int* bins = nullptr;
+  float* rec = nullptr;
+  unsigned char* vis = nullptr;
+  hipError_t e = hipMalloc(&bins, 4 * (size_t)cap);
+  if (e == hipSuccess) e = hipMalloc(&rec, sizeof(float) * kGrad * (size_t)cap);
+  if (e == hipSuccess) e = hipMalloc(&vis, (size_t)cap);
+  if (e != hipSuccess) {
+    (void)hipFree(bins); (void)hipFree(rec); (void)hipFree(vis);
+    return set_error(DCX_ERR_OOM, "dcx_pipeline_autotune: %s", hipGetErrorString(e));
+  }
+  DCX_HIP(hipDeviceSynchronize());  // the old buffers may still be read by queued work
+  for (void*& a : p->allocs) {
+    if (a == d.bins) { (void)hipFree(a); a = bins; }
+    else if (a == d.rec) { (void)hipFree(a); a = rec; }
+    else if (a == d.rec_vis) { (void)hipFree(a); a = vis; }
+  }
+  d.bins = bins; d.rec = rec; d.rec_vis = vis;
+  d.capacity = (unsigned int)cap;
+  drop_graph(p);
+  return DCX_OK;
+}
+
+extern "C" dcx_status dcx_pipeline_autotune(dcx_pipeline* p, const dcx_pipeline_io* io, int reps, void* stream,
+                                            int* tile_w_out, float* ms_out, long long* pairs_out) {
+  DCX_REQUIRE(p && io && reps > 0, "dcx_pipeline_autotune: bad arguments");
+  hipStream_t s = as_stream(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DCX_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_error(DCX_ERR_HIP, "hipEventCreate"); }
+  dcx_status st = DCX_OK;
+  float best_ms = 0.f;
+  int best = -1;
+  unsigned long long most_pairs = 0;
+  for (int tws = kMinTileShift; tws <= kMaxTileShift && st == DCX_OK; ++tws) {
+    st = dcx_pipeline_set_tile(p, 1 << tws);
+    // warm-up step; bins sized to this shape's pairs (plus a margin) so no
+    // pair takes the overflow path
+    if (st == DCX_OK) st = pipe_enqueue(p, io, s);
+    unsigned long long pairs = 0;
+    if (st == DCX_OK) st = pairs_of_last_step(p, s, &pairs);
+    if (st == DCX_OK && pairs > p->d.capacity) {
+      st = grow_bins(p, (long long)(pairs + pairs / 4 + 1024));
+      if (st == DCX_OK) st = pipe_enqueue(p, io, s);
+    }
+    most_pairs = std::max(most_pairs, pairs);
+    if (st == DCX_OK && hipEventRecord(e0, s) != hipSuccess) st = set_error(DCX_ERR_HIP, "hipEventRecord");
+    for (int r = 0; r < reps && st == DCX_OK; ++r) st = pipe_enqueue(p, io, s);
+    if (st == DCX_OK && hipEventRecord(e1, s) != hipSuccess) st = set_error(DCX_ERR_HIP, "hipEventRecord");
+    float ms = 0.f;
+    if (st == DCX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      st = set_error(DCX_ERR_HIP, "dcx_pipeline_autotune: event timing failed");
+    ms /= (float)reps;
+    if (ms_out) ms_out[tws - kMinTileShift] = ms;
+    if (st == DCX_OK && (best < 0 || ms < best_ms)) { best = tws; best_ms = ms; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  st = dcx_pipeline_set_tile(p, 1 << best);
+  if (st) return st;
+  if (tile_w_out) *tile_w_out = 1 << best;
+  if (pairs_out) *pairs_out = (long long)most_pairs;
+  return DCX_OK;
 }

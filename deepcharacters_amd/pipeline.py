@@ -249,6 +249,30 @@ class BatchedPipeline:
         s = stream if stream is not None else self.captured_stream
         _lib.call("dcx_pipeline_replay", self.handle, s.cuda_stream)
 
+    TILE_WIDTHS = (16, 32, 64)  # tiles of 16x16, 32x8 and 64x4 pixels
+
+    @property
+    def tile_width(self) -> int:
+        return _lib.load().dcx_pipeline_tile_width(self.handle)
+
+    def set_tile(self, tile_w: int):
+        """Raster tile shape (tile_w x 256 / tile_w pixels).  Drops a captured graph."""
+        _lib.call("dcx_pipeline_set_tile", self.handle, int(tile_w))
+        self.captured_stream = None
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times `reps` steps per tile shape on the current inputs, keeps the
+        fastest, and grows the record storage of the bins if a shape needs more
+        (triangle, tile) pairs than it holds.  Drops a captured graph."""
+        tw = ctypes.c_int()
+        ms = (ctypes.c_float * 3)()
+        pairs = ctypes.c_longlong()
+        _lib.call("dcx_pipeline_autotune", self.handle, ctypes.byref(self.io), int(reps), stream_ptr(),
+                  ctypes.byref(tw), ms, ctypes.byref(pairs))
+        self.captured_stream = None
+        return {"tile_w": tw.value, "ms": dict(zip(self.TILE_WIDTHS, [float(x) for x in ms])),
+                "pairs": pairs.value}
+
     def stage_times(self, repeats: int = 1) -> dict[str, float]:
         """Mean per-stage time (ms) over `repeats` eager steps, from HIP events
         recorded on the launch stream between the stage launches."""

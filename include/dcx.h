@@ -506,6 +506,21 @@ int dcx_pipeline_num_stages(void);
 const char* dcx_pipeline_stage_name(int stage);
 dcx_status dcx_pipeline_profile(dcx_pipeline* pipe, const dcx_pipeline_io* io, float* stage_ms, int n_stages,
                                 void* stream);
+/* Raster tile shape: tile_w x (256 / tile_w) pixels, tile_w in {16, 32, 64}
+ * (default 32).  The tile-binned rasterizer the north-star design asks for
+ * replaces the reference's per-triangle bounding-box loop
+ * (CUDABasedRasterization.cu:215-451), which has no tiles.  Changing the
+ * shape synchronises with the device and drops a captured graph. */
+dcx_status dcx_pipeline_set_tile(dcx_pipeline* pipe, int tile_w);
+int dcx_pipeline_tile_width(const dcx_pipeline* pipe);
+/* Autotuner: runs one warm-up step and `reps` timed steps per tile shape on
+ * the given inputs, grows the bins' record storage to 1.25x the largest
+ * (triangle, tile) pair count seen if it is short, and keeps the fastest
+ * shape.  Outputs (each may be NULL): the chosen tile width, the mean step
+ * time per shape ms[3] (widths 16, 32, 64), the largest pair count.
+ * Synchronises; drops a captured graph.  No counterpart in the reference. */
+dcx_status dcx_pipeline_autotune(dcx_pipeline* pipe, const dcx_pipeline_io* io, int reps, void* stream,
+                                 int* tile_w, float* ms, long long* pairs);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,11 @@ def test_raster_fixture():
                                     z["vnormal"], vcol=z["vcol"], target=z["target"])
     np.testing.assert_allclose(gvp, z["g_vpos"], rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(gvc, z["g_vcol"], rtol=1e-12, atol=1e-12)
+    gvp, gvc, _, gsh = orc.raster_bwd(z["faces"], z["g_render"], z["vpos"], z["ext"], z["intr"], z["face"],
+                                      z["bary"], z["vnormal"], vcol=z["vcol"], shading="shaded", sh=z["sh"])
+    np.testing.assert_allclose(gvp, z["g_vpos_shaded"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gvc, z["g_vcol_shaded"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gsh, z["g_sh"], rtol=1e-12, atol=1e-12)
 
 
 def test_silhouette_fixture():

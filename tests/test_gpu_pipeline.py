@@ -162,9 +162,9 @@ def test_batched_step_is_deterministic(works):
     """The raster path reduces its gradients without float atomics (per
     (triangle, tile) records gathered in a fixed order) and the losses from
     per-block partials: losses, buffers and colour gradients repeat bit for
-    bit.  (Joint-DQ and node gradients also sum LDS atomics and the
-    silhouette nearest-point scatter, so only their values, not their bits,
-    are order-independent.)"""
+    bit.  (Joint-DQ and node gradients also carry the silhouette's
+    nearest-point scatter, a float atomicAdd, so only their values, not their
+    bits, are order-independent.)"""
     bp = pipeline.BatchedPipeline(works)
     bp.step()
     torch.cuda.synchronize()
@@ -174,3 +174,64 @@ def test_batched_step_is_deterministic(works):
     for a, b in zip(first[:5], (bp.loss, bp.face, bp.bary, bp.render, bp.g_colors)):
         assert torch.equal(a, b)
     assert _rel(bp.g_dq, first[5]) < 1e-5
+
+
+def _outputs(bp):
+    return [x.clone() for x in (bp.face, bp.bary, bp.render, bp.loss, bp.g_colors, bp.g_dq, bp.g_dT, bp.g_dA)]
+
+
+@pytest.mark.parametrize("tile_w", [16, 64])
+def test_tile_shapes_agree(works, tile_w):
+    """Every tile shape rasterises the same pixels with the same rule: the
+    buffers are bit-identical to the default 32x8 tiles; losses and gradients
+    differ only in summation order."""
+    bp = pipeline.BatchedPipeline(works)
+    assert bp.tile_width == 32
+    bp.step()
+    torch.cuda.synchronize()
+    ref = _outputs(bp)
+    bp.set_tile(tile_w)
+    assert bp.tile_width == tile_w
+    bp.step()
+    torch.cuda.synchronize()
+    got = _outputs(bp)
+    for k in range(3):
+        assert torch.equal(got[k], ref[k]), k
+    for k in range(3, 8):
+        assert _rel(got[k], ref[k]) < 1e-5, k
+
+
+def test_set_tile_rejects_other_widths(works):
+    bp = pipeline.BatchedPipeline(works)
+    with pytest.raises(Exception, match="16, 32 or 64"):
+        bp.set_tile(24)
+
+
+def test_autotune_picks_a_shape_and_grows_the_bins(works):
+    """Starting from a record store far too small, the autotuner grows it to
+    hold every (triangle, tile) pair, keeps the fastest shape, and the tuned
+    pipeline (eager and captured) reproduces the default one."""
+    ref = pipeline.BatchedPipeline(works)
+    ref.step()
+    torch.cuda.synchronize()
+    want = _outputs(ref)
+    bp = pipeline.BatchedPipeline(works, bin_capacity=1024)
+    res = bp.autotune(reps=2)
+    assert res["tile_w"] in (16, 32, 64) and bp.tile_width == res["tile_w"]
+    assert res["pairs"] > 1024
+    assert all(v > 0 for v in res["ms"].values())
+    assert min(res["ms"], key=res["ms"].get) == res["tile_w"]
+    bp.step()
+    torch.cuda.synchronize()
+    got = _outputs(bp)
+    for k in range(3):
+        assert torch.equal(got[k], want[k]), k
+    for k in range(3, 8):
+        assert _rel(got[k], want[k]) < 1e-5, k
+    s = torch.cuda.Stream()
+    bp.capture(s)
+    bp.replay()
+    s.synchronize()
+    torch.cuda.synchronize()
+    assert torch.equal(bp.face, want[0]) and torch.equal(bp.render, want[2])
+    assert _rel(bp.g_dq, want[5]) < 1e-5
