@@ -52,6 +52,19 @@ constexpr int kJChunk = 512;  // (vertex, slot) entries per joint-reduction chun
 constexpr int KE = 4;         // ED nodes per vertex
 constexpr int kNodeRow = 15;  // Rd(9) g(3) td(3)
 constexpr int kImgFs = 2;     // image_filter_size of the model-to-data term
+
+// 1 / (L1 norm of the image-gradient tap weights xx / (xx^2 + yy^2)) over the
+// (2 kImgFs + 1)^2 window, accumulated in the order imageGradient uses.
+constexpr float img_grad_inv_norm() {
+  float n = 0.f;
+  for (int yy = -kImgFs; yy <= kImgFs; ++yy)
+    for (int xx = -kImgFs; xx <= kImgFs; ++xx) {
+      const int den = xx * xx + yy * yy;
+      if (den != 0) n += (float)(xx < 0 ? -xx : xx) / (float)den;
+    }
+  return 1.f / n;
+}
+constexpr float kImgGradInvNorm = img_grad_inv_norm();
 constexpr int kGrad = 18;     // per (triangle, tile) record: 3 vertices x (pos 3 + colour 3)
 constexpr int kXcd = 8;       // L2 domains the workgroups are dealt round-robin to
 
@@ -98,6 +111,7 @@ struct Dev {
   float* vdef;            // [sumN,3] skinned vertices
   float* vnrm;            // [sumN,3]
   f4* proj;               // [totVN]
+  float* ke;              // [V,12] per view: K * R (3x3 row-major), then K * t
   f2* nrm2;               // [totVN]
   TriRec* tri;            // [totVF]
   unsigned int* tri_cnt;  // [totVF] tiles touched per (view, face)
@@ -318,6 +332,14 @@ __global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
   const float* E = io.ext + (size_t)w * 12;
   const float* K = io.intr + (size_t)w * 9;
   const int gv = c.v0 + v;
+  if (v == 0) {
+    // K [R | t] for the gradient kernels (one lane per view)
+    float* o = d.ke + (size_t)w * 12;
+    for (int r = 0; r < 3; ++r) {
+      for (int cc = 0; cc < 3; ++cc) o[3 * r + cc] = K[3 * r] * E[cc] + K[3 * r + 1] * E[4 + cc] + K[3 * r + 2] * E[8 + cc];
+      o[9 + r] = K[3 * r] * E[3] + K[3 * r + 1] * E[7] + K[3 * r + 2] * E[11];
+    }
+  }
   const f3 X = ld3(d.vdef + 3 * (size_t)gv);
   d.proj[vw.vn0 + v] = project_vertex(E, K, d.vdef + 3 * (size_t)gv);
   // image-space normal direction (CameraProjectionGPUOp.cu:45-86)
@@ -457,14 +479,50 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
 // Fine tile kernel
 // ---------------------------------------------------------------------------
 
+// Pixels of one triangle inside one tile into the LDS depth/id keys.  The
+// edge functions are exact integers either way; for triangles whose snapped
+// extent is at most kSmallSpan sub-pixels on both axes they are walked in
+// 32-bit arithmetic: at any pixel centre of the bounding box (plus the one
+// step past its end) |e| <= 2 * 2^14 * (2^14 + 2^9) < 2^31, and converting
+// the same integer to float from 32 or 64 bits rounds identically, so the
+// keys are bit-identical to the 64-bit walk.
+constexpr int kSmallSpan = 1 << 14;  // 64 px
+
+__device__ __forceinline__ float interp_iz32(int e0, int e1, int e2, const float* iz) {
+  const float w0 = (float)e0, w1 = (float)e1, w2 = (float)e2;
+  return (w0 * iz[0] + w1 * iz[1] + w2 * iz[2]) / (w0 + w1 + w2);
+}
+
 __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, int tws, int ths,
                                                unsigned long long* s_key) {
   const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
   const int x1 = min(t.bx1_by1 & 0xffff, tx0 + (1 << tws) - 1), y1 = min(t.bx1_by1 >> 16, ty0 + (1 << ths) - 1);
   if (x0 > x1 || y0 > y1) return;
+  const int face = t.face_flags & kFaceMask;
+  const int spx = max(t.X[0], max(t.X[1], t.X[2])) - min(t.X[0], min(t.X[1], t.X[2]));
+  const int spy = max(t.Y[0], max(t.Y[1], t.Y[2])) - min(t.Y[0], min(t.Y[1], t.Y[2]));
   EdgeWalk w;
   w.init(t, x0, y0);
-  const int face = t.face_flags & kFaceMask;
+  if (spx <= kSmallSpan && spy <= kSmallSpan) {
+    int row[3], dx[3], dy[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      row[i] = (int)w.e[i] + w.bias[i];  // bias folded in: inside <=> e + bias >= 0
+      dx[i] = (int)w.dx[i];
+      dy[i] = (int)w.dy[i];
+    }
+    for (int y = y0; y <= y1; ++y) {
+      int e[3] = {row[0], row[1], row[2]};
+      for (int x = x0; x <= x1; ++x) {
+        if (e[0] >= 0 && e[1] >= 0 && e[2] >= 0)
+          atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)),
+                    make_key(interp_iz32(e[0] - w.bias[0], e[1] - w.bias[1], e[2] - w.bias[2], t.iz), face));
+        e[0] += dx[0]; e[1] += dx[1]; e[2] += dx[2];
+      }
+      row[0] += dy[0]; row[1] += dy[1]; row[2] += dy[2];
+    }
+    return;
+  }
   long long row[3] = {w.e[0], w.e[1], w.e[2]};
   for (int y = y0; y <= y1; ++y) {
     long long e[3] = {row[0], row[1], row[2]};
@@ -527,6 +585,46 @@ __device__ void bary_vjp_p(const float* E, const float* K, const f3 X[3], float 
     const float gz = K[2] * gh0 + K[5] * gh1 + K[8] * gh2;
     out[i] = mk3(E[0] * gx + E[4] * gy + E[8] * gz, E[1] * gx + E[5] * gy + E[9] * gz,
                  E[2] * gx + E[6] * gy + E[10] * gz);
+  }
+}
+
+// The same VJP from the stored projections P_i = (u_i, v_i, z_i, 1/z_i) of
+// the three vertices and M = K R of the view, with hardware reciprocals in
+// place of divisions (gradients carry a tolerance; the buffers the forward
+// writes do not go through here).
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ void bary_vjp_fast(const f4 P[3], const float* M, float px, float py, const float gb_in[3],
+                                              f3 out[3]) {
+  float e[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int j = i == 2 ? 0 : i + 1, k = i == 0 ? 2 : i - 1;
+    e[i] = (P[k].x - P[j].x) * (py - P[j].y) - (P[k].y - P[j].y) * (px - P[j].x);
+  }
+  const float q0 = e[0] * P[0].w, q1 = e[1] * P[1].w, q2 = e[2] * P[2].w;
+  const float rQ = rcp(q0 + q1 + q2);
+  const float b0 = q0 * rQ, b1 = q1 * rQ;
+  const float g0 = gb_in[0] - gb_in[2], g1 = gb_in[1] - gb_in[2];
+  const float s = g0 * b0 + g1 * b1;
+  const float gq[3] = {(g0 - s) * rQ, (g1 - s) * rQ, -s * rQ};
+  float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float ge = gq[i] * P[i].w;
+    const int j = i == 2 ? 0 : i + 1, k = i == 0 ? 2 : i - 1;
+    gu[j] += ge * (P[k].y - py);
+    gv[j] += ge * (px - P[k].x);
+    gu[k] += ge * (py - P[j].y);
+    gv[k] += ge * (P[j].x - px);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float iz = P[i].w;
+    const float gh0 = gu[i] * iz, gh1 = gv[i] * iz;
+    const float gh2 = -(gu[i] * P[i].x + gv[i] * P[i].y) * iz - gq[i] * e[i] * iz * iz;
+    out[i] = mk3(M[0] * gh0 + M[3] * gh1 + M[6] * gh2, M[1] * gh0 + M[4] * gh1 + M[7] * gh2,
+                 M[2] * gh0 + M[5] * gh1 + M[8] * gh2);
   }
 }
 
@@ -701,45 +799,40 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         s_g[(6 * k + 5) * kGPad + pix] = g.z * wgt[k];
       }
       const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
-      const float* E = io.ext + (size_t)w * 12;
-      const float* K = io.intr + (size_t)w * 9;
-      const float* vb = d.vdef + 3 * (size_t)c.v0;
-      const f3 X[3] = {ld3(vb + 3 * i0), ld3(vb + 3 * i1), ld3(vb + 3 * i2)};
+      const float* M = d.ke + (size_t)w * 12;
+      const f4 P[3] = {pv[i0], pv[i1], pv[i2]};
       f3 gx[3];
-      bary_vjp_p(E, K, X, px, py, gw, gx);
-      // model-to-data term (CUDABasedRasterizationGrad.cu:413-437)
+      bary_vjp_fast(P, M, px, py, gw, gx);
+      // model-to-data term (CUDABasedRasterizationGrad.cu:413-437): the
+      // target's image gradient (imageGradient, RendererUtil.h:566-620, with
+      // its tap weights xx / (xx^2 + yy^2) and their L1 norm folded into
+      // constants) projected back through the fragment's position
       const int fs = kImgFs;
       if (x >= fs + 1 && y >= fs + 1 && x < vw.W - (fs + 1) && y < vw.H - (fs + 1)) {
-        f3 du = mk3(0.f, 0.f, 0.f), dvv = du;
-        float norm = 0.f;
+        float tu = 0.f, tv = 0.f;  // -g . (dI/du), -g . (dI/dv) before the norm
+#pragma unroll
         for (int yy = -fs; yy <= fs; ++yy)
+#pragma unroll
           for (int xx = -fs; xx <= fs; ++xx) {
+            if (xx == 0 && yy == 0) continue;
             const int a = (py_i + kImgFs + yy) * aw + (px_i + kImgFs + xx);
-            const f3 I = mk3(u2.tgt[a], u2.tgt[an + a], u2.tgt[2 * an + a]);
+            const float gi = g.x * u2.tgt[a] + g.y * u2.tgt[an + a] + g.z * u2.tgt[2 * an + a];
             const float den = (float)(xx * xx + yy * yy);
-            float gu = 0.f, gvv = 0.f;
-            if (den != 0.f) { gu = (float)xx / den; gvv = (float)yy / den; }
-            du = du + I * gu;
-            dvv = dvv + I * gvv;
-            norm += fabsf(gu);
+            tu += gi * ((float)xx / den);
+            tv += gi * ((float)yy / den);
           }
-        du = mk3(du.x / norm, du.y / norm, du.z / norm);
-        dvv = mk3(dvv.x / norm, dvv.y / norm, dvv.z / norm);
-        const float gu = -dot(g, du), gvv = -dot(g, dvv);
-        const f3 fp = X[0] * wgt[0] + X[1] * wgt[1] + X[2] * wgt[2];
-        const float xc = E[0] * fp.x + E[1] * fp.y + E[2] * fp.z + E[3];
-        const float yc = E[4] * fp.x + E[5] * fp.y + E[6] * fp.z + E[7];
-        const float zc = E[8] * fp.x + E[9] * fp.y + E[10] * fp.z + E[11];
-        const float hx = K[0] * xc + K[1] * yc + K[2] * zc;
-        const float hy = K[3] * xc + K[4] * yc + K[5] * zc;
-        const float hz = K[6] * xc + K[7] * yc + K[8] * zc;
+        const float gu = -tu * kImgGradInvNorm, gvv = -tv * kImgGradInvNorm;
+        const float* vb = d.vdef + 3 * (size_t)c.v0;
+        const f3 X0 = ld3(vb + 3 * i0), X1 = ld3(vb + 3 * i1), X2 = ld3(vb + 3 * i2);
+        const f3 fp = X0 * wgt[0] + X1 * wgt[1] + X2 * wgt[2];
+        const float hx = M[0] * fp.x + M[1] * fp.y + M[2] * fp.z + M[9];
+        const float hy = M[3] * fp.x + M[4] * fp.y + M[5] * fp.z + M[10];
+        const float hz = M[6] * fp.x + M[7] * fp.y + M[8] * fp.z + M[11];
         if (fabsf(hz) > 0.0001f) {
-          const float gh0 = gu / hz, gh1 = gvv / hz, gh2 = -(gu * hx + gvv * hy) / (hz * hz);
-          const float cx = K[0] * gh0 + K[3] * gh1 + K[6] * gh2;
-          const float cy = K[1] * gh0 + K[4] * gh1 + K[7] * gh2;
-          const float cz = K[2] * gh0 + K[5] * gh1 + K[8] * gh2;
-          const f3 gX = mk3(E[0] * cx + E[4] * cy + E[8] * cz, E[1] * cx + E[5] * cy + E[9] * cz,
-                            E[2] * cx + E[6] * cy + E[10] * cz);
+          const float rz = rcp(hz);
+          const float gh0 = gu * rz, gh1 = gvv * rz, gh2 = -(gu * hx + gvv * hy) * rz * rz;
+          const f3 gX = mk3(M[0] * gh0 + M[3] * gh1 + M[6] * gh2, M[1] * gh0 + M[4] * gh1 + M[7] * gh2,
+                            M[2] * gh0 + M[5] * gh1 + M[8] * gh2);
 #pragma unroll
           for (int k = 0; k < 3; ++k) gx[k] = gx[k] + gX * wgt[k];
         }
@@ -1507,6 +1600,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.vdef, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.vnrm, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.proj, sizeof(f4) * (size_t)vn);
+  alloc(&d.ke, sizeof(float) * 12 * (size_t)p->V);
   alloc(&d.nrm2, sizeof(f2) * (size_t)vn);
   alloc(&d.tri, sizeof(TriRec) * (size_t)vf);
   alloc(&d.tri_cnt, 4 * (size_t)vf);
