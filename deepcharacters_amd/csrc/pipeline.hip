@@ -494,11 +494,10 @@ __device__ __forceinline__ float interp_iz32(int e0, int e1, int e2, const float
 }
 
 __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, int tws, int ths,
-                                               unsigned long long* s_key) {
+                                               unsigned int tag, unsigned long long* s_key) {
   const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
   const int x1 = min(t.bx1_by1 & 0xffff, tx0 + (1 << tws) - 1), y1 = min(t.bx1_by1 >> 16, ty0 + (1 << ths) - 1);
   if (x0 > x1 || y0 > y1) return;
-  const int face = t.face_flags & kFaceMask;
   const int spx = max(t.X[0], max(t.X[1], t.X[2])) - min(t.X[0], min(t.X[1], t.X[2]));
   const int spy = max(t.Y[0], max(t.Y[1], t.Y[2])) - min(t.Y[0], min(t.Y[1], t.Y[2]));
   EdgeWalk w;
@@ -516,7 +515,7 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
       for (int x = x0; x <= x1; ++x) {
         if (e[0] >= 0 && e[1] >= 0 && e[2] >= 0)
           atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)),
-                    make_key(interp_iz32(e[0] - w.bias[0], e[1] - w.bias[1], e[2] - w.bias[2], t.iz), face));
+                    make_key(interp_iz32(e[0] - w.bias[0], e[1] - w.bias[1], e[2] - w.bias[2], t.iz), (int)tag));
         e[0] += dx[0]; e[1] += dx[1]; e[2] += dx[2];
       }
       row[0] += dy[0]; row[1] += dy[1]; row[2] += dy[2];
@@ -528,7 +527,7 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
     long long e[3] = {row[0], row[1], row[2]};
     for (int x = x0; x <= x1; ++x) {
       if (w.inside(e))
-        atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)), make_key(interp_iz(e[0], e[1], e[2], t.iz), face));
+        atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)), make_key(interp_iz(e[0], e[1], e[2], t.iz), (int)tag));
       e[0] += w.dx[0]; e[1] += w.dx[1]; e[2] += w.dx[2];
     }
     row[0] += w.dy[0]; row[1] += w.dy[1]; row[2] += w.dy[2];
@@ -651,8 +650,24 @@ __device__ __forceinline__ long long xcd_tile(long long wg, long long n) {
 }
 
 constexpr int kFast = 256;          // tiles with at most this many triangles stage their list in LDS
-constexpr int kFaceHash = 512;      // open-addressing table face -> index in the tile list
 constexpr int kGPad = kTilePix + 1;  // row stride of the per-pixel gradient table (bank spread)
+// Per-pixel gradient table rows: 0-8 position gradient (vertex k, axis a at
+// row 3k + a), 9-11 the colour-loss gradient g, 12-13 the weights w0, w1.
+constexpr int kGRows = 14;
+// Fast-path depth keys carry (face << kTagBits) | list index: the order of
+// equal depths stays the face order, and the winner's list slot comes free.
+constexpr int kTagBits = 8;
+static_assert((1 << kTagBits) >= kFast, "list index must fit the tag");
+
+// Record component q of the (triangle, tile) sums at pixel p: q = 6 k + c,
+// c < 3 the position gradient of vertex k, else g[c - 3] * w_k.
+__device__ __forceinline__ float grad_at(const float* s_g, int q, int p) {
+  const int k = q / 6, comp = q - 6 * k;
+  if (comp < 3) return s_g[(3 * k + comp) * kGPad + p];
+  const float w0 = s_g[12 * kGPad + p], w1 = s_g[13 * kGPad + p];
+  const float wk = k == 0 ? w0 : (k == 1 ? w1 : 1.f - w0 - w1);
+  return s_g[(9 + comp - 3) * kGPad + p] * wk;
+}
 
 // Sums of the per-pixel gradient table over the pixels each (triangle, tile)
 // pair won, written as one record per pair (or, without a record slot,
@@ -672,22 +687,28 @@ __device__ __forceinline__ void emit_record(const Dev& d, const IO& io, const Ch
 
 // TWS: log2 of the tile width (tiles are (1 << TWS) x (1 << (8 - TWS)) pixels).
 template <int TWS>
-__global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
+__global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) void fine_kernel(Dev d, IO io) {
   constexpr int tws = TWS, ths = 8 - TWS;
   constexpr int aw = (1 << tws) + 2 * kImgFs, an = aw * ((1 << ths) + 2 * kImgFs);  // apron row length, size
-  // u1: triangle records of the tile (raster) / per-pixel gradient table (backward)
-  __shared__ union U1 { TriRec tri[kFast]; float g[kGrad * kGPad]; } u1;
-  // u2: target colours with apron (per-pixel phase) / counting-sort scratch (records)
+  // u1: triangle records and depth keys of the tile (raster) / per-pixel
+  // gradient table (backward)
+  __shared__ union U1 {
+    struct { TriRec tri[kFast]; unsigned long long key[kTilePix]; } r;
+    float g[kGRows * kGPad];
+  } u1;
+  // u2: target colours with apron (per-pixel phase) / counting-sort scratch
+  // and the list's record slots and faces (fast path) / pixel faces (slow path)
   __shared__ union U2 {
     float tgt[3 * an];
-    struct { unsigned int wcnt[kTilePix / 64][kFast]; unsigned short order[kTilePix]; } srt;
+    struct {
+      unsigned int wcnt[kTilePix / 64][kFast];
+      unsigned short order[kTilePix];
+      unsigned int e[kFast];
+      int fj[kFast];
+    } srt;
+    int face[kTilePix];
   } u2;
-  __shared__ unsigned long long s_key[kTilePix];
-  __shared__ int s_hkey[kFaceHash];             // face -> list index (fast path)
-  __shared__ unsigned short s_hval[kFaceHash];
   __shared__ unsigned int s_start[kFast + 1];
-  __shared__ unsigned int s_e[kFast];
-  __shared__ int s_fj[kFast];
   __shared__ unsigned int s_wave[kTilePix / 64];
   __shared__ float s_red[kTilePix / 64];
   const long long lin = xcd_tile(blockIdx.x, (long long)gridDim.x);
@@ -720,45 +741,44 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
 
   int f = -1;   // winning face of this pixel
   int bj = -1;  // fast path: index of the winning triangle in the tile list
+  unsigned int my_e = 0;  // fast path, thread j < n_items: record slot and face of list entry j
+  int my_f = 0;
+  unsigned long long* s_key = u1.r.key;
   if (fast) {
-    // stage the tile's triangle records in LDS and index their faces
+    // stage the tile's triangle records in LDS
     s_key[threadIdx.x] = kEmptyKey;
-    for (int i = threadIdx.x; i < kFaceHash; i += kTilePix) s_hkey[i] = -1;
-    __syncthreads();
     if (threadIdx.x < n_items) {
       const unsigned int ri = d.bins[off + threadIdx.x];
       const TriRec t = d.tri[ri];
-      u1.tri[threadIdx.x] = t;
-      const int face = t.face_flags & kFaceMask;
-      s_fj[threadIdx.x] = face;
+      u1.r.tri[threadIdx.x] = t;
+      my_f = t.face_flags & kFaceMask;
       const TileRange r = tile_range(t, tws, ths);
-      s_e[threadIdx.x] = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
-      // faces are distinct within a tile list; at most kFast <= kFaceHash / 2
-      // entries, so linear probing always finds a free slot
-      unsigned int h = ((unsigned int)face * 2654435761u) >> 23;
-      for (int pr = 0; pr < kFaceHash; ++pr, h = (h + 1) & (kFaceHash - 1))
-        if (atomicCAS(s_hkey + h, -1, face) == -1) { s_hval[h] = (unsigned short)threadIdx.x; break; }
+      my_e = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
     }
     __syncthreads();
-    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) raster_in_tile(u1.tri[k], tx0, ty0, tws, ths, s_key);
+    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
+      const TriRec& t = u1.r.tri[k];
+      raster_in_tile(t, tx0, ty0, tws, ths, ((unsigned int)(t.face_flags & kFaceMask) << kTagBits) | k, s_key);
+    }
     __syncthreads();
     const unsigned long long key = s_key[threadIdx.x];
     if (inimg && key != kEmptyKey) {
-      f = (int)(key & 0xffffffffull);
-      unsigned int h = ((unsigned int)f * 2654435761u) >> 23;
-      for (int pr = 0; pr < kFaceHash; ++pr, h = (h + 1) & (kFaceHash - 1))
-        if (s_hkey[h] == f) { bj = s_hval[h]; break; }
+      const unsigned int tag = (unsigned int)(key & 0xffffffffull);
+      f = (int)(tag >> kTagBits);
+      bj = (int)(tag & ((1u << kTagBits) - 1u));
     }
-    __syncthreads();  // u1.tri is overwritten by the gradient table below
   } else {
     s_key[threadIdx.x] = kEmptyKey;
     __syncthreads();
-    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix)
-      raster_in_tile(overflow ? tv[k] : d.tri[d.bins[off + k]], tx0, ty0, tws, ths, s_key);
+    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
+      const TriRec& t = overflow ? tv[k] : d.tri[d.bins[off + k]];
+      raster_in_tile(t, tx0, ty0, tws, ths, (unsigned int)(t.face_flags & kFaceMask), s_key);
+    }
     __syncthreads();
     const unsigned long long key = s_key[threadIdx.x];
     f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
   }
+  __syncthreads();  // the keys and staged triangles are overwritten by the gradient table below
 
   // ---- per pixel: outputs, colour loss, gradient contributions -> LDS
   float* s_g = u1.g;
@@ -791,13 +811,9 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       lsum = io.w_col * (fabsf(df.x) + fabsf(df.y) + fabsf(df.z));
       auto sgn = [](float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); };
       const f3 g = mk3(io.w_col * sgn(df.x), io.w_col * sgn(df.y), io.w_col * sgn(df.z));
-      // colour gradient and dL/dweights (vertexColor, shadeless)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        s_g[(6 * k + 3) * kGPad + pix] = g.x * wgt[k];
-        s_g[(6 * k + 4) * kGPad + pix] = g.y * wgt[k];
-        s_g[(6 * k + 5) * kGPad + pix] = g.z * wgt[k];
-      }
+      // colour gradient g * w_k is formed when the records are summed
+      s_g[9 * kGPad + pix] = g.x; s_g[10 * kGPad + pix] = g.y; s_g[11 * kGPad + pix] = g.z;
+      s_g[12 * kGPad + pix] = ab.x; s_g[13 * kGPad + pix] = ab.y;
       const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
       const float* M = d.ke + (size_t)w * 12;
       const f4 P[3] = {pv[i0], pv[i1], pv[i2]};
@@ -839,9 +855,9 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        s_g[(6 * k) * kGPad + pix] = gx[k].x;
-        s_g[(6 * k + 1) * kGPad + pix] = gx[k].y;
-        s_g[(6 * k + 2) * kGPad + pix] = gx[k].z;
+        s_g[(3 * k) * kGPad + pix] = gx[k].x;
+        s_g[(3 * k + 1) * kGPad + pix] = gx[k].y;
+        s_g[(3 * k + 2) * kGPad + pix] = gx[k].z;
       }
     }
   }
@@ -854,6 +870,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
     // then per-wavefront counts and a scan over the triangles
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < (kTilePix / 64) * kFast; i += kTilePix) (&u2.srt.wcnt[0][0])[i] = 0u;
+    if (threadIdx.x < n_items) { u2.srt.e[threadIdx.x] = my_e; u2.srt.fj[threadIdx.x] = my_f; }
     __syncthreads();
     int rank = 0;
     unsigned long long act = __ballot(bj >= 0);
@@ -887,16 +904,15 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
       const int j = i / kGrad, q = i - j * kGrad;
       const unsigned int st = s_start[j], en = s_start[j + 1];
       float a = 0.f;
-      for (unsigned int r = st; r < en; ++r) a += s_g[q * kGPad + u2.srt.order[r]];
-      emit_record(d, io, c, s_e[j], s_fj[j], q, a, en > st);
+      for (unsigned int r = st; r < en; ++r) a += grad_at(s_g, q, u2.srt.order[r]);
+      emit_record(d, io, c, u2.srt.e[j], u2.srt.fj[j], q, a, en > st);
     }
     return;
   }
 
   // ---- slow path (long or overflowed tile lists): the thread that rasterised
   // a triangle sums the pixels it won (row-major order)
-  int* s_face = reinterpret_cast<int*>(s_key);
-  __syncthreads();
+  int* s_face = u2.face;
   s_face[threadIdx.x] = f;
   __syncthreads();
   for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
@@ -917,7 +933,7 @@ __global__ __launch_bounds__(kTilePix) void fine_kernel(Dev d, IO io) {
         if (s_face[p] == face) {
           won = true;
 #pragma unroll
-          for (int q = 0; q < kGrad; ++q) a[q] += s_g[q * kGPad + p];
+          for (int q = 0; q < kGrad; ++q) a[q] += grad_at(s_g, q, p);
         }
       }
     const TileRange r = tile_range(t, d.tws, d.ths);
@@ -1450,7 +1466,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0, maxJ = 0, maxTiles = 0;
   for (int b = 0; b < n_chars; ++b) {
     const dcx_char_desc& c = chars[b];
-    if (!(c.num_vertices > 0 && c.num_faces > 0 && c.num_nodes > 0 && c.num_joints > 0 && c.num_joints <= 256 &&
+    if (!(c.num_vertices > 0 && c.num_faces > 0 && c.num_faces < (1 << (32 - kTagBits)) && c.num_nodes > 0 &&
+          c.num_joints > 0 && c.num_joints <= 256 &&
           c.num_cameras > 0 && c.width > 0 && c.height > 0 && c.dt_size > 2 && c.dt_size <= 16384 &&
           c.width <= 32767 && c.height <= 32767)) {
       delete p;
