@@ -1064,20 +1064,10 @@ __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
 }
 
 constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
+constexpr int kDtApron = kDtTile + 4;  // the tile plus the 2-pixel border of the 5x5 boundary test
 constexpr int kMaxRegionPts = 2048;
 constexpr int kMaxRegionRows = 64;
-constexpr int kMaxRegionCells = 2048;  // entries of the per-cell LDS offset table (rows x (cols + 1))
-
-__device__ __forceinline__ bool dt_boundary(const unsigned char* img, int S, int ui, int vi) {
-  if (img[(size_t)vi * S + ui] != 127) return false;
-  int counter = 0;
-  for (int j = -2; j <= 2; ++j)
-    for (int i = -2; i <= 2; ++i) {
-      const int yy = vi + j, xx = ui + i;
-      if (yy >= 0 && yy < S && xx >= 0 && xx < S && img[(size_t)yy * S + xx] == 127) counter++;
-    }
-  return counter >= 4;
-}
+constexpr int kMaxRegionCells = 1024;  // entries of the per-cell LDS offset table (rows x (cols + 1))
 
 __device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, float& bd, int& best, float& bx,
                                              float& by) {
@@ -1086,20 +1076,27 @@ __device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, 
 
 // Data->model residual (MultiViewSilhouetteLossGPUOp.cu:142-240) + loss +
 // gradient (atomic, as the reference).  One workgroup per 16x16 tile of DT
-// pixels.  The grid cells within R of the tile (each cell row of that region
-// is one contiguous range of the grid) are copied to LDS with a per-cell
-// offset table; each boundary pixel then searches rings of cells around its
-// own cell, nearest first, keeping the minimum of (distance, index) with
-// distance < R.  A point in ring k + 1 or beyond is at least k cells away
-// along one axis, so the search stops once the best distance is below that
-// bound (with a margin for float rounding); cells outside the region hold no
-// point within R of the tile.  The result is the brute-force one.  Regions
-// larger than the LDS tables run the same search on the global grid.
+// pixels; the tile and a 2-pixel border of the DT image are staged in LDS for
+// the boundary test (is-boundary: value 127 and at least four 127s in the
+// 5x5 window, out-of-image pixels not counted).  The grid cells within R of
+// the tile (each cell row of that region is one contiguous range of the
+// grid) are copied to LDS with their per-row cumulative offsets.  Each
+// boundary pixel then visits the cell rows nearest first (its own, then +-1,
+// +-2, ...); in a row it scans the one contiguous range of cells whose
+// x-distance lower bound does not exceed the best distance so far, keeping
+// the minimum of (distance, index) with distance < R.  A row whose y-distance
+// lower bound exceeds the best distance ends the search (all later rows are
+// farther).  Points of skipped cells are strictly farther than the result,
+// so it is the brute-force one; bounds carry a margin for float rounding.
+// Regions larger than the LDS tables fall back to a ring search over the
+// global grid.
 __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
-  __shared__ f4 s_pts[kMaxRegionPts];
+  __shared__ f2 s_pxy[kMaxRegionPts];
+  __shared__ int s_pid[kMaxRegionPts];
   __shared__ unsigned int s_coff[kMaxRegionCells];
   __shared__ unsigned int s_rstart[kMaxRegionRows + 1];
   __shared__ unsigned int s_ra[kMaxRegionRows];
+  __shared__ unsigned char s_dt[kDtApron * kDtApron];
   __shared__ unsigned int s_wave[kB / 64];
   __shared__ float s_red[kB / 64];
   const int w = blockIdx.y;
@@ -1108,17 +1105,35 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
   const int tpr = (S + kDtTile - 1) / kDtTile;
   const int tile = blockIdx.x;
   const bool tile_ok = tile < tpr * tpr;
-  const int ui = (tile % tpr) * kDtTile + (int)(threadIdx.x % kDtTile);
-  const int vi = (tile / tpr) * kDtTile + (int)(threadIdx.x / kDtTile);
+  const int tu0 = (tile % tpr) * kDtTile, tv0 = (tile / tpr) * kDtTile;
+  const int ui = tu0 + (int)(threadIdx.x % kDtTile);
+  const int vi = tv0 + (int)(threadIdx.x / kDtTile);
   const unsigned char* img = io.dt + vw.sil0;
-  const bool bnd = tile_ok && ui < S && vi < S && dt_boundary(img, S, ui, vi);
+  if (tile_ok)
+    for (int i = threadIdx.x; i < kDtApron * kDtApron; i += kB) {
+      const int ay = i / kDtApron, ax = i - ay * kDtApron;
+      const int gx = tu0 + ax - 2, gy = tv0 + ay - 2;
+      s_dt[i] = (gx >= 0 && gx < S && gy >= 0 && gy < S) ? img[(size_t)gy * S + gx] : (unsigned char)0;
+    }
+  __syncthreads();
+  bool bnd = false;
+  if (tile_ok && ui < S && vi < S) {
+    const int cx = (int)(threadIdx.x % kDtTile) + 2, cy = (int)(threadIdx.x / kDtTile) + 2;
+    if (s_dt[cy * kDtApron + cx] == 127) {
+      int counter = 0;
+#pragma unroll
+      for (int j = -2; j <= 2; ++j)
+#pragma unroll
+        for (int i = -2; i <= 2; ++i) counter += s_dt[(cy + j) * kDtApron + cx + i] == 127 ? 1 : 0;
+      bnd = counter >= 4;
+    }
+  }
   float lsum = 0.f;
   if (__syncthreads_or(bnd)) {
     const float* cr = io.crops + (size_t)w * 7;
     const float sc = cr[2];
     const float R = 30.f * sc;
     const GridGeomP gg = grid_geom_p(cr);
-    const int tu0 = (tile % tpr) * kDtTile, tv0 = (tile / tpr) * kDtTile;
     const float xlo = ((float)tu0 - cr[0]) * sc + cr[3] - R, xhi = ((float)(tu0 + kDtTile - 1) - cr[0]) * sc + cr[3] + R;
     const float ylo = ((float)tv0 - cr[1]) * sc + cr[4] - R, yhi = ((float)(tv0 + kDtTile - 1) - cr[1]) * sc + cr[4] + R;
     const int cx0 = max(0, (int)floorf((xlo - gg.ox) / gg.cell) - 1);
@@ -1150,7 +1165,9 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       for (unsigned int i = threadIdx.x; i < total; i += kB) {
         int r = 0;
         while (r + 1 < nrows && s_rstart[r + 1] <= i) ++r;
-        s_pts[i] = d.cell_pts[s_ra[r] + (i - s_rstart[r])];
+        const f4 it = d.cell_pts[s_ra[r] + (i - s_rstart[r])];
+        s_pxy[i] = mk2(it.x, it.y);
+        s_pid[i] = __float_as_int(it.z);
       }
     }
     __syncthreads();
@@ -1159,36 +1176,46 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       float bd = R, bx = 0.f, by = 0.f;
       int best = -1;
       const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
-      for (int k = 0; k <= kRing; ++k) {
-        for (int oy = -k; oy <= k; ++oy) {
-          const int yy = cy + oy;
-          if (yy < 0 || yy >= vw.gy) continue;
-          const int step = (oy == -k || oy == k) ? 1 : 2 * k;
-          for (int ox = -k; ox <= k; ox += step) {
-            const int xx = cx + ox;
-            if (xx < 0 || xx >= vw.gx) continue;
-            if (in_lds) {
-              if (xx < cx0 || xx > cx1 || yy < cy0 || yy > cy1) continue;
-              const int ci = (yy - cy0) * (ncols + 1) + (xx - cx0);
-              const unsigned int q1 = s_coff[ci + 1];
-              for (unsigned int q = s_coff[ci]; q < q1; ++q) {
-                const f4 it = s_pts[q];
-                const float dx = it.x - ix, dy = it.y - iy;
-                nearest_take(sqrtf(dx * dx + dy * dy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
-              }
-            } else {
+      if (in_lds) {
+        const float step = gg.cell * (1.f - 1e-4f);  // lower bound of the distance per cell of separation
+        for (int t = 0; t <= 2 * (kRing + 1); ++t) {
+          const int dy = (t & 1) ? (t + 1) >> 1 : -(t >> 1);  // 0, +1, -1, +2, -2, ...
+          const int ady = dy < 0 ? -dy : dy;
+          if ((float)(ady > 0 ? ady - 1 : 0) * step > bd) break;
+          const int yy = cy + dy;
+          if (yy < cy0 || yy > cy1) continue;
+          const int kx = (int)(bd / step) + 1;  // cells with (|xx - cx| - 1) * step <= bd
+          const int xa = max(cx - kx, cx0), xb = min(cx + kx, cx1);
+          if (xa > xb) continue;
+          const int rb = (yy - cy0) * (ncols + 1);
+          const unsigned int q1 = s_coff[rb + xb - cx0 + 1];
+          for (unsigned int q = s_coff[rb + xa - cx0]; q < q1; ++q) {
+            const f2 it = s_pxy[q];
+            const float dx = it.x - ix, dyy = it.y - iy;
+            nearest_take(sqrtf(dx * dx + dyy * dyy), s_pid[q], it.x, it.y, bd, best, bx, by);
+          }
+        }
+      } else {
+        for (int k = 0; k <= kRing; ++k) {
+          for (int oy = -k; oy <= k; ++oy) {
+            const int yy = cy + oy;
+            if (yy < 0 || yy >= vw.gy) continue;
+            const int stp = (oy == -k || oy == k) ? 1 : 2 * k;
+            for (int ox = -k; ox <= k; ox += stp) {
+              const int xx = cx + ox;
+              if (xx < 0 || xx >= vw.gx) continue;
               const long long cell = vw.cell0 + yy * vw.gx + xx;
               const unsigned int q1 = d.cell_off[cell + 1];
               for (unsigned int q = d.cell_off[cell]; q < q1; ++q) {
                 const f4 it = d.cell_pts[q];
-                const float dx = it.x - ix, dy = it.y - iy;
-                nearest_take(sqrtf(dx * dx + dy * dy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+                const float dx = it.x - ix, dyy = it.y - iy;
+                nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
               }
             }
           }
+          const float reach = (float)k * gg.cell * (1.f - 1e-4f);
+          if (bd < reach || reach >= R) break;
         }
-        const float reach = (float)k * gg.cell * (1.f - 1e-4f);
-        if (bd < reach || reach >= R) break;
       }
       if (best >= 0) {
         const float r0 = (bx - ix) / (30.f * sc), r1 = (by - iy) / (30.f * sc);
