@@ -730,6 +730,23 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   const int x = tx0 + px_i, y = ty0 + py_i;
   const bool inimg = x < vw.W && y < vw.H;
 
+  if (n_items == 0) {
+    // no triangle touches the tile: background outputs and the colour loss
+    // against the target, nothing staged
+    float lsum = 0.f;
+    if (inimg) {
+      const long long o = vw.pix0 + (long long)y * vw.W + x;
+      const f3 tgt = ld3(io.target + 3 * o);
+      io.face[o] = -1;
+      io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
+      io.render[3 * o] = 0.f; io.render[3 * o + 1] = 0.f; io.render[3 * o + 2] = 0.f;
+      lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
+    }
+    const float lt_sum = block_sum(lsum, s_red);
+    if (threadIdx.x == 0) d.loss_tile[tile] = lt_sum;
+    return;
+  }
+
   // target colours of the tile plus a kImgFs apron (clamped reads; the apron is
   // only used by pixels whose whole filter window lies inside the image)
   for (int i = threadIdx.x; i < an; i += kTilePix) {
