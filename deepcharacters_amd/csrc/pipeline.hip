@@ -916,13 +916,28 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     __syncthreads();
     if (bj >= 0) u2.srt.order[s_start[bj] + u2.srt.wcnt[wv][bj] + rank] = (unsigned short)pix;
     __syncthreads();
-    // ---- one (triangle, component) pair per work item
-    for (int i = threadIdx.x; i < (int)n_items * kGrad; i += kTilePix) {
-      const int j = i / kGrad, q = i - j * kGrad;
+    // ---- one (triangle, vertex) pair per work item: the vertex's 3 position
+    // and 3 colour components, each summed over the triangle's pixels in
+    // pixel order
+    for (int i = threadIdx.x; i < 3 * (int)n_items; i += kTilePix) {
+      const int j = i / 3, k = i - 3 * j;
       const unsigned int st = s_start[j], en = s_start[j + 1];
-      float a = 0.f;
-      for (unsigned int r = st; r < en; ++r) a += grad_at(s_g, q, u2.srt.order[r]);
-      emit_record(d, io, c, u2.srt.e[j], u2.srt.fj[j], q, a, en > st);
+      float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      for (unsigned int r = st; r < en; ++r) {
+        const int p = u2.srt.order[r];
+        const float w0 = s_g[12 * kGPad + p], w1 = s_g[13 * kGPad + p];
+        const float wk = k == 0 ? w0 : (k == 1 ? w1 : 1.f - w0 - w1);
+        a[0] += s_g[(3 * k) * kGPad + p];
+        a[1] += s_g[(3 * k + 1) * kGPad + p];
+        a[2] += s_g[(3 * k + 2) * kGPad + p];
+        a[3] += s_g[9 * kGPad + p] * wk;
+        a[4] += s_g[10 * kGPad + p] * wk;
+        a[5] += s_g[11 * kGPad + p] * wk;
+      }
+      const unsigned int e = u2.srt.e[j];
+      const int fj = u2.srt.fj[j];
+#pragma unroll
+      for (int cmp = 0; cmp < 6; ++cmp) emit_record(d, io, c, e, fj, 6 * k + cmp, a[cmp], en > st);
     }
     return;
   }
