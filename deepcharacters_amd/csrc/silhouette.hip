@@ -313,6 +313,79 @@ __global__ __launch_bounds__(kB) void boundary_kernel(const int* __restrict__ fa
   out[i] = f;
 }
 
+// ProjectedMeshBoundaryGpu's own coverage raster (Skeletool/Mesh/
+// CUDABasedRasterization.cu:64-140).  The vertices are projected as
+// getCamSpacePoint / projectPointFloat3 do (project_vertex: z clamped to 1e-5
+// when not in front); one lane per (view, face) walks the face's pixel box
+// [min - 0.5, max + 0.5], clamped to the image and truncated, and marks every
+// pixel centre whose 2D barycentrics (uv2barycentric) lie in [-0.001, 1.001].
+// No depth test: the bitmap is the union of the faces' pixels, every writer
+// stores the same byte.
+__global__ __launch_bounds__(kB) void cov_project_kernel(const float* __restrict__ vpos, const float* __restrict__ ext,
+                                                          const float* __restrict__ intr, int B, int C, int N,
+                                                          float* __restrict__ uv) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)B * C * N) return;
+  const int view = (int)(i / N), v = (int)(i - (long long)view * N);
+  const int b = view / C;
+  const f4 p = project_vertex(ext + (size_t)view * 12, intr + (size_t)view * 9, vpos + ((size_t)b * N + v) * 3);
+  uv[2 * i] = p.x;
+  uv[2 * i + 1] = p.y;
+}
+
+__device__ __forceinline__ f3 uv2bary(float u, float v, f2 a, f2 b, f2 c) {
+  const float e2 = ((a.y - b.y) * u + (b.x - a.x) * v + a.x * b.y - b.x * a.y) /
+                   ((a.y - b.y) * c.x + (b.x - a.x) * c.y + a.x * b.y - b.x * a.y);
+  const float e1 = ((a.y - c.y) * u + (c.x - a.x) * v + a.x * c.y - c.x * a.y) /
+                   ((a.y - c.y) * b.x + (c.x - a.x) * b.y + a.x * c.y - c.x * a.y);
+  return mk3(1.f - e2 - e1, e1, e2);
+}
+
+__global__ __launch_bounds__(kB) void cov_raster_kernel(const float* __restrict__ uv, const int* __restrict__ faces,
+                                                         int B, int C, int N, int F, int H, int W,
+                                                         unsigned char* __restrict__ cov) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)B * C * F) return;
+  const int view = (int)(i / F), f = (int)(i - (long long)view * F);
+  const f2* P = (const f2*)uv + (size_t)view * N;
+  const f2 p0 = P[faces[3 * f]], p1 = P[faces[3 * f + 1]], p2 = P[faces[3 * f + 2]];
+  const float bx0 = fmaxf(fminf(p0.x, fminf(p1.x, p2.x)) - 0.5f, 0.f);
+  const float by0 = fmaxf(fminf(p0.y, fminf(p1.y, p2.y)) - 0.5f, 0.f);
+  const float bx1 = fminf(fmaxf(p0.x, fmaxf(p1.x, p2.x)) + 0.5f, (float)(W - 1));
+  const float by1 = fminf(fmaxf(p0.y, fmaxf(p1.y, p2.y)) + 0.5f, (float)(H - 1));
+  unsigned char* cv = cov + (size_t)view * H * W;
+  for (int u = (int)bx0; (float)u <= bx1; ++u)
+    for (int v = (int)by0; (float)v <= by1; ++v) {
+      const f3 e = uv2bary((float)u + 0.5f, (float)v + 0.5f, p0, p1, p2);
+      if (e.x >= -0.001f && e.y >= -0.001f && e.z >= -0.001f && e.x <= 1.001f && e.y <= 1.001f && e.z <= 1.001f)
+        cv[(size_t)v * W + u] = 1;
+    }
+}
+
+// The boundary test of boundary_kernel on the coverage bitmap (1 = covered).
+__global__ __launch_bounds__(kB) void cov_boundary_kernel(const unsigned char* __restrict__ cov,
+                                                           const float* __restrict__ uv, int B, int C, int N, int H,
+                                                           int W, unsigned char* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)B * C * N) return;
+  const int view = (int)(i / N);
+  const float pu = uv[2 * i], pw = uv[2 * i + 1];
+  unsigned char f = 0;
+  if (pu >= 1.f && pu < (float)(W - 1) && pw >= 1.f && pw < (float)(H - 1)) {
+    const int u = (int)pu, w = (int)pw;
+    const unsigned char* cv = cov + (size_t)view * H * W;
+    for (int cc = u - 1; cc <= u + 1; ++cc)
+      for (int r = w - 1; r <= w + 1; ++r)
+        if (!cv[(size_t)r * W + cc]) f = 1;
+  }
+  out[i] = f;
+}
+
+__global__ __launch_bounds__(kB) void fill_u8_kernel(unsigned char* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct SilWs {
@@ -411,6 +484,35 @@ extern "C" dcx_status dcx_camproj_bwd(const float* points_image_grad, const floa
   hipLaunchKernelGGL(camproj_bwd_kernel, dim3(ceil_div((long long)B * N, kB)), dim3(kB), 0, as_stream(stream),
                      points_image_grad, points, extrinsics, intrinsics, B, C, N, points_grad);
   return check_launch("dcx_camproj_bwd");
+}
+
+extern "C" size_t dcx_projected_mesh_boundary_workspace_size(int B, int C, int N, int H, int W) {
+  if (B <= 0 || C <= 0 || N <= 0 || H <= 0 || W <= 0) return 0;
+  return align256(sizeof(float) * 2 * (size_t)B * C * N) + (size_t)B * C * H * W;
+}
+
+extern "C" dcx_status dcx_projected_mesh_boundary(const dcx_topo* topo, int B, int C, int H, int W,
+                                                  const float* points, const float* extrinsics,
+                                                  const float* intrinsics, uint8_t* is_boundary, uint8_t* coverage,
+                                                  void* workspace, size_t workspace_bytes, void* stream) {
+  DCX_REQUIRE(topo && B > 0 && C > 0 && H > 0 && W > 0, "dcx_projected_mesh_boundary: bad sizes");
+  DCX_REQUIRE(points && extrinsics && intrinsics && is_boundary && workspace,
+              "dcx_projected_mesh_boundary: null argument");
+  const int N = topo->N, F = topo->F;
+  DCX_REQUIRE(workspace_bytes >= dcx_projected_mesh_boundary_workspace_size(B, C, N, H, W),
+              "dcx_projected_mesh_boundary: workspace too small");
+  hipStream_t s = as_stream(stream);
+  float* uv = (float*)workspace;
+  unsigned char* cov = coverage ? coverage : (unsigned char*)workspace + align256(sizeof(float) * 2 * (size_t)B * C * N);
+  const long long npix = (long long)B * C * H * W;
+  hipLaunchKernelGGL(fill_u8_kernel, dim3(ceil_div(npix, kB)), dim3(kB), 0, s, cov, npix);
+  hipLaunchKernelGGL(cov_project_kernel, dim3(ceil_div((long long)B * C * N, kB)), dim3(kB), 0, s, points, extrinsics,
+                     intrinsics, B, C, N, uv);
+  hipLaunchKernelGGL(cov_raster_kernel, dim3(ceil_div((long long)B * C * F, kB)), dim3(kB), 0, s, uv, topo->faces, B,
+                     C, N, F, H, W, cov);
+  hipLaunchKernelGGL(cov_boundary_kernel, dim3(ceil_div((long long)B * C * N, kB)), dim3(kB), 0, s, cov, uv, B, C, N,
+                     H, W, is_boundary);
+  return check_launch("dcx_projected_mesh_boundary");
 }
 
 extern "C" dcx_status dcx_mesh_boundary(const int32_t* face, const float* points_image, int B, int C, int N, int H,

@@ -123,6 +123,35 @@ class CameraProjectionFn(torch.autograd.Function):
         return gp, None, None, None, None
 
 
+def projected_mesh_boundary_gpu(points_global_space, extrinsics, intrinsics, topology, use_gap_detection=False,
+                                number_batches=None, number_cameras=None, render_u=None, render_v=None,
+                                return_coverage=False):
+    """ProjectedMeshBoundaryGpu (ProjectedMeshBoundaryGPUOp.cpp:5-18,139-153):
+    is_boundary [B,C,N] (bool) from the op's own coverage raster of the mesh
+    (`topology`, a renderer.Topology, in place of mesh_file_path_boundary_check).
+    use_gap_detection is accepted and, as in the reference kernel, has no effect.
+    With return_coverage, also the coverage bitmap [B,C,H,W] (uint8)."""
+    import ctypes
+    P = dev(points_global_space, "points_global_space")
+    B, N = P.shape[0], P.shape[1]
+    if number_batches is not None and number_batches != B:
+        raise ValueError(f"number_batches={number_batches} but points_global_space has {B}")
+    if N != topology.N:
+        raise ValueError(f"points_global_space has {N} points, the mesh {topology.N}")
+    ext = dev(extrinsics, "extrinsics")
+    C = ext.shape[-1] // 12 if number_cameras is None else number_cameras
+    ext = ext.reshape(B, 12 * C)
+    intr = dev(intrinsics, "intrinsics").reshape(B, 9 * C)
+    W, H = int(render_u), int(render_v)
+    out = torch.empty((B, C, N), dtype=torch.uint8, device=P.device)
+    cov = torch.empty((B, C, H, W), dtype=torch.uint8, device=P.device) if return_coverage else None
+    nbytes = _lib.load().dcx_projected_mesh_boundary_workspace_size(B, C, N, H, W)
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=P.device)
+    _lib.call("dcx_projected_mesh_boundary", topology.handle, B, C, H, W, ptr(P), ptr(ext), ptr(intr), ptr(out),
+              ptr(cov), ptr(ws), ctypes.c_size_t(nbytes), stream_ptr())
+    return (out.bool(), cov) if return_coverage else out.bool()
+
+
 def mesh_boundary_from_faces(face_buffer, points_image_space):
     """is_boundary [B,C,N] (uint8) from a face buffer [B,C,H,W] and projected points [B,C,N,2]."""
     f = dev(face_buffer, "face_buffer", torch.int32)

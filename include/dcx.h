@@ -359,6 +359,23 @@ dcx_status dcx_camproj_bwd(const float* points_image_grad, const float* points,
 dcx_status dcx_mesh_boundary(const int32_t* face, const float* points_image, int B, int C, int N,
                              int H, int W, uint8_t* is_boundary, void* stream);
 
+/* ProjectedMeshBoundaryGpu itself (ProjectedMeshBoundaryGPUOp.cpp:139-153 ->
+ * Skeletool/Mesh/CUDABasedRasterization.cu:64-200): the op's own coverage
+ * raster -- vertices projected with the camera (z clamped to 1e-5 when not
+ * in front), per face every pixel centre of its box [min - 0.5, max + 0.5]
+ * (clamped, truncated) whose 2D barycentrics lie in [-0.001, 1.001], no
+ * depth test -- then the 3x3-window boundary test above.  The mesh comes as
+ * a topology instead of the op's mesh_file_path_boundary_check attribute;
+ * use_gap_detection has no effect in the reference kernel and has no
+ * counterpart.
+ *   points [B,N,3]; extrinsics [B,12C]; intrinsics [B,9C]; H x W = render_v x render_u
+ *   is_boundary [B,C,N] (0/1); coverage [B,C,H,W] (0/1, may be NULL: then
+ *   kept in the workspace) */
+size_t dcx_projected_mesh_boundary_workspace_size(int B, int C, int N, int H, int W);
+dcx_status dcx_projected_mesh_boundary(const dcx_topo* topo, int B, int C, int H, int W, const float* points,
+                                       const float* extrinsics, const float* intrinsics, uint8_t* is_boundary,
+                                       uint8_t* coverage, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Distance transforms and the native DT silhouette loss                     */
 /* ------------------------------------------------------------------------ */

@@ -292,6 +292,24 @@ def camproj_bwd(g, points, ext, intr):
     return out
 
 
+def mesh_coverage(faces, vpos, ext, intr, H, W):
+    """ProjectedMeshBoundaryGpu's coverage raster: (proj [B,C,N,2] float32, coverage [B,C,H,W] uint8)."""
+    faces = _c(faces, np.int32).reshape(-1, 3)
+    B, N = vpos.shape[0], vpos.shape[1]
+    C = ext.shape[1] // 12
+    proj = np.zeros((B, C, N, 2), np.float32)
+    cov = np.zeros((B, C, H, W), np.uint8)
+    _call("orc_mesh_coverage", faces, faces.shape[0], _c(vpos, np.float32), _c(ext, np.float32),
+          _c(intr, np.float32), B, C, N, H, W, proj, cov)
+    return proj, cov
+
+
+def projected_mesh_boundary(faces, vpos, ext, intr, H, W):
+    """ProjectedMeshBoundaryGpu: is_boundary [B,C,N] uint8 from its own coverage raster."""
+    proj, cov = mesh_coverage(faces, vpos, ext, intr, H, W)
+    return boundary_flags(proj, cov)
+
+
 def boundary_flags(proj_uv, covered):
     """ProjectedMeshBoundaryGpu semantics on a coverage bitmap [B,C,H,W] -> bool [B,C,N]."""
     B, C, N = proj_uv.shape[0], proj_uv.shape[1], proj_uv.shape[2]

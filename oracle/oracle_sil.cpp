@@ -17,6 +17,14 @@
 //               bitmap: a vertex is on the boundary when its pixel's 3x3
 //               window lies in the image and holds an uncovered pixel
 //               (Skeletool/Mesh/CUDABasedRasterization.cu:168-200)
+//   coverage    that op's own coverage bitmap (same file, :64-140): vertices
+//               projected as getCamSpacePoint / projectPointFloat3 do (z
+//               clamped to 1e-5 when not in front), per face the pixel box
+//               [min - 0.5, max + 0.5] clamped to the image and truncated to
+//               int, a pixel centre covered when its 2D barycentrics
+//               (uv2barycentric, RendererUtil-style float formulas) all lie in
+//               [-0.001, 1.001].  No depth test: the bitmap is the union of
+//               the faces' pixels.
 //
 // Pinning (tests/test_oracle_sil.py): hand-built DT images (constant, ramps,
 // sign flips, boundary bands), crop mapping, brute-force nearest search vs a
@@ -199,14 +207,70 @@ void boundary_flags(const float* proj_uv, const uint8_t* covered, int B, int C, 
     for (int c = 0; c < C; ++c)
       for (int v = 0; v < N; ++v) {
         const size_t pi = ((size_t)b * C + c) * N + v;
-        const int u = (int)proj_uv[2 * pi], w = (int)proj_uv[2 * pi + 1];
+        const float pu = proj_uv[2 * pi], pw = proj_uv[2 * pi + 1];
         uint8_t f = 0;
-        if (u - 1 >= 0 && u + 1 < W && w - 1 >= 0 && w + 1 < H)
+        // (int)p - 1 >= 0 && (int)p + 1 < W, tested in float so that huge or
+        // NaN positions never reach the integer conversion
+        if (pu >= 1.f && pu < (float)(W - 1) && pw >= 1.f && pw < (float)(H - 1)) {
+          const int u = (int)pu, w = (int)pw;
           for (int cc = u - 1; cc <= u + 1; ++cc)
             for (int r = w - 1; r <= w + 1; ++r)
               if (!covered[(((size_t)b * C + c) * H + r) * W + cc]) f = 1;
+        }
         out[pi] = f;
       }
+}
+
+// 2D barycentrics of (u, v) in the projected triangle (uv2barycentric).
+static void uv2bary(float u, float v, const float* a, const float* b, const float* c, float* e) {
+  const float e2 = ((a[1] - b[1]) * u + (b[0] - a[0]) * v + a[0] * b[1] - b[0] * a[1]) /
+                   ((a[1] - b[1]) * c[0] + (b[0] - a[0]) * c[1] + a[0] * b[1] - b[0] * a[1]);
+  const float e1 = ((a[1] - c[1]) * u + (c[0] - a[0]) * v + a[0] * c[1] - c[0] * a[1]) /
+                   ((a[1] - c[1]) * b[0] + (c[0] - a[0]) * b[1] + a[0] * c[1] - c[0] * a[1]);
+  e[0] = 1.f - e2 - e1;
+  e[1] = e1;
+  e[2] = e2;
+}
+
+void mesh_coverage(const int32_t* faces, int F, const float* vpos, const float* ext, const float* intr, int B, int C,
+                   int N, int H, int W, float* proj, uint8_t* cov) {
+  std::memset(cov, 0, (size_t)B * C * H * W);
+  for (int b = 0; b < B; ++b)
+    for (int c = 0; c < C; ++c) {
+      const float* E = ext + ((size_t)b * C + c) * 12;
+      const float* K = intr + ((size_t)b * C + c) * 9;
+      float* P = proj + ((size_t)b * C + c) * N * 2;
+      for (int v = 0; v < N; ++v) {
+        const float* X = vpos + ((size_t)b * N + v) * 3;
+        const float xc = E[0] * X[0] + E[1] * X[1] + E[2] * X[2] + E[3];
+        const float yc = E[4] * X[0] + E[5] * X[1] + E[6] * X[2] + E[7];
+        const float zc = E[8] * X[0] + E[9] * X[1] + E[10] * X[2] + E[11];
+        const float x = K[0] * xc + K[1] * yc + K[2] * zc;
+        const float y = K[3] * xc + K[4] * yc + K[5] * zc;
+        float z = K[6] * xc + K[7] * yc + K[8] * zc;
+        if (!(z > 0.0000001f)) z = 0.00001f;
+        P[2 * v] = x / z;
+        P[2 * v + 1] = y / z;
+      }
+      uint8_t* cv = cov + ((size_t)b * C + c) * H * W;
+      for (int f = 0; f < F; ++f) {
+        const float* p0 = P + 2 * faces[3 * f];
+        const float* p1 = P + 2 * faces[3 * f + 1];
+        const float* p2 = P + 2 * faces[3 * f + 2];
+        const float bx0 = std::fmax(std::fmin(p0[0], std::fmin(p1[0], p2[0])) - 0.5f, 0.f);
+        const float by0 = std::fmax(std::fmin(p0[1], std::fmin(p1[1], p2[1])) - 0.5f, 0.f);
+        const float bx1 = std::fmin(std::fmax(p0[0], std::fmax(p1[0], p2[0])) + 0.5f, (float)(W - 1));
+        const float by1 = std::fmin(std::fmax(p0[1], std::fmax(p1[1], p2[1])) + 0.5f, (float)(H - 1));
+        for (int u = (int)bx0; (float)u <= bx1; ++u)
+          for (int v = (int)by0; (float)v <= by1; ++v) {
+            float e[3];
+            uv2bary((float)u + 0.5f, (float)v + 0.5f, p0, p1, p2, e);
+            if (e[0] >= -0.001f && e[1] >= -0.001f && e[2] >= -0.001f && e[0] <= 1.001f && e[1] <= 1.001f &&
+                e[2] <= 1.001f)
+              cv[(size_t)v * W + u] = 1;
+          }
+      }
+    }
 }
 
 }  // namespace orc
@@ -231,6 +295,10 @@ extern "C" void orc_camproj_f32(const float* P, const float* D, const float* ext
 extern "C" void orc_camproj_bwd_f64(const double* g, const double* P, const double* ext, const double* intr, int B,
                                     int C, int N, double* out) {
   orc::camproj_bwd<double>(g, P, ext, intr, B, C, N, out);
+}
+extern "C" void orc_mesh_coverage(const int32_t* faces, int F, const float* vpos, const float* ext, const float* intr,
+                                  int B, int C, int N, int H, int W, float* proj, uint8_t* cov) {
+  orc::mesh_coverage(faces, F, vpos, ext, intr, B, C, N, H, W, proj, cov);
 }
 extern "C" void orc_boundary_flags(const float* proj_uv, const uint8_t* covered, int B, int C, int N, int H, int W,
                                    uint8_t* out) {

@@ -109,3 +109,31 @@ def test_mesh_boundary_from_face_buffer():
     ref = orc.boundary_flags(uv.cpu().numpy(), (face.cpu().numpy() >= 0).astype(np.uint8))
     assert np.array_equal(flags.cpu().numpy().astype(bool), ref)
     assert 0 < ref.mean() < 0.5
+
+
+@pytest.mark.parametrize("near", [False, True])
+def test_projected_mesh_boundary_op_matches_oracle(near):
+    """ProjectedMeshBoundaryGpu with its own coverage raster: coverage bitmap
+    and boundary flags bit-exact against the oracle, two poses (B=2), three
+    cameras; with `near`, one camera sits inside the body's reach so some
+    vertices are behind it (projections clamped to z = 1e-5, huge boxes)."""
+    ch = scenes.ellipsoid_character(3, scale=(0.5, 0.4, 0.3), seed=4)
+    rng = np.random.default_rng(9)
+    H, W = 72, 96
+    ext, intr = scenes.ring_cameras(3, 0.45 if near else 2.5, 80.0, W, H, rng=rng, phase=0.7)
+    V = np.stack([ch.verts, ch.verts * np.array([1.1, 0.9, 1.0]) + 0.02]).astype(np.float32)
+    ext2 = np.tile(ext.reshape(1, -1), (2, 1)).astype(np.float32)
+    intr2 = np.tile(intr.reshape(1, -1), (2, 1)).astype(np.float32)
+    topo = renderer.Topology(ch.faces, ch.N)
+    flags, cov = silhouette.projected_mesh_boundary_gpu(_t(V), _t(ext2), _t(intr2), topo, number_batches=2,
+                                                         number_cameras=3, render_u=W, render_v=H,
+                                                         return_coverage=True)
+    proj, ocov = orc.mesh_coverage(ch.faces, V, ext2, intr2, H, W)
+    oflags = orc.boundary_flags(proj, ocov)
+    assert np.array_equal(cov.cpu().numpy(), ocov)
+    assert np.array_equal(flags.cpu().numpy().astype(np.uint8), oflags)
+    assert ocov.any()
+    if near:
+        assert (proj[np.isfinite(proj)] > 1e4).any()  # clamped projections of vertices behind a camera
+    else:
+        assert oflags.any() and not oflags.all()

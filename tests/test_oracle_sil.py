@@ -163,3 +163,94 @@ def test_boundary_flags():
     uv = np.array([[[[10.5, 10.5], [6.2, 9.9], [17.9, 14.9], [0.5, 0.5], [12.0, 14.0]]]], np.float32)
     f = orc.boundary_flags(uv, cov)
     assert f.tolist() == [[[False, True, True, False, True]]]
+
+
+# --- ProjectedMeshBoundaryGpu's own coverage raster -------------------------
+
+def _pinhole(f=1.0, cx=0.0, cy=0.0):
+    """Identity extrinsics, K = [[f,0,cx],[0,f,cy],[0,0,1]]: a point (x, y, 1)
+    projects to (f x + cx, f y + cy)."""
+    ext = np.array([[1.0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]])
+    intr = np.array([[f, 0, cx, 0, f, cy, 0, 0, 1]])
+    return ext, intr
+
+
+def _bary64(p, a, b, c):
+    T = np.array([[a[0] - c[0], b[0] - c[0]], [a[1] - c[1], b[1] - c[1]]])
+    l0, l1 = np.linalg.solve(T, np.asarray(p, float) - np.asarray(c, float))
+    return np.array([l0, l1, 1 - l0 - l1])
+
+
+@pytest.mark.parametrize("tri", [[(3.2, 4.1), (27.6, 9.3), (12.4, 28.8)],
+                                 [(-500.0, 15.3), (600.0, 17.1), (50.0, 900.0)]])
+def test_mesh_coverage_is_the_barycentric_band_inside_the_pixel_box(tri):
+    """A pixel is covered iff its centre's barycentrics lie in [-0.001, 1.001]
+    (checked in FP64 away from the band edges) and the pixel lies in the box
+    [min - 0.5, max + 0.5] truncated.  The second triangle is large (its
+    near-horizontal top edge crosses the image), so the band reaches ~0.9 px
+    past that edge."""
+    H = W = 32
+    V = np.array([[x, y, 1.0] for x, y in tri])[None]
+    ext, intr = _pinhole()
+    proj, cov = orc.mesh_coverage(np.array([[0, 1, 2]]), V, ext, intr, H, W)
+    np.testing.assert_allclose(proj[0, 0], np.array(tri), rtol=1e-6)
+    xs, ys = [p[0] for p in tri], [p[1] for p in tri]
+    bx0, by0 = int(max(min(xs) - 0.5, 0)), int(max(min(ys) - 0.5, 0))
+    bx1, by1 = min(max(xs) + 0.5, W - 1), min(max(ys) + 0.5, H - 1)
+    checked = band = 0
+    for y in range(H):
+        for x in range(W):
+            e = _bary64((x + 0.5, y + 0.5), *tri)
+            lo, hi = e.min(), e.max()
+            if abs(lo + 0.001) < 1e-5 or abs(hi - 1.001) < 1e-5:
+                continue  # FP32 and FP64 may disagree on the band edge
+            in_box = bx0 <= x <= bx1 and by0 <= y <= by1
+            expect = in_box and lo >= -0.001 and hi <= 1.001
+            assert cov[0, 0, y, x] == expect, (x, y, e)
+            checked += 1
+            band += int(expect and lo < 0)
+    assert checked > 900
+    if tri[1][0] > 100:
+        assert band >= 20  # pixels covered only through the tolerance band
+
+
+def test_mesh_coverage_has_no_depth_test_and_degenerate_faces_cover_nothing():
+    H = W = 24
+    ext, intr = _pinhole()
+    V = np.array([[[2.0, 2.0, 1.0], [20.0, 3.0, 1.0], [5.0, 20.0, 1.0],
+                   [4.0, 4.0, 3.0], [22.0, 6.0, 3.0], [10.0, 22.0, 3.0],
+                   [1.0, 1.0, 1.0], [10.0, 10.0, 1.0], [19.0, 19.0, 1.0]]])
+    V[0, 3:6, :2] *= 3.0  # the far triangle projects to the same region
+    _, c_near = orc.mesh_coverage(np.array([[0, 1, 2]]), V, ext, intr, H, W)
+    _, c_far = orc.mesh_coverage(np.array([[3, 4, 5]]), V, ext, intr, H, W)
+    _, c_both = orc.mesh_coverage(np.array([[0, 1, 2], [3, 4, 5]]), V, ext, intr, H, W)
+    assert np.array_equal(c_both, c_near | c_far)
+    assert (c_near & c_far).any() and (c_far & ~c_near).any()
+    _, c_line = orc.mesh_coverage(np.array([[6, 7, 8]]), V, ext, intr, H, W)  # collinear
+    assert not c_line.any()
+
+
+def test_projected_mesh_boundary_window_rule():
+    """A planar quad covering pixels [8, 24) x [8, 24) of a 32x32 image, with
+    vertices on a 5x5 grid: interior grid vertices see only covered pixels,
+    vertices on the quad's border see uncovered ones, and a vertex whose 3x3
+    window leaves the image is never a boundary vertex."""
+    H = W = 32
+    ext, intr = _pinhole()
+    g = np.linspace(8.0, 24.0, 5)
+    X, Y = np.meshgrid(g, g)
+    V = np.stack([X.ravel(), Y.ravel(), np.ones(25)], 1)
+    faces = []
+    for i in range(4):
+        for j in range(4):
+            a, b, c, d = i * 5 + j, i * 5 + j + 1, (i + 1) * 5 + j, (i + 1) * 5 + j + 1
+            faces += [(a, b, d), (a, d, c)]
+    extra = np.array([[0.5, 16.0, 1.0], [16.0, 31.2, 1.0]])  # windows leave the image
+    V = np.concatenate([V, extra])[None]
+    flags = orc.projected_mesh_boundary(np.array(faces), V, ext, intr, H, W)[0, 0]
+    grid = flags[:25].reshape(5, 5)
+    assert not grid[1:4, 1:4].any()
+    border = np.ones((5, 5), bool)
+    border[1:4, 1:4] = False
+    assert grid[border].all()
+    assert not flags[25] and not flags[26]
