@@ -64,9 +64,10 @@ __device__ __forceinline__ A34 a34_mul(const A34& a, const A34& b) {
 
 // One workgroup per pose: lane 0 walks the tree parents-first into LDS (the
 // chain is short and sequential), then the lanes write joint positions and
-// axes and convert the skinning transforms to dual quaternions.
+// axes and the skinning transforms, as dual quaternions (out_dq) and/or as
+// rotation + translation (out_T, the linear-blend-skinning form).
 __global__ __launch_bounds__(kFkBlock) void fk_kernel(SkelArgs s, const float* dofs, float* out_dq,
-                                                       float* out_jpos, float* out_jaxis) {
+                                                       float* out_jpos, float* out_jaxis, float* out_T) {
   __shared__ float s_g[kMaxFkJoints * 12];
   const int b = blockIdx.x;
   if (threadIdx.x == 0) {
@@ -114,6 +115,15 @@ __global__ __launch_bounds__(kFkBlock) void fk_kernel(SkelArgs s, const float* d
     for (int r = 0; r < 3; ++r)
       T.m[4 * r + 3] = G.m[4 * r + 3] + G.m[4 * r] * sc * Bd.m[3] + G.m[4 * r + 1] * sc * Bd.m[7] +
                        G.m[4 * r + 2] * sc * Bd.m[11];
+    if (out_T) {
+      // [R row-major | t], the layout of LbSkinningGpu's transformation
+      float* o = out_T + ((size_t)b * s.Js + i) * 12;
+      for (int r = 0; r < 3; ++r) {
+        o[3 * r] = T.m[4 * r]; o[3 * r + 1] = T.m[4 * r + 1]; o[3 * r + 2] = T.m[4 * r + 2];
+        o[9 + r] = T.m[4 * r + 3];
+      }
+    }
+    if (!out_dq) continue;
     // unit quaternion (x, y, z, w) of the rotation part: the largest of
     // 4w^2, 4x^2, 4y^2, 4z^2 picks the branch (Shepperd's method)
     const float m00 = T.m[0], m01 = T.m[1], m02 = T.m[2], m10 = T.m[4], m11 = T.m[5], m12 = T.m[6];
@@ -217,8 +227,20 @@ extern "C" dcx_status dcx_fk(const dcx_skeleton* s, int B, const float* dofs, fl
   const SkelArgs a{s->J, s->D, s->Js, s->parent, s->type, s->dof, s->offset, s->axis, s->skin_joint, s->skin_scale,
                    s->skin_bind};
   hipLaunchKernelGGL(fk_kernel, dim3(B), dim3(kFkBlock), 0, as_stream(stream), a, dofs, dual_quaternions, joint_pos,
-                     joint_axis);
+                     joint_axis, (float*)nullptr);
   return check_launch("dcx_fk");
+}
+
+extern "C" dcx_status dcx_fk_transforms(const dcx_skeleton* s, int B, const float* dofs, float* transforms,
+                                        float* joint_pos, float* joint_axis, void* stream) {
+  DCX_REQUIRE(s && B > 0, "dcx_fk_transforms: bad arguments");
+  DCX_REQUIRE(dofs || s->D == 0, "dcx_fk_transforms: null dofs");
+  DCX_REQUIRE(transforms && joint_pos && joint_axis, "dcx_fk_transforms: null output");
+  const SkelArgs a{s->J, s->D, s->Js, s->parent, s->type, s->dof, s->offset, s->axis, s->skin_joint, s->skin_scale,
+                   s->skin_bind};
+  hipLaunchKernelGGL(fk_kernel, dim3(B), dim3(kFkBlock), 0, as_stream(stream), a, dofs, (float*)nullptr, joint_pos,
+                     joint_axis, transforms);
+  return check_launch("dcx_fk_transforms");
 }
 
 extern "C" dcx_status dcx_dof_influence(const dcx_skeleton* s, int N, int K, const int32_t* joint_idx,

@@ -228,6 +228,32 @@ def dq_skinning_gpu_dofs(skeleton: Skeleton, dofs: torch.Tensor, skinning_weight
     return pos, nrm, jp, ja, dq_out, w_out
 
 
+def skeleton_fk_transforms(skeleton: Skeleton, dofs: torch.Tensor):
+    """dofs [B,D] -> (transformation [B,Js,12] (R row-major, t), global_joint_position
+    [B,J,3], global_joint_axis [B,J,3]): the forward kinematics of LbSkinningGpu."""
+    d = dev(dofs, "dofs")
+    B = d.shape[0]
+    if d.shape[1] != skeleton.D:
+        raise ValueError(f"dofs has {d.shape[1]} columns, the skeleton {skeleton.D}")
+    tr = torch.empty((B, skeleton.Js, 12), device=d.device)
+    jp = torch.empty((B, skeleton.J, 3), device=d.device)
+    ja = torch.empty((B, skeleton.J, 3), device=d.device)
+    _lib.call("dcx_fk_transforms", skeleton.handle, B, ptr(d), ptr(tr), ptr(jp), ptr(ja), stream_ptr())
+    return tr, jp, ja
+
+
+def lb_skinning_gpu_dofs(skeleton: Skeleton, dofs: torch.Tensor, skinning_weights: torch.Tensor,
+                         joint_idx: torch.Tensor, rest_pos: torch.Tensor, rest_nrm: torch.Tensor | None,
+                         displacement: torch.Tensor):
+    """LbSkinningGpu (LBSkinningGPUOp.cpp:5-16): (dofs, skinning_weights,
+    displacement) -> (skinned_vertex_positions, skinned_vertex_normals,
+    global_joint_position, global_joint_axis, transformation,
+    output_skinning_weights), with the forward kinematics on the GPU."""
+    tr, jp, ja = skeleton_fk_transforms(skeleton, dofs)
+    pos, nrm, tr_out, w_out = lb_skinning_gpu(tr, skinning_weights, joint_idx, rest_pos, rest_nrm, displacement)
+    return pos, nrm, jp, ja, tr_out, w_out
+
+
 def dq_skinning_gpu_grad_dofs(skinned_vertex_positions_grad, skinned_vertex_positions, global_joint_position,
                               global_joint_axis, dual_quaternions, skinning_weights, joint_idx, rest_pos,
                               influence: torch.Tensor, ref_compat: bool = True):

@@ -437,6 +437,13 @@ dcx_status dcx_skeleton_destroy(dcx_skeleton* skeleton);
  * its global_joint_position / global_joint_axis outputs. */
 dcx_status dcx_fk(const dcx_skeleton* skeleton, int B, const float* dofs, float* dual_quaternions, float* joint_pos,
                   float* joint_axis, void* stream);
+/* dofs [B,D] -> skinning-joint transforms [B,Js,12] (R row-major, then t:
+ * the rotation part of G_joint * diag(scale) * T_bind and its translation),
+ * global joint positions and axes: the per-batch host FK of LbSkinningGpu
+ * (LBSkinningGPUOp.cpp:263-345) and its transformation /
+ * global_joint_position / global_joint_axis outputs. */
+dcx_status dcx_fk_transforms(const dcx_skeleton* skeleton, int B, const float* dofs, float* transforms,
+                             float* joint_pos, float* joint_axis, void* stream);
 
 /* HOST: the DOF influence table of DqSkinningGpuGrad
  * (DQSkinningGPUOpGrad.cpp:90-217): rows (vertex, joint type, joint, weight),

@@ -323,8 +323,9 @@ def boundary_flags(proj_uv, covered):
 # Skeleton forward kinematics and the DOF influence table
 # ----------------------------------------------------------------------------
 
-def fk(skel, dofs, dtype=np.float64):
-    """(dual quaternions [B,Js,8], joint positions [B,J,3], joint axes [B,J,3]).
+def fk(skel, dofs, dtype=np.float64, transforms=False):
+    """(dual quaternions [B,Js,8], joint positions [B,J,3], joint axes [B,J,3]),
+    plus the skinning transforms [B,Js,12] (R row-major, t) with transforms=True.
     skel: dict with parent, type, dof, offset, axis, skin_joint, skin_scale, skin_bind."""
     dofs = np.atleast_2d(np.asarray(dofs))
     B, D = dofs.shape
@@ -333,14 +334,15 @@ def fk(skel, dofs, dtype=np.float64):
     dq = np.zeros((B, Js, 8), dtype)
     jp = np.zeros((B, J, 3), dtype)
     ja = np.zeros((B, J, 3), dtype)
+    T = np.zeros((B, Js, 12), dtype)
     scale = skel.get("skin_scale")
     if scale is None:
         scale = np.ones(Js)
     _call(f"orc_fk_{_suf(dtype)}", B, J, D, _c(skel["parent"], np.int32), _c(skel["type"], np.int32),
           _c(skel["dof"], np.int32), _c(skel["offset"], dtype), _c(skel["axis"], dtype), Js,
           _c(skel["skin_joint"], np.int32), _c(scale, dtype), _c(skel["skin_bind"], dtype), _c(dofs, dtype), dq, jp,
-          ja)
-    return dq, jp, ja
+          ja, T)
+    return (dq, jp, ja, T) if transforms else (dq, jp, ja)
 
 
 def dof_influence(skel, joint_idx, weights, max_entries=None):

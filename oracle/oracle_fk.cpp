@@ -5,8 +5,10 @@
 // update the kinematic tree, for every skinning joint form
 //   T_i = G_joint(i) * diag(scale_i) * T_bind(i)
 // and convert its rotation and translation into a unit dual quaternion
-// (DualQuaternion.h:58-69); also the global joint positions and axes the
-// gradient op consumes (DQSkinningGPUOp.cpp:309-345).  And the DOF
+// (DualQuaternion.h:58-69), or, for LbSkinningGpu (LBSkinningGPUOp.cpp:263-
+// 345), keep it as rotation (Eigen's rotation(): the scale-free part) plus
+// translation; also the global joint positions and axes the gradient op
+// consumes (DQSkinningGPUOp.cpp:309-345).  And the DOF
 // influence table of DqSkinningGpuGrad (DQSkinningGPUOpGrad.cpp:90-217).
 //
 // Kinematic model: joints ordered parents-first; joint j has a parent (-1 for
@@ -82,7 +84,7 @@ void quat_of(const Aff<T>& A, T q[4]) {
 template <class T>
 void fk(int B, int J, int D, const int32_t* parent, const int32_t* type, const int32_t* dof, const T* offset,
         const T* axis, int Js, const int32_t* skin_joint, const T* skin_scale, const T* skin_bind, const T* dofs,
-        T* out_dq, T* out_jpos, T* out_jaxis) {
+        T* out_dq, T* out_jpos, T* out_jaxis, T* out_T) {
   std::vector<Aff<T>> G(J);
   for (int b = 0; b < B; ++b) {
     for (int j = 0; j < J; ++j) {
@@ -115,6 +117,13 @@ void fk(int B, int J, int D, const int32_t* parent, const int32_t* type, const i
         for (int k = 0; k < 3; ++k) t += Gj.m[4 * r + k] * s * Bd.m[4 * k + 3];
         Rt.m[4 * r + 3] = t;
       }
+      if (out_T) {  // LbSkinningGpu's transformation: R row-major, then t
+        T* o = out_T + ((size_t)b * Js + i) * 12;
+        for (int r = 0; r < 3; ++r) {
+          for (int k = 0; k < 3; ++k) o[3 * r + k] = Rt.m[4 * r + k];
+          o[9 + r] = Rt.m[4 * r + 3];
+        }
+      }
       T q[4];
       quat_of(Rt, q);
       const T tx = Rt.m[3], ty = Rt.m[7], tz = Rt.m[11];
@@ -137,9 +146,9 @@ void fk(int B, int J, int D, const int32_t* parent, const int32_t* type, const i
   extern "C" void orc_fk_##SUF(int B, int J, int D, const int32_t* parent, const int32_t* type, const int32_t* dof, \
                                const T* offset, const T* axis, int Js, const int32_t* skin_joint,                 \
                                const T* skin_scale, const T* skin_bind, const T* dofs, T* out_dq, T* out_jpos,   \
-                               T* out_jaxis) {                                                                     \
+                               T* out_jaxis, T* out_T) {                                                           \
     orc::fk<T>(B, J, D, parent, type, dof, offset, axis, Js, skin_joint, skin_scale, skin_bind, dofs, out_dq,       \
-               out_jpos, out_jaxis);                                                                               \
+               out_jpos, out_jaxis, out_T);                                                                        \
   }
 ORC_FK(double, f64)
 ORC_FK(float, f32)

@@ -61,6 +61,28 @@ def test_dof_skinning_pair_matches_oracle():
     assert _rel(g_w.cpu().numpy(), ref_gw) < 1e-4
 
 
+def test_lbs_dof_op_matches_oracle():
+    """LbSkinningGpu with its forward kinematics on the GPU: the transforms,
+    joint positions / axes and the skinned mesh (with a displacement)
+    against the FP64 oracle."""
+    desc = scenes.body_skeleton()
+    sk = skinning.Skeleton(desc)
+    ch = scenes.capsule_body(3000, seed=4, ed_nodes=0)
+    rng = np.random.default_rng(6)
+    B = 2
+    dofs = scenes.random_dofs(rng, desc, B, 0.8, 0.05).astype(np.float32)
+    w = np.broadcast_to(ch.joint_w, (B,) + ch.joint_w.shape).astype(np.float32)
+    disp = rng.normal(0, 0.005, (B, ch.N, 3)).astype(np.float32)
+    pos, nrm, jp, ja, tr, w_out = skinning.lb_skinning_gpu_dofs(sk, _t(dofs), _t(w), _t(ch.joint_idx, torch.int32),
+                                                                _t(ch.verts), _t(ch.normals), _t(disp))
+    _, rjp, rja, rT = orc.fk(desc, dofs.astype(np.float64), transforms=True)
+    assert _rel(tr.cpu().numpy(), rT) < 1e-5
+    assert _rel(jp.cpu().numpy(), rjp) < 1e-5 and _rel(ja.cpu().numpy(), rja) < 1e-5
+    rpos, rnrm = orc.lbs_fwd(rT, w, ch.joint_idx, ch.verts, ch.normals, disp)
+    assert _rel(pos.cpu().numpy(), rpos) < 1e-5 and _rel(nrm.cpu().numpy(), rnrm) < 1e-5
+    assert torch.equal(w_out, _t(w))
+
+
 def test_fk_rejects_bad_trees():
     desc = scenes.body_skeleton()
     bad = dict(desc)

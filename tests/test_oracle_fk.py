@@ -145,3 +145,31 @@ def test_rigid_dof_gradient_matches_fd_for_rigidly_attached_vertices():
         e[0, d] = h
         fd = (energy(dofs + e) - energy(dofs - e)) / (2 * h)
         assert abs(gd[0, d] - fd) <= 1e-6 * max(1.0, abs(fd)), (d, gd[0, d], fd)
+
+
+def test_lbs_transforms_are_the_dual_quaternions_as_matrices():
+    """The LbSkinningGpu form of the skinning transforms equals the DQ form:
+    R is the matrix of the DQ's rotation quaternion, t = 2 q_d conj(q_r); and
+    both skinning methods map a vertex bound to a single joint identically."""
+    sk = scenes.body_skeleton()
+    dofs = scenes.random_dofs(np.random.default_rng(8), sk, 3, 1.2, 0.1)
+    dq, _, _, T = orc.fk(sk, dofs, transforms=True)
+    R = T[..., :9].reshape(dofs.shape[0], -1, 3, 3)
+    t = T[..., 9:]
+    for b in range(dofs.shape[0]):
+        np.testing.assert_allclose(R[b], scenes.quat_to_mat(dq[b, :, :4]), atol=1e-12)
+        np.testing.assert_allclose(np.einsum("jab,jbc->jac", R[b], np.transpose(R[b], (0, 2, 1))),
+                                   np.broadcast_to(np.eye(3), R[b].shape), atol=1e-12)
+        for j in range(dq.shape[1]):
+            np.testing.assert_allclose(t[b, j], dq_apply(dq[b, j], np.zeros(3)), atol=1e-12)
+    Js = dq.shape[1]
+    rng = np.random.default_rng(2)
+    N = 3 * Js
+    jidx = np.stack([np.arange(N) % Js, (np.arange(N) + 1) % Js, (np.arange(N) + 2) % Js,
+                     (np.arange(N) + 3) % Js], 1).astype(np.int32)
+    w = np.zeros((dofs.shape[0], N, 4))
+    w[..., 0] = 1.0  # rigidly bound: one joint per vertex
+    P = rng.normal(size=(N, 3))
+    p_dqs, _ = orc.dqs_fwd(dq, w, jidx, P)
+    p_lbs, _ = orc.lbs_fwd(T, w, jidx, P, np.zeros((N, 3)), np.zeros((dofs.shape[0], N, 3)))
+    np.testing.assert_allclose(p_lbs, p_dqs, atol=1e-12)
