@@ -154,10 +154,81 @@ __global__ __launch_bounds__(kFkBlock) void fk_kernel(SkelArgs s, const float* d
   }
 }
 
+// Pose2EmbeddedGraphCpu (Pose2EmbeddedGraphCPUOp.cpp:129-302) on the GPU: one
+// lane per (pose, node).  The node's base vertex blends its skinning joints'
+// dual quaternions in slot order, signs aligned to the first slot's rotation,
+// normalised by the rotation norm when it exceeds 1e-6; the blended rigid
+// motion is returned as a translation t = 2 q_d conj(q_r) and the XYZ Euler
+// angles of R (x = atan2(R21, R22), y = atan2(-R20, sy), z = atan2(R10, R00),
+// sy = |(R00, R10)|; below sy = 1e-6, x = atan2(-R12, R11) and z = 0), the
+// inverse of the embedded graph's R = Rz Ry Rx.  Joint ids outside [0, Js)
+// contribute nothing; a node whose base vertex is outside [0, N) gets the
+// identity.
+__global__ __launch_bounds__(kFkBlock) void pose_graph_kernel(const float* __restrict__ dq, const int* __restrict__ jidx,
+                                                               const float* __restrict__ jw,
+                                                               const int* __restrict__ node_vertex, int B, int N, int K,
+                                                               int Js, int M, float* __restrict__ out_t,
+                                                               float* __restrict__ out_a) {
+  const long long i = (long long)blockIdx.x * kFkBlock + threadIdx.x;
+  if (i >= (long long)B * M) return;
+  const int b = (int)(i / M), k = (int)(i - (long long)b * M);
+  const int v = node_vertex[k];
+  const float* q = dq + (size_t)b * Js * 8;
+  float r[4] = {0.f, 0.f, 0.f, 0.f}, d[4] = {0.f, 0.f, 0.f, 0.f}, first[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < K && v >= 0 && v < N; ++s) {
+    const int j = jidx[(size_t)v * K + s];
+    if (j < 0 || j >= Js) continue;  // invalid slot: no contribution
+    const float w = jw[(size_t)v * K + s];
+    const float* qj = q + (size_t)j * 8;
+    float sign = 1.f;
+    if (s == 0) {
+      for (int c = 0; c < 4; ++c) first[c] = qj[c];
+    } else if (first[0] * qj[0] + first[1] * qj[1] + first[2] * qj[2] + first[3] * qj[3] < 0.f) {
+      sign = -1.f;
+    }
+    for (int c = 0; c < 4; ++c) {
+      r[c] = r[c] + qj[c] * w * sign;
+      d[c] = d[c] + qj[4 + c] * w * sign;
+    }
+  }
+  const float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+  const float sc = len > 0.000001f ? 1.f / len : 1.f;
+  for (int c = 0; c < 4; ++c) { r[c] = r[c] * sc; d[c] = d[c] * sc; }
+  const float x = r[0], y = r[1], z = r[2], w = r[3];
+  const float twx = 2.f * x * w, twy = 2.f * y * w, twz = 2.f * z * w;
+  const float txx = 2.f * x * x, txy = 2.f * y * x, txz = 2.f * z * x;
+  const float tyy = 2.f * y * y, tyz = 2.f * z * y, tzz = 2.f * z * z;
+  const float R00 = 1.f - tyy - tzz, R10 = txy + twz, R11 = 1.f - txx - tzz, R12 = tyz - twx;
+  const float R20 = txz - twy, R21 = tyz + twx, R22 = 1.f - txx - tyy;
+  float* t = out_t + (size_t)i * 3;
+  t[0] = 2.f * (-d[3] * x + d[0] * w - d[1] * z + d[2] * y);
+  t[1] = 2.f * (-d[3] * y + d[0] * z + d[1] * w - d[2] * x);
+  t[2] = 2.f * (-d[3] * z - d[0] * y + d[1] * x + d[2] * w);
+  const float sy = sqrtf(R00 * R00 + R10 * R10);
+  float* a = out_a + (size_t)i * 3;
+  if (!(sy < 1e-6f)) {
+    a[0] = atan2f(R21, R22); a[1] = atan2f(-R20, sy); a[2] = atan2f(R10, R00);
+  } else {
+    a[0] = atan2f(-R12, R11); a[1] = atan2f(-R20, sy); a[2] = 0.f;
+  }
+}
+
 }  // namespace
 }  // namespace dcx
 
 using namespace dcx;
+
+extern "C" dcx_status dcx_pose_to_graph(const float* dual_quaternions, const int32_t* joint_idx, const float* weights,
+                                        const int32_t* node_vertex, int B, int N, int K, int Js, int M,
+                                        float* skinned_translation, float* skinned_rotation, void* stream) {
+  DCX_REQUIRE(dual_quaternions && joint_idx && weights && node_vertex && skinned_translation && skinned_rotation,
+              "dcx_pose_to_graph: null argument");
+  DCX_REQUIRE(B > 0 && N > 0 && K > 0 && Js > 0 && M > 0, "dcx_pose_to_graph: bad sizes");
+  hipLaunchKernelGGL(pose_graph_kernel, dim3(ceil_div((long long)B * M, kFkBlock)), dim3(kFkBlock), 0,
+                     as_stream(stream), dual_quaternions, joint_idx, weights, node_vertex, B, N, K, Js, M,
+                     skinned_translation, skinned_rotation);
+  return check_launch("dcx_pose_to_graph");
+}
 
 static void skel_free(dcx_skeleton* s) {
   (void)hipFree(s->parent); (void)hipFree(s->type); (void)hipFree(s->dof); (void)hipFree(s->offset);

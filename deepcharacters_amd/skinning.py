@@ -254,6 +254,34 @@ def lb_skinning_gpu_dofs(skeleton: Skeleton, dofs: torch.Tensor, skinning_weight
     return pos, nrm, jp, ja, tr_out, w_out
 
 
+def pose_to_embedded_graph(dual_quaternions: torch.Tensor, joint_idx: torch.Tensor, skinning_weights: torch.Tensor,
+                           node_vertex: torch.Tensor):
+    """Node transforms of the embedded graph from skinning DQs [B,Js,8]: the
+    blend of Pose2EmbeddedGraphCpu (Pose2EmbeddedGraphCPUOp.cpp:129-302) at
+    each node's base vertex.  Returns (d_skinned_translation [B,M,3],
+    d_skinned_rotation [B,M,3], XYZ Euler angles)."""
+    dq = dev(dual_quaternions, "dual_quaternions")
+    B, Js = dq.shape[0], dq.shape[1]
+    ji = dev(joint_idx, "joint_idx", torch.int32)
+    N, K = ji.shape
+    w = dev(skinning_weights, "skinning_weights", shape=(N, K))
+    nv = dev(node_vertex, "node_vertex", torch.int32)
+    M = nv.shape[0]
+    t = torch.empty((B, M, 3), device=dq.device)
+    a = torch.empty((B, M, 3), device=dq.device)
+    _lib.call("dcx_pose_to_graph", ptr(dq), ptr(ji), ptr(w), ptr(nv), B, N, K, Js, M, ptr(t), ptr(a), stream_ptr())
+    return t, a
+
+
+def pose2embedded_graph_gpu(skeleton: Skeleton, dofs: torch.Tensor, joint_idx: torch.Tensor,
+                            skinning_weights: torch.Tensor, node_vertex: torch.Tensor):
+    """Pose2EmbeddedGraphCpu (Pose2EmbeddedGraphCPUOp.cpp:5-12) with the forward
+    kinematics and the node blend on the GPU: dofs [B,D] ->
+    (d_skinned_translation [B,M,3], d_skinned_rotation [B,M,3])."""
+    dq, _, _ = skeleton_fk(skeleton, dofs)
+    return pose_to_embedded_graph(dq, joint_idx, skinning_weights, node_vertex)
+
+
 def dq_skinning_gpu_grad_dofs(skinned_vertex_positions_grad, skinned_vertex_positions, global_joint_position,
                               global_joint_axis, dual_quaternions, skinning_weights, joint_idx, rest_pos,
                               influence: torch.Tensor, ref_compat: bool = True):

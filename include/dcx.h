@@ -445,6 +445,17 @@ dcx_status dcx_fk(const dcx_skeleton* skeleton, int B, const float* dofs, float*
 dcx_status dcx_fk_transforms(const dcx_skeleton* skeleton, int B, const float* dofs, float* transforms,
                              float* joint_pos, float* joint_axis, void* stream);
 
+/* Pose2EmbeddedGraphCpu (Pose2EmbeddedGraphCPUOp.cpp:129-302), on the GPU:
+ * per pose and embedded-graph node, the skinning dual quaternions [B,Js,8]
+ * of the node's base vertex (node_vertex [M]) blended with its skinning
+ * weights (joint_idx / weights [N,K], slot 0 first, signs aligned to slot 0,
+ * normalised by the rotation norm above 1e-6), returned as the op's
+ * d_skinned_translation [B,M,3] and d_skinned_rotation [B,M,3] (XYZ Euler
+ * angles of R = Rz Ry Rx, the embedded graph's convention). */
+dcx_status dcx_pose_to_graph(const float* dual_quaternions, const int32_t* joint_idx, const float* weights,
+                             const int32_t* node_vertex, int B, int N, int K, int Js, int M,
+                             float* skinned_translation, float* skinned_rotation, void* stream);
+
 /* HOST: the DOF influence table of DqSkinningGpuGrad
  * (DQSkinningGPUOpGrad.cpp:90-217): rows (vertex, joint type, joint, weight),
  * max_entries per DOF, -1 padded, from the static skinning (joint_idx /

@@ -345,6 +345,20 @@ def fk(skel, dofs, dtype=np.float64, transforms=False):
     return (dq, jp, ja, T) if transforms else (dq, jp, ja)
 
 
+def pose_to_graph(dq, joint_idx, weights, node_vertex):
+    """Pose2EmbeddedGraphCpu: (skinned translation [B,M,3], skinned rotation (XYZ Euler) [B,M,3])."""
+    dq = _c(dq, np.float64)
+    B, Js = dq.shape[0], dq.shape[1]
+    ji = _c(joint_idx, np.int32)
+    N, K = ji.shape
+    nv = _c(node_vertex, np.int32)
+    M = nv.shape[0]
+    t = np.zeros((B, M, 3))
+    a = np.zeros((B, M, 3))
+    _call("orc_pose_to_graph", dq, ji, _c(weights, np.float64), nv, B, N, K, Js, M, t, a)
+    return t, a
+
+
 def dof_influence(skel, joint_idx, weights, max_entries=None):
     """The DqSkinningGpuGrad influence table [D, E, 4] (-1 padded)."""
     ji = _c(joint_idx, np.int32)

@@ -9,7 +9,8 @@
 // 345), keep it as rotation (Eigen's rotation(): the scale-free part) plus
 // translation; also the global joint positions and axes the gradient op
 // consumes (DQSkinningGPUOp.cpp:309-345).  And the DOF
-// influence table of DqSkinningGpuGrad (DQSkinningGPUOpGrad.cpp:90-217).
+// influence table of DqSkinningGpuGrad (DQSkinningGPUOpGrad.cpp:90-217), and
+// Pose2EmbeddedGraphCpu's node transforms (Pose2EmbeddedGraphCPUOp.cpp:129-302).
 //
 // Kinematic model: joints ordered parents-first; joint j has a parent (-1 for
 // the root), a fixed offset from the parent frame, a unit axis in its own
@@ -139,8 +140,61 @@ void fk(int B, int J, int D, const int32_t* parent, const int32_t* type, const i
   }
 }
 
+// Pose2EmbeddedGraphCpu (Pose2EmbeddedGraphCPUOp.cpp:129-302): per node, the
+// skinning DQs of its base vertex blended in slot order (signs aligned to
+// slot 0, normalised by the rotation norm above 1e-6), returned as the
+// translation 2 q_d conj(q_r) and the XYZ Euler angles of the rotation.
+template <class T>
+void pose_to_graph(const T* dq, const int32_t* jidx, const T* jw, const int32_t* node_vertex, int B, int N, int K,
+                   int Js, int M, T* out_t, T* out_a) {
+  for (int b = 0; b < B; ++b)
+    for (int k = 0; k < M; ++k) {
+      const int v = node_vertex[k];
+      T r[4] = {0, 0, 0, 0}, d[4] = {0, 0, 0, 0}, first[4] = {0, 0, 0, 0};
+      for (int s = 0; s < K && v >= 0 && v < N; ++s) {
+        const int j = jidx[(size_t)v * K + s];
+        if (j < 0 || j >= Js) continue;
+        const T w = jw[(size_t)v * K + s];
+        const T* qj = dq + ((size_t)b * Js + j) * 8;
+        T sign = 1;
+        if (s == 0) {
+          for (int c = 0; c < 4; ++c) first[c] = qj[c];
+        } else if (first[0] * qj[0] + first[1] * qj[1] + first[2] * qj[2] + first[3] * qj[3] < 0) {
+          sign = -1;
+        }
+        for (int c = 0; c < 4; ++c) {
+          r[c] += qj[c] * w * sign;
+          d[c] += qj[4 + c] * w * sign;
+        }
+      }
+      const T len = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+      const T sc = len > T(0.000001) ? T(1) / len : T(1);
+      for (int c = 0; c < 4; ++c) { r[c] *= sc; d[c] *= sc; }
+      const T x = r[0], y = r[1], z = r[2], w = r[3];
+      const T R00 = 1 - 2 * y * y - 2 * z * z, R10 = 2 * x * y + 2 * z * w, R11 = 1 - 2 * x * x - 2 * z * z;
+      const T R12 = 2 * y * z - 2 * x * w, R20 = 2 * x * z - 2 * y * w, R21 = 2 * y * z + 2 * x * w;
+      const T R22 = 1 - 2 * x * x - 2 * y * y;
+      T* t = out_t + ((size_t)b * M + k) * 3;
+      t[0] = 2 * (-d[3] * x + d[0] * w - d[1] * z + d[2] * y);
+      t[1] = 2 * (-d[3] * y + d[0] * z + d[1] * w - d[2] * x);
+      t[2] = 2 * (-d[3] * z - d[0] * y + d[1] * x + d[2] * w);
+      const T sy = std::sqrt(R00 * R00 + R10 * R10);
+      T* a = out_a + ((size_t)b * M + k) * 3;
+      if (!(sy < T(1e-6))) {
+        a[0] = std::atan2(R21, R22); a[1] = std::atan2(-R20, sy); a[2] = std::atan2(R10, R00);
+      } else {
+        a[0] = std::atan2(-R12, R11); a[1] = std::atan2(-R20, sy); a[2] = 0;
+      }
+    }
+}
+
 }  // namespace
 }  // namespace orc
+
+extern "C" void orc_pose_to_graph(const double* dq, const int32_t* jidx, const double* jw, const int32_t* node_vertex,
+                                  int B, int N, int K, int Js, int M, double* out_t, double* out_a) {
+  orc::pose_to_graph<double>(dq, jidx, jw, node_vertex, B, N, K, Js, M, out_t, out_a);
+}
 
 #define ORC_FK(T, SUF)                                                                                            \
   extern "C" void orc_fk_##SUF(int B, int J, int D, const int32_t* parent, const int32_t* type, const int32_t* dof, \

@@ -83,6 +83,30 @@ def test_lbs_dof_op_matches_oracle():
     assert torch.equal(w_out, _t(w))
 
 
+def test_pose_to_graph_matches_oracle():
+    """Pose2EmbeddedGraphCpu on the GPU (FK + per-node DQ blend + Euler
+    angles) against the FP64 oracle on a capsule body's embedded graph."""
+    desc = scenes.body_skeleton()
+    sk = skinning.Skeleton(desc)
+    ch = scenes.capsule_body(3000, seed=5, ed_nodes=150)
+    rng = np.random.default_rng(11)
+    dofs = scenes.random_dofs(rng, desc, 3, 0.8, 0.05).astype(np.float32)
+    nv = ch.graph.node_vertex.astype(np.int32)
+    t, a = skinning.pose2embedded_graph_gpu(sk, _t(dofs), _t(ch.joint_idx, torch.int32), _t(ch.joint_w), _t(nv, torch.int32))
+    rdq, _, _ = orc.fk(desc, dofs.astype(np.float64))
+    rt, ra = orc.pose_to_graph(rdq, ch.joint_idx, ch.joint_w, nv)
+    assert _rel(t.cpu().numpy(), rt) < 1e-5
+    # angles: compare the rotations they encode (Euler angles jump near the singularity)
+    def R(e):
+        cx, sx, cy, sy, cz, sz = np.cos(e[..., 0]), np.sin(e[..., 0]), np.cos(e[..., 1]), np.sin(e[..., 1]), \
+            np.cos(e[..., 2]), np.sin(e[..., 2])
+        return np.stack([cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx,
+                         sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx,
+                         -sy, cy * sx, cy * cx], -1)
+    assert np.abs(R(a.cpu().numpy().astype(np.float64)) - R(ra)).max() < 1e-5
+    assert np.abs(a.cpu().numpy() - ra).max() < 1e-4  # no node of this pose sits at the singularity
+
+
 def test_fk_rejects_bad_trees():
     desc = scenes.body_skeleton()
     bad = dict(desc)

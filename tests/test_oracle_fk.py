@@ -173,3 +173,68 @@ def test_lbs_transforms_are_the_dual_quaternions_as_matrices():
     p_dqs, _ = orc.dqs_fwd(dq, w, jidx, P)
     p_lbs, _ = orc.lbs_fwd(T, w, jidx, P, np.zeros((N, 3)), np.zeros((dofs.shape[0], N, 3)))
     np.testing.assert_allclose(p_lbs, p_dqs, atol=1e-12)
+
+
+def _euler_R(a):
+    """R = Rz(a2) Ry(a1) Rx(a0), the embedded graph's convention."""
+    cx, sx, cy, sy, cz, sz = np.cos(a[0]), np.sin(a[0]), np.cos(a[1]), np.sin(a[1]), np.cos(a[2]), np.sin(a[2])
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def test_pose_to_graph_identity_and_rigid_nodes():
+    """Pose2EmbeddedGraphCpu: zero DOFs give zero translations and angles; a
+    node whose base vertex is bound to one joint takes that joint's rigid
+    motion (the Euler angles rebuild its rotation in the graph's convention);
+    flipping a joint DQ's sign changes nothing."""
+    sk = scenes.body_skeleton()
+    Js = len(sk["skin_joint"])
+    M = 3 * Js
+    node_vertex = np.arange(M, dtype=np.int32)
+    jidx = np.stack([np.arange(M) % Js] + [(np.arange(M) + s) % Js for s in (1, 2, 3)], 1).astype(np.int32)
+    w = np.zeros((M, 4))
+    w[:, 0] = 1.0
+    dq0, _, _ = orc.fk(sk, np.zeros((1, sk["num_dofs"])))
+    t0, a0 = orc.pose_to_graph(dq0, jidx, w, node_vertex)
+    np.testing.assert_allclose(t0, 0, atol=1e-12)
+    np.testing.assert_allclose(a0, 0, atol=1e-12)
+    dofs = scenes.random_dofs(np.random.default_rng(5), sk, 2, 0.9, 0.1)
+    dq, _, _ = orc.fk(sk, dofs)
+    t, a = orc.pose_to_graph(dq, jidx, w, node_vertex)
+    for b in range(2):
+        for k in range(M):
+            q = dq[b, jidx[k, 0]]
+            np.testing.assert_allclose(_euler_R(a[b, k]), scenes.quat_to_mat(q[None, :4])[0], atol=1e-10)
+            np.testing.assert_allclose(t[b, k], dq_apply(q, np.zeros(3)), atol=1e-10)
+    flipped = dq.copy()
+    flipped[:, ::2] *= -1
+    tf, af = orc.pose_to_graph(flipped, jidx, w, node_vertex)
+    np.testing.assert_allclose(tf, t, atol=1e-12)
+    np.testing.assert_allclose(af, a, atol=1e-12)
+
+
+def test_pose_to_graph_blend_and_singular_branch():
+    """Two joints at 0.5 / 0.5 with the second given as -q: the blend aligns
+    the sign and returns the normalised mean rotation; a rotation of +-90 deg
+    about y (the Euler singularity) is still rebuilt exactly."""
+    def dq_of(R, t):
+        q = np.zeros(4)
+        from scipy.spatial.transform import Rotation
+        q[:] = Rotation.from_matrix(R).as_quat()  # x, y, z, w
+        return scenes.dq_from_rt(q[None], np.asarray(t, float)[None])[0]
+    qa = dq_of(rot([0, 0, 1], 0.4), [0.1, 0, 0])
+    qb = dq_of(rot([0, 0, 1], 0.8), [0.1, 0, 0])
+    dq = np.stack([qa, -qb])[None]
+    jidx = np.array([[0, 1, 0, 0]], np.int32)
+    w = np.array([[0.5, 0.5, 0.0, 0.0]])
+    t, a = orc.pose_to_graph(dq, jidx, w, np.array([0], np.int32))
+    np.testing.assert_allclose(_euler_R(a[0, 0]), rot([0, 0, 1], 0.6), atol=1e-12)
+    for ang in (np.pi / 2, -np.pi / 2):
+        R = rot([0, 1, 0], ang) @ rot([1, 0, 0], 0.3)
+        d1 = dq_of(R, [0, 0.2, 0])[None, None]
+        t1, a1 = orc.pose_to_graph(d1, np.array([[0, 0, 0, 0]], np.int32), np.array([[1.0, 0, 0, 0]]),
+                                   np.array([0], np.int32))
+        np.testing.assert_allclose(_euler_R(a1[0, 0]), R, atol=1e-6)
+        np.testing.assert_allclose(t1[0, 0], [0, 0.2, 0], atol=1e-12)
