@@ -44,7 +44,8 @@ SIGNATURES: dict[str, str] = {
     "dcx_pipeline_profile": "pppiS", "dcx_pipeline_set_tile": "pi", "dcx_projected_mesh_boundary": "piiiippppppzS", "dcx_pipeline_autotune": "ppiSppp",
     "dcx_skeleton_create": "iipppppipppp",
     "dcx_skeleton_destroy": "p",
-    "dcx_fk": "pippppS", "dcx_fk_transforms": "pippppS", "dcx_pose_to_graph": "ppppiiiiippS",
+    "dcx_fk": "pippppS", "dcx_fk_transforms": "pippppS", "dcx_pose_to_graph": "ppppiiiiippS", "dcx_arap_fwd": "ppppppiiipppppS",
+    "dcx_arap_bwd": "pppppiiippppS",
     "dcx_dof_influence": "piippipp",
     "dcx_topo_texel_table": "piiS",
     "dcx_render_normal_map": "ppiipiippppppzS",

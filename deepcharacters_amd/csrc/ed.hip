@@ -208,6 +208,75 @@ void free_graph(dcx_edgraph* g) {
   delete g;
 }
 
+// As-rigid-as-possible residuals of the node graph (EmbeddedGraphArapGPUOp.cu:
+// 10-105): one lane per (pose, node j, connection slot c).  For c below the
+// node's neighbour count, with k its c-th neighbour,
+//   r_jc = R_j (g_k - g_j) + g_j + t_j - (g_k + t_k),  w_jc = (rig_j + rig_k) / 2,
+// and zeros past the neighbour count (neighbours past max_conn are dropped).
+__global__ __launch_bounds__(kB) void arap_fwd_kernel(const float* __restrict__ base, const int* __restrict__ node_vertex,
+                                                      const int* __restrict__ nn_off, const int* __restrict__ nn_size,
+                                                      const int* __restrict__ nn_idx, const float* __restrict__ rig,
+                                                      int B, int M, int C, const float* __restrict__ dT,
+                                                      const float* __restrict__ dA, float* __restrict__ res,
+                                                      float* __restrict__ cw) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)B * M * C) return;
+  const int c = (int)(i % C);
+  const long long bj = i / C;
+  const int j = (int)(bj % M), b = (int)(bj / M);
+  f3 r = mk3(0.f, 0.f, 0.f);
+  float w = 0.f;
+  if (c < nn_size[j]) {
+    const int k = nn_idx[nn_off[j] + c];
+    const m3 R = euler_mat(euler_sc(ld3(dA + ((size_t)b * M + j) * 3)));
+    const f3 gj = ld3(base + 3 * (size_t)node_vertex[j]), gk = ld3(base + 3 * (size_t)node_vertex[k]);
+    const f3 tj = ld3(dT + ((size_t)b * M + j) * 3), tk = ld3(dT + ((size_t)b * M + k) * 3);
+    r = mul(R, gk - gj) + gj + tj - (gk + tk);
+    w = (rig[j] + rig[k]) / 2.f;
+  }
+  res[3 * i] = r.x; res[3 * i + 1] = r.y; res[3 * i + 2] = r.z;
+  cw[i] = w;
+}
+
+// Its gradient (EmbeddedGraphArapGPUOpGrad.cu:11-100), one lane per (pose,
+// node i): dL/dA_i = sum over i's slots c of (dR/dA (g_k - g_i))^T G_ic and
+// dL/dt_i = sum over i's neighbours k of (G_ik - G_ki), G_ki read from the
+// slot of k's list that holds i (the last such slot; zero if k does not list
+// i) -- the exact adjoint for a symmetric node graph.
+__global__ __launch_bounds__(kB) void arap_bwd_kernel(const float* __restrict__ base, const int* __restrict__ node_vertex,
+                                                      const int* __restrict__ nn_off, const int* __restrict__ nn_size,
+                                                      const int* __restrict__ nn_idx, int B, int M, int C,
+                                                      const float* __restrict__ g_res, const float* __restrict__ dA,
+                                                      float* __restrict__ g_t, float* __restrict__ g_a) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)B * M) return;
+  const int n = (int)(i % M), b = (int)(i / M);
+  const EulerSC sc = euler_sc(ld3(dA + 3 * (size_t)i));
+  const m3 Ra = euler_dA(sc), Rb = euler_dB(sc), Rg = euler_dG(sc);
+  const f3 gi = ld3(base + 3 * (size_t)node_vertex[n]);
+  const float* G = g_res + (size_t)b * M * C * 3;
+  f3 gt = mk3(0.f, 0.f, 0.f), ga = mk3(0.f, 0.f, 0.f);
+  const int cn = min(nn_size[n], C);  // slots past max_conn hold no residual
+  for (int c = 0; c < cn; ++c) {
+    const int k = nn_idx[nn_off[n] + c];
+    const f3 Gik = ld3(G + ((size_t)n * C + c) * 3);
+    const f3 d = ld3(base + 3 * (size_t)node_vertex[k]) - gi;
+    ga = ga + mk3(dot(mul(Ra, d), Gik), dot(mul(Rb, d), Gik), dot(mul(Rg, d), Gik));
+    f3 Gki = mk3(0.f, 0.f, 0.f);
+    const int ck = min(nn_size[k], C);
+    for (int c2 = 0; c2 < ck; ++c2)
+      if (nn_idx[nn_off[k] + c2] == n) Gki = ld3(G + ((size_t)k * C + c2) * 3);
+    gt = gt + (Gik - Gki);
+  }
+  g_t[3 * i] = gt.x; g_t[3 * i + 1] = gt.y; g_t[3 * i + 2] = gt.z;
+  g_a[3 * i] = ga.x; g_a[3 * i + 1] = ga.y; g_a[3 * i + 2] = ga.z;
+}
+
+__global__ __launch_bounds__(kB) void copy_f_kernel(const float* __restrict__ src, float* __restrict__ dst, long long n) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 }  // namespace dcx
 
@@ -322,4 +391,36 @@ extern "C" dcx_status dcx_ed_bwd(const dcx_edgraph* g, int B, const float* g_ver
   hipLaunchKernelGGL(ed_node_bwd_kernel, dim3(ceil_div((long long)B * g->M * 64, kB)), dim3(kB), 0, s, *g, B, table,
                      g_displacements, g->Mk ? g_markers : nullptr, g_delta_t, g_delta_a, g_skinned_t, g_skinned_a);
   return check_launch("dcx_ed_bwd");
+}
+
+extern "C" dcx_status dcx_arap_fwd(const float* base_pos, const int32_t* node_vertex, const int32_t* nn_offsets,
+                                   const int32_t* nn_sizes, const int32_t* nn_indices, const float* rigidity, int B,
+                                   int M, int max_conn, const float* delta_t, const float* delta_a, float* residual,
+                                   float* connection_weights, float* rotation_out, void* stream) {
+  DCX_REQUIRE(base_pos && node_vertex && nn_offsets && nn_sizes && nn_indices && rigidity && delta_t && delta_a &&
+                  residual && connection_weights,
+              "dcx_arap_fwd: null argument");
+  DCX_REQUIRE(B > 0 && M > 0 && max_conn > 0, "dcx_arap_fwd: bad sizes");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(arap_fwd_kernel, dim3(ceil_div((long long)B * M * max_conn, kB)), dim3(kB), 0, s, base_pos,
+                     node_vertex, nn_offsets, nn_sizes, nn_indices, rigidity, B, M, max_conn, delta_t, delta_a,
+                     residual, connection_weights);
+  if (rotation_out)
+    hipLaunchKernelGGL(copy_f_kernel, dim3(ceil_div(3ll * B * M, kB)), dim3(kB), 0, s, delta_a, rotation_out,
+                       3ll * B * M);
+  return check_launch("dcx_arap_fwd");
+}
+
+extern "C" dcx_status dcx_arap_bwd(const float* base_pos, const int32_t* node_vertex, const int32_t* nn_offsets,
+                                   const int32_t* nn_sizes, const int32_t* nn_indices, int B, int M, int max_conn,
+                                   const float* residual_grad, const float* delta_a, float* delta_t_grad,
+                                   float* delta_a_grad, void* stream) {
+  DCX_REQUIRE(base_pos && node_vertex && nn_offsets && nn_sizes && nn_indices && residual_grad && delta_a &&
+                  delta_t_grad && delta_a_grad,
+              "dcx_arap_bwd: null argument");
+  DCX_REQUIRE(B > 0 && M > 0 && max_conn > 0, "dcx_arap_bwd: bad sizes");
+  hipLaunchKernelGGL(arap_bwd_kernel, dim3(ceil_div((long long)B * M, kB)), dim3(kB), 0, as_stream(stream), base_pos,
+                     node_vertex, nn_offsets, nn_sizes, nn_indices, B, M, max_conn, residual_grad, delta_a,
+                     delta_t_grad, delta_a_grad);
+  return check_launch("dcx_arap_bwd");
 }

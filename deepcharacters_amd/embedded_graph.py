@@ -134,3 +134,75 @@ class EmbeddedGraphGpu:
     def getNode(self):
         o = self.embeddedGraphGpuOperator
         return [o[0], o[1], o[2], o[3]]
+
+
+class ArapGraph:
+    """Node graph of EmbeddedGraphArapGpu on the device: base positions [N,3],
+    node -> base vertex [M], node -> node adjacency (offsets, sizes, indices;
+    e.g. scenes.node_neighbours) and per-node rigidity weights."""
+
+    def __init__(self, base_pos, node_vertex, nn_offsets, nn_sizes, nn_indices, rigidity, max_conn=None,
+                 device="cuda"):
+        t = lambda a, dt=torch.float32: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+        self.base = t(base_pos)
+        self.node_vertex = t(node_vertex, torch.int32)
+        self.off = t(nn_offsets, torch.int32)
+        self.size = t(nn_sizes, torch.int32)
+        self.idx = t(nn_indices, torch.int32)
+        self.rig = t(rigidity)
+        self.M = int(self.node_vertex.shape[0])
+        sizes = np.asarray(nn_sizes)
+        self.max_conn = int(sizes.max()) if max_conn is None else int(max_conn)
+        if self.max_conn <= 0:
+            raise ValueError("max_number_of_node_connections must be positive")
+        if (np.asarray(nn_indices) < 0).any() or (np.asarray(nn_indices) >= self.M).any():
+            raise ValueError("node neighbour index out of range")
+        if (np.asarray(node_vertex) < 0).any() or (np.asarray(node_vertex) >= self.base.shape[0]).any():
+            raise ValueError("node vertex index out of range")
+
+
+def embedded_graph_arap_gpu(g: ArapGraph, nodes_translation, nodes_rotation):
+    """EmbeddedGraphArapGpu: (nodes_arap_loss [B,M,C,3], d_connection_weights
+    [B,M,C], d_rotation [B,M,3])."""
+    dT = dev(nodes_translation, "nodes_translation")
+    B = dT.shape[0]
+    dA = dev(nodes_rotation, "nodes_rotation", shape=(B, g.M, 3))
+    dT = dev(dT, "nodes_translation", shape=(B, g.M, 3))
+    res = torch.empty((B, g.M, g.max_conn, 3), device=dT.device)
+    cw = torch.empty((B, g.M, g.max_conn), device=dT.device)
+    rot = torch.empty_like(dA)
+    _lib.call("dcx_arap_fwd", ptr(g.base), ptr(g.node_vertex), ptr(g.off), ptr(g.size), ptr(g.idx), ptr(g.rig), B,
+              g.M, g.max_conn, ptr(dT), ptr(dA), ptr(res), ptr(cw), ptr(rot), stream_ptr())
+    return res, cw, rot
+
+
+def embedded_graph_arap_gpu_grad(g: ArapGraph, nodes_arap_loss_grad, d_a):
+    """EmbeddedGraphArapGpuGrad: (nodes_t_grad [B,M,3], nodes_r_grad [B,M,3])."""
+    G = dev(nodes_arap_loss_grad, "nodes_arap_loss")
+    B = G.shape[0]
+    G = dev(G, "nodes_arap_loss", shape=(B, g.M, g.max_conn, 3))
+    dA = dev(d_a, "d_a", shape=(B, g.M, 3))
+    gT = torch.empty_like(dA)
+    gA = torch.empty_like(dA)
+    _lib.call("dcx_arap_bwd", ptr(g.base), ptr(g.node_vertex), ptr(g.off), ptr(g.size), ptr(g.idx), B, g.M,
+              g.max_conn, ptr(G), ptr(dA), ptr(gT), ptr(gA), stream_ptr())
+    return gT, gA
+
+
+class ArapFn(torch.autograd.Function):
+    """nodes_arap_loss as a differentiable function of (translation, rotation),
+    paired with EmbeddedGraphArapGpuGrad."""
+
+    @staticmethod
+    def forward(ctx, nodes_translation, nodes_rotation, g):
+        res, cw, rot = embedded_graph_arap_gpu(g, nodes_translation, nodes_rotation)
+        ctx.save_for_backward(rot)
+        ctx.g = g
+        ctx.mark_non_differentiable(cw)
+        return res, cw
+
+    @staticmethod
+    def backward(ctx, g_res, g_cw):
+        (rot,) = ctx.saved_tensors
+        gT, gA = embedded_graph_arap_gpu_grad(ctx.g, g_res.contiguous(), rot)
+        return gT, gA, None

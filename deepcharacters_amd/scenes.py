@@ -257,6 +257,25 @@ def embedded_graph(verts: np.ndarray, M: int, K: int = 4, sigma: float = 0.05) -
 # Cameras
 # ----------------------------------------------------------------------------
 
+def node_neighbours(verts: np.ndarray, node_vertex: np.ndarray, k: int = 6):
+    """Symmetric node -> node adjacency of an embedded graph: each node's k
+    nearest nodes (by base position), symmetrised.  Returns (offsets [M],
+    sizes [M], indices [E]) int32, neighbours in increasing id order."""
+    from scipy.spatial import cKDTree
+    P = np.asarray(verts, np.float64)[node_vertex]
+    M = len(node_vertex)
+    _, nn = cKDTree(P).query(P, k=min(k + 1, M))
+    adj = [set() for _ in range(M)]
+    for i in range(M):
+        for j in np.atleast_1d(nn[i])[1:]:
+            adj[i].add(int(j))
+            adj[int(j)].add(i)
+    sizes = np.array([len(a) for a in adj], np.int32)
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+    indices = np.concatenate([sorted(a) for a in adj]).astype(np.int32)
+    return offsets, sizes, indices
+
+
 def look_at_extrinsic(centre: np.ndarray, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)) -> np.ndarray:
     """3x4 world->camera [R|t]; camera x right, y down, z forward."""
     c = np.asarray(centre, float)

@@ -305,6 +305,32 @@ dcx_status dcx_ed_bwd(const dcx_edgraph* graph, int B, const float* g_vertices, 
                       float* g_skinned_t, float* g_skinned_a, float* g_displacements, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/* EmbeddedGraphArapGpu (EmbeddedGraphArapGPUOp.cpp:6-16 ->
+ * EmbeddedGraphArapGPUOp.cu:10-105): as-rigid-as-possible residuals of the
+ * node graph.  For node j and connection slot c < max_conn, with k the c-th
+ * entry of j's neighbour list (nn_indices[nn_offsets[j] + c], c < nn_sizes[j]):
+ *   residual[b,j,c] = R_j (g_k - g_j) + g_j + t_j - (g_k + t_k)
+ *   connection_weights[b,j,c] = (rigidity[j] + rigidity[k]) / 2
+ * (zeros for c >= nn_sizes[j]); R_j = Rz Ry Rx of delta_a[b,j], g = base_pos
+ * at the node's base vertex.  rotation_out (may be NULL) receives a copy of
+ * delta_a (the op's d_rotation output).  The node graph comes as arrays
+ * instead of the character / graph files.
+ *   base_pos [N,3], node_vertex [M], nn_offsets/nn_sizes [M], rigidity [M]
+ *   delta_t, delta_a [B,M,3]; residual [B,M,max_conn,3]; connection_weights [B,M,max_conn] */
+dcx_status dcx_arap_fwd(const float* base_pos, const int32_t* node_vertex, const int32_t* nn_offsets,
+                        const int32_t* nn_sizes, const int32_t* nn_indices, const float* rigidity, int B, int M,
+                        int max_conn, const float* delta_t, const float* delta_a, float* residual,
+                        float* connection_weights, float* rotation_out, void* stream);
+/* EmbeddedGraphArapGpuGrad (EmbeddedGraphArapGPUOpGrad.cu:11-100): from
+ * dL/d(residual) [B,M,max_conn,3] and delta_a, dL/d(delta_t) and dL/d(delta_a)
+ * [B,M,3].  The translation gradient of node i takes its own slots' terms and,
+ * for each neighbour k, the term of k's slot that lists i (the reference's
+ * formula: the exact adjoint when the node graph is symmetric). */
+dcx_status dcx_arap_bwd(const float* base_pos, const int32_t* node_vertex, const int32_t* nn_offsets,
+                        const int32_t* nn_sizes, const int32_t* nn_indices, int B, int M, int max_conn,
+                        const float* residual_grad, const float* delta_a, float* delta_t_grad,
+                        float* delta_a_grad, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Silhouette loss and its feeders                                          */
 /* ------------------------------------------------------------------------ */

@@ -345,6 +345,29 @@ def fk(skel, dofs, dtype=np.float64, transforms=False):
     return (dq, jp, ja, T) if transforms else (dq, jp, ja)
 
 
+def arap_fwd(base, node_vertex, nn_offsets, nn_sizes, nn_indices, rigidity, dT, dA, max_conn):
+    """EmbeddedGraphArapGpu: (residuals [B,M,C,3], connection weights [B,M,C])."""
+    dT = _c(dT, np.float64)
+    B, M = dT.shape[0], dT.shape[1]
+    res = np.zeros((B, M, max_conn, 3))
+    cw = np.zeros((B, M, max_conn))
+    _call("orc_arap_fwd", _c(base, np.float64), _c(node_vertex, np.int32), _c(nn_offsets, np.int32),
+          _c(nn_sizes, np.int32), _c(nn_indices, np.int32), _c(rigidity, np.float64), B, M, max_conn, dT,
+          _c(dA, np.float64), res, cw)
+    return res, cw
+
+
+def arap_bwd(base, node_vertex, nn_offsets, nn_sizes, nn_indices, g_res, dA):
+    """EmbeddedGraphArapGpuGrad (exact adjoint): (dL/dT [B,M,3], dL/dA [B,M,3])."""
+    g_res = _c(g_res, np.float64)
+    B, M, C = g_res.shape[0], g_res.shape[1], g_res.shape[2]
+    gT = np.zeros((B, M, 3))
+    gA = np.zeros((B, M, 3))
+    _call("orc_arap_bwd", _c(base, np.float64), _c(node_vertex, np.int32), _c(nn_offsets, np.int32),
+          _c(nn_sizes, np.int32), _c(nn_indices, np.int32), B, M, C, g_res, _c(dA, np.float64), gT, gA)
+    return gT, gA
+
+
 def pose_to_graph(dq, joint_idx, weights, node_vertex):
     """Pose2EmbeddedGraphCpu: (skinned translation [B,M,3], skinned rotation (XYZ Euler) [B,M,3])."""
     dq = _c(dq, np.float64)

@@ -7,6 +7,11 @@
 //   ed_bwd  EmbeddedGraphGPUOpGrad.cu:9-201: node translation / rotation
 //           gradients (node-centric sums), the marker term, displacement
 //           gradient; skinned-node gradients are zero as in the reference
+//   arap    EmbeddedGraphArapGPUOp.cu:10-105 / ...Grad.cu:11-100: node-graph
+//           as-rigid-as-possible residuals and weights; the gradient is the
+//           exact adjoint (each residual's +t_j / -t_k / rotation terms),
+//           which is the reference's neighbour-list formula on a symmetric
+//           graph
 //
 // Pinning (tests/test_oracle_ed.py): identity at zero parameters, pure
 // translation, single-node rigid rotation about the node, the Euler
@@ -206,7 +211,74 @@ void ed_bwd(const EdGraph& g, const T* base, const T* base_markers, const T* gV,
   }
 }
 
+// As-rigid-as-possible residuals of the node graph and their gradient
+// (EmbeddedGraphArapGPUOp.cu:10-105, EmbeddedGraphArapGPUOpGrad.cu:11-100):
+// r_jc = R_j (g_k - g_j) + g_j + t_j - (g_k + t_k) for the c-th neighbour k of
+// node j (c < max_conn), weight (rig_j + rig_k) / 2; the backward takes dL/dr
+// and returns dL/dt (each residual's +t_j and -t_k terms) and dL/dA (the
+// rotation derivative applied to g_k - g_j).
+template <class T>
+void arap_fwd(const T* base, const int32_t* node_vertex, const int32_t* off, const int32_t* size, const int32_t* idx,
+              const T* rig, int B, int M, int C, const T* dT, const T* dA, T* res, T* cw) {
+  std::memset(res, 0, sizeof(T) * (size_t)B * M * C * 3);
+  std::memset(cw, 0, sizeof(T) * (size_t)B * M * C);
+  for (int b = 0; b < B; ++b)
+    for (int j = 0; j < M; ++j) {
+      T R[9];
+      euler(dA + ((size_t)b * M + j) * 3, R);
+      const T* gj = base + 3 * (size_t)node_vertex[j];
+      const T* tj = dT + ((size_t)b * M + j) * 3;
+      for (int c = 0; c < size[j] && c < C; ++c) {
+        const int k = idx[off[j] + c];
+        const T* gk = base + 3 * (size_t)node_vertex[k];
+        const T* tk = dT + ((size_t)b * M + k) * 3;
+        const T d[3] = {gk[0] - gj[0], gk[1] - gj[1], gk[2] - gj[2]};
+        T* r = res + (((size_t)b * M + j) * C + c) * 3;
+        for (int a = 0; a < 3; ++a)
+          r[a] = R[3 * a] * d[0] + R[3 * a + 1] * d[1] + R[3 * a + 2] * d[2] + gj[a] + tj[a] - (gk[a] + tk[a]);
+        cw[((size_t)b * M + j) * C + c] = (rig[j] + rig[k]) / T(2);
+      }
+    }
+}
+
+template <class T>
+void arap_bwd(const T* base, const int32_t* node_vertex, const int32_t* off, const int32_t* size, const int32_t* idx,
+              int B, int M, int C, const T* gres, const T* dA, T* gT, T* gA) {
+  std::memset(gT, 0, sizeof(T) * (size_t)B * M * 3);
+  std::memset(gA, 0, sizeof(T) * (size_t)B * M * 3);
+  for (int b = 0; b < B; ++b)
+    for (int j = 0; j < M; ++j) {
+      T Rk[3][9];
+      for (int k = 0; k < 3; ++k) euler_d(dA + ((size_t)b * M + j) * 3, k, Rk[k]);
+      const T* gj = base + 3 * (size_t)node_vertex[j];
+      for (int c = 0; c < size[j] && c < C; ++c) {
+        const int k = idx[off[j] + c];
+        const T* gk = base + 3 * (size_t)node_vertex[k];
+        const T d[3] = {gk[0] - gj[0], gk[1] - gj[1], gk[2] - gj[2]};
+        const T* G = gres + (((size_t)b * M + j) * C + c) * 3;
+        for (int q = 0; q < 3; ++q)
+          for (int a = 0; a < 3; ++a)
+            gA[((size_t)b * M + j) * 3 + q] += (Rk[q][3 * a] * d[0] + Rk[q][3 * a + 1] * d[1] + Rk[q][3 * a + 2] * d[2]) * G[a];
+        for (int a = 0; a < 3; ++a) {
+          gT[((size_t)b * M + j) * 3 + a] += G[a];   // +t_j
+          gT[((size_t)b * M + k) * 3 + a] -= G[a];   // -t_k
+        }
+      }
+    }
+}
+
 }  // namespace orc
+
+extern "C" void orc_arap_fwd(const double* base, const int32_t* node_vertex, const int32_t* off, const int32_t* size,
+                             const int32_t* idx, const double* rig, int B, int M, int C, const double* dT,
+                             const double* dA, double* res, double* cw) {
+  orc::arap_fwd<double>(base, node_vertex, off, size, idx, rig, B, M, C, dT, dA, res, cw);
+}
+extern "C" void orc_arap_bwd(const double* base, const int32_t* node_vertex, const int32_t* off, const int32_t* size,
+                             const int32_t* idx, int B, int M, int C, const double* gres, const double* dA, double* gT,
+                             double* gA) {
+  orc::arap_bwd<double>(base, node_vertex, off, size, idx, B, M, C, gres, dA, gT, gA);
+}
 
 #define ORC_ED_EXPORT(T, SUF)                                                                                 \
   extern "C" void orc_ed_fwd_##SUF(const int* dims, const int32_t* node_vertex, const int32_t* off,           \

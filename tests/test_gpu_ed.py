@@ -113,3 +113,58 @@ def test_ed_edge_cases():
     ref = orc.ed_bwd(g, base, gv, dA, sA)
     assert _rel(out[0].cpu(), ref[0]) < 1e-5 and _rel(out[1].cpu(), ref[1]) < 1e-5
     assert float(out[0][:, M - 1].abs().max()) == 0.0
+
+
+def _arap_graph(seed, M=120, k=6):
+    ch = scenes.capsule_body(3000, seed=seed, ed_nodes=M)
+    nv = ch.graph.node_vertex.astype(np.int32)
+    off, size, idx = scenes.node_neighbours(ch.verts, nv, k=k)
+    rig = np.random.default_rng(seed).uniform(0.5, 2.0, len(nv))
+    return ch.verts, nv, off, size, idx, rig
+
+
+def test_arap_forward_backward_and_autograd_match_oracle():
+    base, nv, off, size, idx, rig = _arap_graph(3)
+    g = edm.ArapGraph(base, nv, off, size, idx, rig)
+    M, C = len(nv), g.max_conn
+    rng = np.random.default_rng(5)
+    dT = rng.normal(0, 0.03, (2, M, 3)).astype(np.float32)
+    dA = rng.normal(0, 0.3, (2, M, 3)).astype(np.float32)
+    res, cw, rot = edm.embedded_graph_arap_gpu(g, torch.as_tensor(dT, device="cuda"),
+                                                          torch.as_tensor(dA, device="cuda"))
+    rres, rcw = orc.arap_fwd(base, nv, off, size, idx, rig, dT, dA, C)
+    assert float(np.abs(res.cpu().numpy() - rres).max()) < 1e-6
+    assert np.allclose(cw.cpu().numpy(), rcw, rtol=1e-6, atol=0)
+    assert torch.equal(rot.cpu(), torch.as_tensor(dA))
+    G = rng.normal(size=(2, M, C, 3)).astype(np.float32)
+    for j in range(M):
+        G[:, j, size[j]:] = 0.0
+    gT, gA = edm.embedded_graph_arap_gpu_grad(g, torch.as_tensor(G, device="cuda"), rot)
+    rT, rA = orc.arap_bwd(base, nv, off, size, idx, G, dA)
+    rel = lambda a, b: float(np.abs(a.cpu().numpy() - b).max() / np.abs(b).max())
+    assert rel(gT, rT) < 1e-5 and rel(gA, rA) < 1e-5
+    # autograd: L = sum(w * |r|^2) through ArapFn
+    t = torch.as_tensor(dT, device="cuda").requires_grad_(True)
+    a = torch.as_tensor(dA, device="cuda").requires_grad_(True)
+    r, w = edm.ArapFn.apply(t, a, g)
+    (w[..., None] * r * r).sum().backward()
+    Gw = (2 * rcw[..., None] * rres)
+    rT2, rA2 = orc.arap_bwd(base, nv, off, size, idx, Gw, dA)
+    assert float(np.abs(t.grad.cpu().numpy() - rT2).max() / np.abs(rT2).max()) < 1e-5
+    assert float(np.abs(a.grad.cpu().numpy() - rA2).max() / np.abs(rA2).max()) < 1e-5
+
+
+def test_arap_truncates_neighbour_lists_at_max_connections():
+    base, nv, off, size, idx, rig = _arap_graph(4, M=80)
+    C = int(size.max()) - 2
+    g = edm.ArapGraph(base, nv, off, size, idx, rig, max_conn=C)
+    rng = np.random.default_rng(6)
+    dT = rng.normal(0, 0.03, (1, len(nv), 3)).astype(np.float32)
+    dA = rng.normal(0, 0.3, (1, len(nv), 3)).astype(np.float32)
+    res, cw, rot = edm.embedded_graph_arap_gpu(g, torch.as_tensor(dT, device="cuda"),
+                                                          torch.as_tensor(dA, device="cuda"))
+    rres, rcw = orc.arap_fwd(base, nv, off, size, idx, rig, dT, dA, C)
+    assert float(np.abs(res.cpu().numpy() - rres).max()) < 1e-6
+    G = rng.normal(size=(1, len(nv), C, 3)).astype(np.float32)
+    gT, gA = edm.embedded_graph_arap_gpu_grad(g, torch.as_tensor(G, device="cuda"), rot)
+    assert torch.isfinite(gT).all() and torch.isfinite(gA).all()

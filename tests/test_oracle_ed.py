@@ -127,3 +127,68 @@ def test_backward_matches_finite_differences():
             flat[i] = old
             fd = (lp - lm) / (2 * h)
             assert abs(fd - grad.reshape(-1)[i]) < 1e-6 * max(1, abs(fd)), (i, fd, grad.reshape(-1)[i])
+
+
+# --- node-graph ARAP (EmbeddedGraphArapGpu / ...Grad) ----------------------
+
+def _arap_scene(M=40, seed=0):
+    ch = scenes.ellipsoid_character(2, seed=seed)
+    rng = np.random.default_rng(seed)
+    nv = rng.choice(ch.N, M, replace=False).astype(np.int32)
+    off, size, idx = scenes.node_neighbours(ch.verts, nv, k=5)
+    rig = rng.uniform(0.5, 2.0, M)
+    return ch.verts.astype(np.float64), nv, off, size, idx, rig, int(size.max())
+
+
+def test_arap_zero_and_translation_closed_forms():
+    base, nv, off, size, idx, rig, C = _arap_scene()
+    M = len(nv)
+    z = np.zeros((2, M, 3))
+    res, cw = orc.arap_fwd(base, nv, off, size, idx, rig, z, z, C)
+    np.testing.assert_allclose(res, 0, atol=1e-15)
+    for j in range(M):
+        for c in range(C):
+            want = (rig[j] + rig[idx[off[j] + c]]) / 2 if c < size[j] else 0.0
+            assert cw[0, j, c] == want
+    t = np.random.default_rng(1).normal(size=(2, M, 3))
+    res, _ = orc.arap_fwd(base, nv, off, size, idx, rig, t, z, C)
+    for j in range(M):
+        for c in range(size[j]):
+            np.testing.assert_allclose(res[:, j, c], t[:, j] - t[:, idx[off[j] + c]], atol=1e-14)
+
+
+def test_arap_rigid_motion_has_zero_residual():
+    """Every node rotated by the same R and translated so that node positions
+    follow x -> R x + T: the graph moves rigidly, all residuals vanish."""
+    base, nv, off, size, idx, rig, C = _arap_scene(seed=2)
+    M = len(nv)
+    ang = np.array([0.3, -0.7, 1.1])
+    ca, sa, cb, sb, cg, sg = np.cos(ang[0]), np.sin(ang[0]), np.cos(ang[1]), np.sin(ang[1]), np.cos(ang[2]), np.sin(ang[2])
+    R = (np.array([[cg, -sg, 0], [sg, cg, 0], [0, 0, 1]]) @ np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+         @ np.array([[1, 0, 0], [0, ca, -sa], [0, sa, ca]]))
+    T = np.array([0.2, -0.1, 0.4])
+    g = base[nv]
+    dT = (g @ R.T + T - g)[None]
+    dA = np.broadcast_to(ang, (1, M, 3)).copy()
+    res, _ = orc.arap_fwd(base, nv, off, size, idx, rig, dT, dA, C)
+    np.testing.assert_allclose(res, 0, atol=1e-13)
+
+
+def test_arap_backward_matches_finite_differences():
+    base, nv, off, size, idx, rig, C = _arap_scene(M=25, seed=3)
+    M = len(nv)
+    rng = np.random.default_rng(4)
+    dT = rng.normal(0, 0.05, (2, M, 3))
+    dA = rng.normal(0, 0.4, (2, M, 3))
+    G = rng.normal(size=(2, M, C, 3))
+    for j in range(M):
+        G[:, j, size[j]:] = 0.0
+    gT, gA = orc.arap_bwd(base, nv, off, size, idx, G, dA)
+    L = lambda t, a: np.sum(G * orc.arap_fwd(base, nv, off, size, idx, rig, t, a, C)[0])
+    h = 1e-6
+    for arr, grad in ((dT, gT), (dA, gA)):
+        for i in rng.choice(arr.size, 30, replace=False):
+            p = arr.copy(); p.reshape(-1)[i] += h
+            m = arr.copy(); m.reshape(-1)[i] -= h
+            fd = ((L(p, dA) - L(m, dA)) if arr is dT else (L(dT, p) - L(dT, m))) / (2 * h)
+            assert abs(fd - grad.reshape(-1)[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad.reshape(-1)[i])
