@@ -1500,11 +1500,18 @@ struct dcx_pipeline {
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // second stream (default priority) and the fork / join events that let the
+  // per-face record sums run beside the silhouette stages
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 static void pipe_free(dcx_pipeline* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   for (void* a : p->allocs) (void)hipFree(a);
   delete p;
 }
@@ -1707,6 +1714,13 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     pipe_free(p);
     return set_error(DCX_ERR_OOM, "dcx_pipeline_create: %s", hipGetErrorString(e));
   }
+  e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    pipe_free(p);
+    return set_error(DCX_ERR_HIP, "dcx_pipeline_create: %s", hipGetErrorString(e));
+  }
   *out = p;
   return DCX_OK;
 }
@@ -1782,6 +1796,17 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     else hipLaunchKernelGGL(fine_kernel<5>, grid, dim3(kTilePix), 0, s, d, io);
   }
   mark();
+  // Eager and captured steps run the per-face record sums (which need only
+  // the fine kernel's records) on the side stream beside the silhouette
+  // stages, and the joint-DQ reduction beside the node gradients; a profiled
+  // step keeps every stage on one stream so that its events bracket it.
+  const bool fork = ev == nullptr;
+  if (fork) {
+    DCX_HIP(hipEventRecord(p->ev_fork, s));
+    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, p->side, d);
+    DCX_HIP(hipEventRecord(p->ev_join, p->side));
+  }
   hipLaunchKernelGGL(boundary_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(sil_point_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
@@ -1795,17 +1820,30 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
+  if (fork) {
+    DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+  } else {
+    hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
+  }
   mark();
   hipLaunchKernelGGL(vertex_bwd_kernel, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, s, d, p->nChunk);
+  hipStream_t sj = s;
+  if (fork) {
+    DCX_HIP(hipEventRecord(p->ev_fork, s));
+    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    sj = p->side;
+  }
+  hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, sj, d,
+                     p->nChunk);
   mark();
-  hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, s, d, io, p->sumJ);
+  hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, sj, d, io, p->sumJ);
+  if (fork) DCX_HIP(hipEventRecord(p->ev_join, p->side));
   mark();
   hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
   mark();
   hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, s, d, io);
+  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
   mark();
   return check_launch("dcx_pipeline_step");
 }
