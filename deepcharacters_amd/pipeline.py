@@ -92,8 +92,11 @@ def pose_params(rng, ch: scenes.Character, max_angle=0.3, trans=0.01, ed_angle=0
 
 def encode_dt(mask: np.ndarray) -> np.ndarray:
     """uint8 DT image of a binary mask (True = foreground): 127 on the contour,
-    the inside distance (< 127) inside, 127 + distance / 2 outside."""
+    the inside distance (< 127) inside, 127 + distance / 2 outside.  A mask
+    with no foreground is 255 (the farthest outside value) everywhere."""
     from scipy import ndimage
+    if not mask.any():
+        return np.full(mask.shape, 255, np.uint8)
     inside = ndimage.distance_transform_edt(mask)
     outside = ndimage.distance_transform_edt(~mask)
     img = np.where(mask, np.clip(np.rint(inside) - 1, 0, 126), 127 + np.clip(np.rint(outside / 2), 1, 128))

@@ -65,3 +65,34 @@ def test_cpu_step_runs_and_is_finite(small_works):
         r = pipeline_ref.step(wk, 1e-4, 1.0)
         assert np.isfinite(r["loss"]).all() and r["loss"][0] > 0
         assert np.isfinite(r["g_dq"]).all() and np.abs(r["g_dq"]).max() > 0
+
+
+def test_dt_of_an_empty_mask_is_far_outside_everywhere():
+    """A view that sees nothing has no contour: every DT pixel takes the
+    farthest outside value, so neither silhouette term finds a boundary."""
+    for shape in ((5, 5), (16, 16), (75, 75)):
+        img = pipeline.encode_dt(np.zeros(shape, bool))
+        assert img.dtype == np.uint8 and (img == 255).all()
+    m = np.zeros((16, 16), bool)
+    m[5:9, 6:11] = True
+    img = pipeline.encode_dt(m)
+    assert (img[m] <= 127).all() and (img[~m] >= 127).all() and (img == 127).any()
+
+
+def test_degenerate_batch_is_valid_input():
+    """The degenerate batch of the GPU edge tests (one joint, one node, views
+    smaller than a tile, all-culled and off-image views) passes the shape
+    checks, and the oracle's composed step gives a character that none of its
+    views sees exactly zero loss and gradients."""
+    from test_gpu_pipeline_edge import build_edge_works
+    from oracle import pipeline_ref
+    ws, kinds = build_edge_works(_cpu_render)
+    pipeline.check_works(ws)
+    assert ws[0].ch.J == 1 and ws[0].ch.graph.num_nodes == 1 and ws[0].S == 5
+    for wk, kd in zip(ws, kinds):
+        face, _ = _cpu_render(wk, wk.dq, wk.dT, wk.dA)
+        for c, k in enumerate(kd):
+            assert ((face[c] >= 0).any()) == (k in ("seen", "near")), (k, c)
+    ref = pipeline_ref.step(ws[-1], 1.0, 1.0)
+    for key in ("loss", "g_dq", "g_dT", "g_dA", "g_col"):
+        assert not np.any(ref[key]), key
