@@ -650,6 +650,41 @@ __device__ __forceinline__ long long xcd_tile(long long wg, long long n) {
 }
 
 constexpr int kFast = 256;          // tiles with at most this many triangles stage their list in LDS
+
+// Fast-path walk record of one listed triangle, clipped to the tile: the
+// three biased edge values (inside <=> all >= 0) at the centre of the clipped
+// box's first pixel, the per-pixel steps (x in the low, y in the high 16 bits,
+// in units of kSub), 1/z per vertex, the box (tile-relative x0 | x1 << 8 |
+// y0 << 16 | y1 << 24), the depth-key tag and the three biases.  Only
+// triangles whose snapped extent is at most kSmallSpan sub-pixels get one:
+// their edge values at every pixel centre of the box fit 32 bits.
+struct RowRec {
+  int e[3];
+  int dxy[3];
+  float iz[3];
+  int box;
+  unsigned int tag;
+  int bias;  // bit i set: edge i is not top-left (bias -1)
+};
+static_assert(sizeof(RowRec) == sizeof(TriRec), "the walk record takes the staged triangle's place");
+
+__device__ __forceinline__ int step_x(int v) { return (v << 16) >> 8; }  // sign-extended low half * kSub
+__device__ __forceinline__ int step_y(int v) { return (v >> 16) << 8; }
+
+// One row of a walk record: pixels of row r of the clipped box into the LDS keys.
+template <int TWS>
+__device__ __forceinline__ void raster_row(const RowRec& q, int r, unsigned long long* s_key) {
+  const int x0 = q.box & 0xff, x1 = (q.box >> 8) & 0xff, y = ((q.box >> 16) & 0xff) + r;
+  const int b0 = -(q.bias & 1), b1 = -((q.bias >> 1) & 1), b2 = -((q.bias >> 2) & 1);
+  int e0 = q.e[0] + step_y(q.dxy[0]) * r, e1 = q.e[1] + step_y(q.dxy[1]) * r, e2 = q.e[2] + step_y(q.dxy[2]) * r;
+  const int d0 = step_x(q.dxy[0]), d1 = step_x(q.dxy[1]), d2 = step_x(q.dxy[2]);
+  unsigned long long* row = s_key + (y << TWS);
+  for (int x = x0; x <= x1; ++x) {
+    if ((e0 | e1 | e2) >= 0)
+      atomicMin(row + x, make_key(interp_iz32(e0 - b0, e1 - b1, e2 - b2, q.iz), (int)q.tag));
+    e0 += d0; e1 += d1; e2 += d2;
+  }
+}
 constexpr int kGPad = kTilePix + 1;  // row stride of the per-pixel gradient table (bank spread)
 // Per-pixel gradient table rows: 0-8 position gradient (vertex k, axis a at
 // row 3k + a), 9-11 the colour-loss gradient g, 12-13 the weights w0, w1.
@@ -693,7 +728,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   // u1: triangle records and depth keys of the tile (raster) / per-pixel
   // gradient table (backward)
   __shared__ union U1 {
-    struct { TriRec tri[kFast]; unsigned long long key[kTilePix]; } r;
+    struct { RowRec rec[kFast]; unsigned long long key[kTilePix]; unsigned char item[kTilePix]; } r;
     float g[kGRows * kGPad];
   } u1;
   // u2: target colours with apron (per-pixel phase) / counting-sort scratch
@@ -762,21 +797,61 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   int my_f = 0;
   unsigned long long* s_key = u1.r.key;
   if (fast) {
-    // stage the tile's triangle records in LDS
+    // lane k < n_items builds the walk record of list entry k; the rows of
+    // all records are then dealt out to the lanes, 256 rows per round, so a
+    // wavefront's time follows its widest row rather than its largest
+    // triangle.  Triangles wider or taller than kSmallSpan sub-pixels are
+    // walked whole by their own lane afterwards (64-bit edge values).
     s_key[threadIdx.x] = kEmptyKey;
+    unsigned int nrows = 0, ri = 0;
+    bool wide = false;
     if (threadIdx.x < n_items) {
-      const unsigned int ri = d.bins[off + threadIdx.x];
+      ri = d.bins[off + threadIdx.x];
       const TriRec t = d.tri[ri];
-      u1.r.tri[threadIdx.x] = t;
       my_f = t.face_flags & kFaceMask;
       const TileRange r = tile_range(t, tws, ths);
       my_e = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
+      const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
+      const int x1 = min(t.bx1_by1 & 0xffff, tx0 + (1 << tws) - 1), y1 = min(t.bx1_by1 >> 16, ty0 + (1 << ths) - 1);
+      const int spx = max(t.X[0], max(t.X[1], t.X[2])) - min(t.X[0], min(t.X[1], t.X[2]));
+      const int spy = max(t.Y[0], max(t.Y[1], t.Y[2])) - min(t.Y[0], min(t.Y[1], t.Y[2]));
+      if (x0 <= x1 && y0 <= y1) {
+        if (spx <= kSmallSpan && spy <= kSmallSpan) {
+          EdgeWalk w;
+          w.init(t, x0, y0);
+          RowRec q;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            q.e[i] = (int)w.e[i] + w.bias[i];
+            q.dxy[i] = (((int)(w.dx[i] / kSub)) & 0xffff) | ((int)(w.dy[i] / kSub) << 16);
+            q.iz[i] = t.iz[i];
+          }
+          q.box = (x0 - tx0) | ((x1 - tx0) << 8) | ((y0 - ty0) << 16) | ((y1 - ty0) << 24);
+          q.tag = ((unsigned int)my_f << kTagBits) | threadIdx.x;
+          q.bias = (w.bias[0] & 1) | ((w.bias[1] & 1) << 1) | ((w.bias[2] & 1) << 2);
+          u1.r.rec[threadIdx.x] = q;
+          nrows = (unsigned int)(y1 - y0 + 1);
+        } else {
+          wide = true;
+        }
+      }
     }
-    __syncthreads();
-    for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
-      const TriRec& t = u1.r.tri[k];
-      raster_in_tile(t, tx0, ty0, tws, ths, ((unsigned int)(t.face_flags & kFaceMask) << kTagBits) | k, s_key);
+    unsigned int n_rows;
+    const unsigned int row0 = block_scan256(nrows, s_wave, n_rows);
+    s_start[threadIdx.x] = row0;
+    for (unsigned int base = 0; base < n_rows; base += kTilePix) {
+      // entry k's rows [row0, row0 + nrows) that fall in this round
+      const unsigned int lo = max(row0, base), hi = min(row0 + nrows, base + (unsigned int)kTilePix);
+      for (unsigned int j = lo; j < hi; ++j) u1.r.item[j - base] = (unsigned char)threadIdx.x;
+      __syncthreads();
+      if (base + threadIdx.x < n_rows) {
+        const int k = u1.r.item[threadIdx.x];
+        raster_row<tws>(u1.r.rec[k], (int)(base + threadIdx.x - s_start[k]), s_key);
+      }
+      __syncthreads();
     }
+    if (__syncthreads_or(wide))
+      if (wide) raster_in_tile(d.tri[ri], tx0, ty0, tws, ths, ((unsigned int)my_f << kTagBits) | threadIdx.x, s_key);
     __syncthreads();
     const unsigned long long key = s_key[threadIdx.x];
     if (inimg && key != kEmptyKey) {
