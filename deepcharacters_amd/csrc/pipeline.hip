@@ -490,7 +490,7 @@ constexpr int kSmallSpan = 1 << 14;  // 64 px
 
 __device__ __forceinline__ float interp_iz32(int e0, int e1, int e2, const float* iz) {
   const float w0 = (float)e0, w1 = (float)e1, w2 = (float)e2;
-  return (w0 * iz[0] + w1 * iz[1] + w2 * iz[2]) / (w0 + w1 + w2);
+  return w0 * iz[0] + w1 * iz[1] + w2 * iz[2];
 }
 
 __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0, int tws, int ths,

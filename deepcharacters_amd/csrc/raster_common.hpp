@@ -57,7 +57,7 @@ __device__ __forceinline__ int snap(float u) { return (int)rintf(u * (float)kSub
 // Compact per-(view, triangle) record consumed by the fine rasterizer: 48 B.
 struct TriRec {
   int X[3], Y[3];    // snapped coordinates
-  float iz[3];       // 1/z per vertex
+  float iz[3];       // 1/z per vertex divided by the doubled snapped area (sub-pixel units)
   int face_flags;    // face id (bits 0..27) | sgn<0 (bit 28) | not-top-left edge bits (29..31)
   int bx0_by0;       // pixel range lo (x in low 16 bits, y in high 16 bits)
   int bx1_by1;       // pixel range hi
@@ -86,7 +86,10 @@ __device__ __forceinline__ bool setup_tri(f4 p0, f4 p1, f4 p2, int face, int W, 
     if (!top_left(dx, dy)) flags |= 1 << (29 + i);
   }
   t.face_flags = face | flags;
-  t.iz[0] = p0.w; t.iz[1] = p1.w; t.iz[2] = p2.w;
+  // the edge values at any point sum to |area|: dividing 1/z by it once
+  // here leaves a dot product per pixel for the interpolated 1/z
+  const float ra = 1.f / (float)(area > 0 ? area : -area);
+  t.iz[0] = p0.w * ra; t.iz[1] = p1.w * ra; t.iz[2] = p2.w * ra;
   const int mnx = min(t.X[0], min(t.X[1], t.X[2])), mxx = max(t.X[0], max(t.X[1], t.X[2]));
   const int mny = min(t.Y[0], min(t.Y[1], t.Y[2])), mxy = max(t.Y[0], max(t.Y[1], t.Y[2]));
   int x0 = -((-(mnx - kSub / 2)) >> 8), x1 = (mxx - kSub / 2) >> 8;
@@ -103,9 +106,11 @@ __device__ __forceinline__ unsigned long long make_key(float izp, int face) {
   return ((unsigned long long)(~__float_as_uint(izp)) << 32) | (unsigned int)face;
 }
 
+// Perspective-correct interpolated 1/z from the exact edge values and the
+// area-scaled vertex 1/z of the triangle record.
 __device__ __forceinline__ float interp_iz(long long e0, long long e1, long long e2, const float* iz) {
   const float w0 = (float)e0, w1 = (float)e1, w2 = (float)e2;
-  return (w0 * iz[0] + w1 * iz[1] + w2 * iz[2]) / (w0 + w1 + w2);
+  return w0 * iz[0] + w1 * iz[1] + w2 * iz[2];
 }
 
 // Edge-function walker over a rectangle of pixel centres.  Values are exact;

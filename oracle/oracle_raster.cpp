@@ -98,6 +98,7 @@ static inline bool top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 
 struct TriSetup {
   int32_t X[3], Y[3];
   int64_t sgn;
+  int64_t area;  // |doubled signed area| in sub-pixel units: the three edge values sum to it
   int bias[3];
   bool valid;
 };
@@ -111,6 +112,7 @@ static TriSetup setup_tri(const Proj<T>& a, const Proj<T>& b, const Proj<T>& c) 
   int64_t area = (int64_t)(t.X[1] - t.X[0]) * (t.Y[2] - t.Y[0]) - (int64_t)(t.X[2] - t.X[0]) * (t.Y[1] - t.Y[0]);
   if (area == 0) { t.valid = false; return t; }
   t.sgn = area > 0 ? 1 : -1;
+  t.area = area * t.sgn;
   for (int i = 0; i < 3; ++i) {
     int j = (i + 1) % 3, k = (i + 2) % 3;  // edge opposite vertex i runs j -> k
     int64_t dx = (int64_t)(t.X[k] - t.X[j]) * t.sgn, dy = (int64_t)(t.Y[k] - t.Y[j]) * t.sgn;
@@ -142,10 +144,17 @@ static inline void pixel_range(const TriSetup& t, int W, int H, int& x0, int& x1
   x1 = std::min(x1, W - 1); y1 = std::min(y1, H - 1);
 }
 
-// Perspective-correct interpolated 1/z from exact edge values (float).
+// Perspective-correct interpolated 1/z from exact edge values (float):
+// sum_i e_i * (1/z_i) / |area| with the division taken once per triangle,
+// on the vertex values (area_scaled_iz); the e_i sum to |area| exactly.
 static inline float native_iz(const int64_t e[3], const float iz[3]) {
   float w0 = (float)e[0], w1 = (float)e[1], w2 = (float)e[2];
-  return (w0 * iz[0] + w1 * iz[1] + w2 * iz[2]) / (w0 + w1 + w2);
+  return w0 * iz[0] + w1 * iz[1] + w2 * iz[2];
+}
+
+static inline void area_scaled_iz(const TriSetup& t, float iz0, float iz1, float iz2, float out[3]) {
+  const float ra = 1.0f / (float)t.area;
+  out[0] = iz0 * ra; out[1] = iz1 * ra; out[2] = iz2 * ra;
 }
 
 static inline uint64_t native_key(float izp, int face) {
@@ -380,7 +389,8 @@ void raster_fwd(const FwdCfg& cf, const int32_t* faces, const T* tc, const T* vp
         if (!t.valid) continue;
         int x0, x1, y0, y1;
         pixel_range(t, cf.W, cf.H, x0, x1, y0, y1);
-        const float iz[3] = {(float)pr[fv[0]].iz, (float)pr[fv[1]].iz, (float)pr[fv[2]].iz};
+        float iz[3];
+        area_scaled_iz(t, (float)pr[fv[0]].iz, (float)pr[fv[1]].iz, (float)pr[fv[2]].iz, iz);
         for (int py = y0; py <= y1; ++py)
           for (int px = x0; px <= x1; ++px) {
             int64_t e[3];
