@@ -917,6 +917,8 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       // constants) projected back through the fragment's position
       const int fs = kImgFs;
       if (x >= fs + 1 && y >= fs + 1 && x < vw.W - (fs + 1) && y < vw.H - (fs + 1)) {
+        // g . I at each tap of the window, into both sums with the tap weights
+        // xx / (xx^2 + yy^2) and yy / (xx^2 + yy^2); zero weights are skipped
         float tu = 0.f, tv = 0.f;  // -g . (dI/du), -g . (dI/dv) before the norm
 #pragma unroll
         for (int yy = -fs; yy <= fs; ++yy)
@@ -926,8 +928,8 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
             const int a = (py_i + kImgFs + yy) * aw + (px_i + kImgFs + xx);
             const float gi = g.x * u2.tgt[a] + g.y * u2.tgt[an + a] + g.z * u2.tgt[2 * an + a];
             const float den = (float)(xx * xx + yy * yy);
-            tu += gi * ((float)xx / den);
-            tv += gi * ((float)yy / den);
+            if (xx != 0) tu += gi * ((float)xx / den);
+            if (yy != 0) tv += gi * ((float)yy / den);
           }
         const float gu = -tu * kImgGradInvNorm, gvv = -tv * kImgGradInvNorm;
         const float* vb = d.vdef + 3 * (size_t)c.v0;
@@ -997,22 +999,35 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     for (int i = threadIdx.x; i < 3 * (int)n_items; i += kTilePix) {
       const int j = i / 3, k = i - 3 * j;
       const unsigned int st = s_start[j], en = s_start[j + 1];
+      // w_k = s0 w0 + s1 w1 + s2 (1 - w0 - w1) with one s = 1: exact, branch-free
+      const float s0 = k == 0 ? 1.f : 0.f, s1 = k == 1 ? 1.f : 0.f, s2 = k == 2 ? 1.f : 0.f;
+      const float* gk = s_g + (3 * k) * kGPad;
       float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       for (unsigned int r = st; r < en; ++r) {
         const int p = u2.srt.order[r];
         const float w0 = s_g[12 * kGPad + p], w1 = s_g[13 * kGPad + p];
-        const float wk = k == 0 ? w0 : (k == 1 ? w1 : 1.f - w0 - w1);
-        a[0] += s_g[(3 * k) * kGPad + p];
-        a[1] += s_g[(3 * k + 1) * kGPad + p];
-        a[2] += s_g[(3 * k + 2) * kGPad + p];
+        const float wk = (w0 * s0 + w1 * s1) + (1.f - w0 - w1) * s2;
+        a[0] += gk[p];
+        a[1] += gk[kGPad + p];
+        a[2] += gk[2 * kGPad + p];
         a[3] += s_g[9 * kGPad + p] * wk;
         a[4] += s_g[10 * kGPad + p] * wk;
         a[5] += s_g[11 * kGPad + p] * wk;
       }
       const unsigned int e = u2.srt.e[j];
-      const int fj = u2.srt.fj[j];
+      const bool won = en > st;
+      if (e < d.capacity) {
+        if (k == 0) d.rec_vis[e] = won ? 1 : 0;
+        if (won) {
+          float2* rp = reinterpret_cast<float2*>(d.rec + (size_t)e * kGrad + 6 * k);
+          rp[0] = make_float2(a[0], a[1]);
+          rp[1] = make_float2(a[2], a[3]);
+          rp[2] = make_float2(a[4], a[5]);
+        }
+      } else if (won) {
 #pragma unroll
-      for (int cmp = 0; cmp < 6; ++cmp) emit_record(d, io, c, e, fj, 6 * k + cmp, a[cmp], en > st);
+        for (int cmp = 0; cmp < 6; ++cmp) emit_record(d, io, c, e, u2.srt.fj[j], 6 * k + cmp, a[cmp], true);
+      }
     }
     return;
   }
