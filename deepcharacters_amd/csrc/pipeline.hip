@@ -27,8 +27,9 @@
 //   sil grid + sil_pixel   data->model residual via the binned nearest
 //                    boundary-point search, squared loss, gradient
 //   face_sum         per face: its (triangle, tile) records summed over tiles
-//                    and views
-//   vertex_bwd       per vertex: gather of its faces' sums, camera-projection
+//                    and views, each corner's sums stored at its slot of the
+//                    vertex -> face-corner list
+//   vertex_bwd       per vertex: sum of its corner slots, camera-projection
 //                    adjoint, DQ-skinning adjoint, colour gradient
 //   joint_partial    one wavefront per chunk of <= 512 (vertex, slot) entries
 //                    of one joint (static CSR cut into chunks): DQ gradient
@@ -101,7 +102,8 @@ struct Dev {
   const int* n2v_vtx;     // local vertex ids
   const float* n2v_w;
   const int* vf_off;      // [sumN+1] global offsets
-  const int* vf_list;     // local face ids, increasing per vertex
+  const int* vf_list;     // local face ids, increasing per vertex (one entry per face corner)
+  const int* corner_slot; // [sumF*3] global vf_list slot of each face corner
   const int* jv_off;      // [sumJ+1] global offsets: joint -> (vertex, slot) entries
   const int* jv_ent;      // local vertex * 4 + slot, increasing
   const int4* jchunk;     // [nChunk] (first entry, end entry, character vertex base, joint)
@@ -129,7 +131,8 @@ struct Dev {
   unsigned int* cell_fill;
   f4* cell_pts;           // [totVN] (x, y, point id bits, 0) grouped by cell
   unsigned int* scan_aux;
-  float* face_sum;        // [sumF,18] raster gradient per face, summed over tiles and views
+  float* corner_sum;      // [sumF*3,6] raster gradient per face corner (position 3, colour 3), summed over
+                          // tiles and views, stored at the corner's vf_list slot
   float* g_v;             // [sumN,3] position gradient from records beyond capacity
   float* g_pts;           // [totVN,2]
   float* u;               // [sumN,3] gradient w.r.t. the ED-deformed position
@@ -1375,9 +1378,15 @@ __global__ __launch_bounds__(kB) void face_sum_kernel(Dev d) {
       for (int q = 0; q < kGrad; ++q) a[q] += r[q];
     }
   }
-  float* o = d.face_sum + ((size_t)c.f0 + f) * kGrad;
+  // each corner's six sums go to its slot of the vertex's incidence list, so
+  // the vertex pass reads one contiguous range per vertex
 #pragma unroll
-  for (int q = 0; q < kGrad; ++q) o[q] = a[q];
+  for (int k = 0; k < 3; ++k) {
+    float2* o = reinterpret_cast<float2*>(d.corner_sum + 6 * (size_t)d.corner_slot[3 * ((size_t)c.f0 + f) + k]);
+    o[0] = make_float2(a[6 * k], a[6 * k + 1]);
+    o[1] = make_float2(a[6 * k + 2], a[6 * k + 3]);
+    o[2] = make_float2(a[6 * k + 4], a[6 * k + 5]);
+  }
 }
 
 // dL/dv (per-face raster sums + projection adjoint over views) -> DQ-skinning adjoint.
@@ -1392,20 +1401,12 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
     f3 g = ld3(d.g_v + 3 * (size_t)gv);
     f3 gc = ld3(io.g_col + 3 * (size_t)gv);
     const f3 X = ld3(d.vdef + 3 * (size_t)gv);
-    // raster: the per-face sums of the incident faces
-    int prev = -1;
+    // raster: the sums of the vertex's face corners (ascending face, then corner)
     for (int q = d.vf_off[gv]; q < d.vf_off[gv + 1]; ++q) {
-      const int fl = d.vf_list[q];
-      if (fl == prev) continue;  // a face listing this vertex twice is handled once, all slots
-      prev = fl;
-      const int* fv = d.faces + 3 * ((size_t)c.f0 + fl);
-      const float* r = d.face_sum + ((size_t)c.f0 + fl) * kGrad;
-#pragma unroll
-      for (int s = 0; s < 3; ++s)
-        if (fv[s] == v) {
-          g += ld3(r + 6 * s);
-          gc += ld3(r + 6 * s + 3);
-        }
+      const float2* r = reinterpret_cast<const float2*>(d.corner_sum + 6 * (size_t)q);
+      const float2 r0 = r[0], r1 = r[1], r2 = r[2];
+      g += mk3(r0.x, r0.y, r1.x);
+      gc += mk3(r1.y, r2.x, r2.y);
     }
     for (int k = 0; k < c.C; ++k) {
       const int w = c.view0 + k;
@@ -1615,6 +1616,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   std::vector<ViewD> vw;
   std::vector<float> rest, jw, ew;
   std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list, jv_off(1, 0), jv_ent;
+  std::vector<int> corner_slot;
   std::vector<int> jc_off(1, 0);
   std::vector<int4> jchunk;
   std::vector<float> n2v_w;
@@ -1675,8 +1677,13 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     std::vector<int> vfill(vc.begin(), vc.end() - 1);
     const int vbase = (int)vf_list.size();
     vf_list.resize(vbase + 3 * d.F);
+    corner_slot.resize((size_t)3 * (d.f0 + d.F));
     for (int f = 0; f < d.F; ++f)
-      for (int k = 0; k < 3; ++k) vf_list[vbase + vfill[c.faces[3 * f + k]]++] = f;
+      for (int k = 0; k < 3; ++k) {
+        const int slot = vbase + vfill[c.faces[3 * f + k]]++;
+        vf_list[slot] = f;
+        corner_slot[(size_t)3 * (d.f0 + f) + k] = slot;
+      }
     for (int v = 0; v < d.N; ++v) vf_off.push_back(vbase + vc[v + 1]);
     // joint -> (vertex, slot) CSR, entries in increasing (vertex, slot) order
     std::vector<int> jc(d.J + 1, 0);
@@ -1755,20 +1762,20 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     if (e == hipSuccess) track((void*)*dst);
   };
   CharD* dch; ViewD* dvw;
-  float* drest; int *dfaces, *djidx, *denode, *dnv, *dn2o, *dn2v, *dvfo, *dvfl, *djvo, *djve;
+  float* drest; int *dfaces, *djidx, *denode, *dnv, *dn2o, *dn2v, *dvfo, *dvfl, *djvo, *djve, *dcsl;
   float *djw, *dew, *dn2w;
   upload(&dch, ch); upload(&dvw, vw);
   upload(&drest, rest); upload(&dfaces, faces); upload(&djidx, jidx); upload(&djw, jw);
   upload(&denode, enode); upload(&dew, ew); upload(&dnv, node_vtx); upload(&dn2o, n2v_off);
   upload(&dn2v, n2v_vtx); upload(&dn2w, n2v_w); upload(&dvfo, vf_off); upload(&dvfl, vf_list);
-  upload(&djvo, jv_off); upload(&djve, jv_ent);
+  upload(&djvo, jv_off); upload(&djve, jv_ent); upload(&dcsl, corner_slot);
   int4* djck; int* djco;
   upload(&djck, jchunk); upload(&djco, jc_off);
   d.jchunk = djck; d.jc_off = djco;
   p->nChunk = (int)jchunk.size();
   d.ch = dch; d.vw = dvw; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
   d.enode = denode; d.ew = dew; d.node_vtx = dnv; d.n2v_off = dn2o; d.n2v_vtx = dn2v; d.n2v_w = dn2w;
-  d.vf_off = dvfo; d.vf_list = dvfl; d.jv_off = djvo; d.jv_ent = djve;
+  d.vf_off = dvfo; d.vf_list = dvfl; d.jv_off = djvo; d.jv_ent = djve; d.corner_slot = dcsl;
   alloc(&d.node_tab, sizeof(float) * kNodeRow * (size_t)p->sumM);
   alloc(&d.vdef, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.vnrm, sizeof(float) * 3 * (size_t)p->sumN);
@@ -1790,7 +1797,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.cell_off, 4 * (size_t)(cells + 1));
   alloc(&d.cell_fill, 4 * (size_t)cells);
   alloc(&d.cell_pts, sizeof(f4) * (size_t)vn);
-  alloc(&d.face_sum, sizeof(float) * kGrad * (size_t)p->sumF);
+  alloc(&d.corner_sum, sizeof(float) * kGrad * (size_t)p->sumF);
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
   alloc(&d.u, sizeof(float) * 3 * (size_t)p->sumN);
