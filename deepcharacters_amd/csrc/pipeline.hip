@@ -1222,6 +1222,7 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
   __shared__ unsigned int s_rstart[kMaxRegionRows + 1];
   __shared__ unsigned int s_ra[kMaxRegionRows];
   __shared__ unsigned char s_dt[kDtApron * kDtApron];
+  __shared__ unsigned char s_bp[kDtTile * kDtTile];  // the tile's boundary pixels, compacted
   __shared__ unsigned int s_wave[kB / 64];
   __shared__ float s_red[kB / 64];
   const int w = blockIdx.y;
@@ -1296,29 +1297,61 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       }
     }
     __syncthreads();
-    if (bnd) {
-      const float ix = ((float)ui - cr[0]) * sc + cr[3], iy = ((float)vi - cr[1]) * sc + cr[4];
+    // compact the boundary pixels, then give each a group of L lanes (a power
+    // of two, L * count <= 256, at most a wavefront) that stride the points
+    // of every cell row.  The group shares its best distance after each row
+    // for pruning only; each lane keeps its own (distance, index) minimum and
+    // a butterfly merge takes the minimum with the lower index on ties, so
+    // the result is the single-lane one.
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long bm = __ballot(bnd);
+    if (lane == 0) s_wave[wv] = (unsigned int)__popcll(bm);
+    __syncthreads();
+    int nb = 0, before = 0;
+#pragma unroll
+    for (int q = 0; q < kB / 64; ++q) {
+      before += q < wv ? (int)s_wave[q] : 0;
+      nb += (int)s_wave[q];
+    }
+    if (bnd) s_bp[before + __popcll(bm & ((1ull << lane) - 1ull))] = (unsigned char)threadIdx.x;
+    __syncthreads();
+    int lg = 0;  // log2 of the group size
+    if (in_lds)
+      while (lg < 6 && (nb << (lg + 1)) <= kB) ++lg;
+    const int L = 1 << lg, item = (int)threadIdx.x >> lg, sub = (int)threadIdx.x & (L - 1);
+    if (item < nb) {
+      const int pp = s_bp[item];
+      const int pu = tu0 + pp % kDtTile, pv = tv0 + pp / kDtTile;
+      const float ix = ((float)pu - cr[0]) * sc + cr[3], iy = ((float)pv - cr[1]) * sc + cr[4];
       float bd = R, bx = 0.f, by = 0.f;
       int best = -1;
       const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
       if (in_lds) {
         const float step = gg.cell * (1.f - 1e-4f);  // lower bound of the distance per cell of separation
+        float gbd = R;  // the group's best distance so far (pruning bound)
         for (int t = 0; t <= 2 * (kRing + 1); ++t) {
           const int dy = (t & 1) ? (t + 1) >> 1 : -(t >> 1);  // 0, +1, -1, +2, -2, ...
           const int ady = dy < 0 ? -dy : dy;
-          if ((float)(ady > 0 ? ady - 1 : 0) * step > bd) break;
+          if ((float)(ady > 0 ? ady - 1 : 0) * step > gbd) break;
           const int yy = cy + dy;
           if (yy < cy0 || yy > cy1) continue;
-          const int kx = (int)(bd / step) + 1;  // cells with (|xx - cx| - 1) * step <= bd
+          const int kx = (int)(gbd / step) + 1;  // cells with (|xx - cx| - 1) * step <= gbd
           const int xa = max(cx - kx, cx0), xb = min(cx + kx, cx1);
           if (xa > xb) continue;
           const int rb = (yy - cy0) * (ncols + 1);
           const unsigned int q1 = s_coff[rb + xb - cx0 + 1];
-          for (unsigned int q = s_coff[rb + xa - cx0]; q < q1; ++q) {
+          for (unsigned int q = s_coff[rb + xa - cx0] + (unsigned int)sub; q < q1; q += (unsigned int)L) {
             const f2 it = s_pxy[q];
             const float dx = it.x - ix, dyy = it.y - iy;
             nearest_take(sqrtf(dx * dx + dyy * dyy), s_pid[q], it.x, it.y, bd, best, bx, by);
           }
+          gbd = fminf(gbd, bd);
+          for (int o = L >> 1; o > 0; o >>= 1) gbd = fminf(gbd, __shfl_xor(gbd, o, 64));
+        }
+        for (int o = L >> 1; o > 0; o >>= 1) {
+          const float obd = __shfl_xor(bd, o, 64), obx = __shfl_xor(bx, o, 64), oby = __shfl_xor(by, o, 64);
+          const int ob = __shfl_xor(best, o, 64);
+          if (ob >= 0 && (best < 0 || obd < bd || (obd == bd && ob < best))) { bd = obd; best = ob; bx = obx; by = oby; }
         }
       } else {
         for (int k = 0; k <= kRing; ++k) {
@@ -1342,7 +1375,7 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
           if (bd < reach || reach >= R) break;
         }
       }
-      if (best >= 0) {
+      if (sub == 0 && best >= 0) {
         const float r0 = (bx - ix) / (30.f * sc), r1 = (by - iy) / (30.f * sc);
         lsum = io.w_sil * (r0 * r0 + r1 * r1);
         atomicAdd(d.g_pts + 2 * (vw.vn0 + best), 2.f * io.w_sil * r0 / (30.f * sc));
