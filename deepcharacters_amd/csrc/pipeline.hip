@@ -9,9 +9,11 @@
 //   node_fwd         per ED node: Euler delta rotation -> 15-float node row
 //   vertex_fwd       per vertex: ED inner sum (4 nodes) then DQ skinning
 //   normals          per vertex: normalised face-normal sum of the skinned mesh
-//   project          per (view, vertex): image position / depth and the
-//                    image-space normal direction (CameraProjection point +
-//                    vector modes)
+//   view_normals     per (view, vertex): image-space normal direction
+//                    (CameraProjection vector mode); with normals on a second
+//                    stream beside project and the binning stages
+//   project          per (view, vertex): image position / depth
+//                    (CameraProjection point mode)
 //   tri_count        per (view, face): triangle record, tiles touched
 //   tile_scan, tri_scan, bin_scatter   per-tile triangle lists and, per
 //                    triangle, the slots of its (triangle, tile) records
@@ -343,9 +345,22 @@ __global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
       o[9 + r] = K[3 * r] * E[3] + K[3 * r + 1] * E[7] + K[3 * r + 2] * E[11];
     }
   }
-  const f3 X = ld3(d.vdef + 3 * (size_t)gv);
   d.proj[vw.vn0 + v] = project_vertex(E, K, d.vdef + 3 * (size_t)gv);
-  // image-space normal direction (CameraProjectionGPUOp.cu:45-86)
+}
+
+// Image-space normal direction per (view, vertex) for the silhouette
+// (CameraProjectionGPUOp.cu:45-86); only the model->data residual reads it,
+// so it runs beside the binning stages.
+__global__ __launch_bounds__(kB) void view_normal_kernel(Dev d, IO io) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N) return;
+  const float* E = io.ext + (size_t)w * 12;
+  const float* K = io.intr + (size_t)w * 9;
+  const int gv = c.v0 + v;
+  const f3 X = ld3(d.vdef + 3 * (size_t)gv);
   const f3 n = ld3(d.vnrm + 3 * (size_t)gv);
   const f3 xc = mk3(E[0] * X.x + E[1] * X.y + E[2] * X.z + E[3], E[4] * X.x + E[5] * X.y + E[6] * X.z + E[7],
                     E[8] * X.x + E[9] * X.y + E[10] * X.z + E[11]);
@@ -1289,8 +1304,11 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
         s_coff[i] = s_rstart[r] + (d.cell_off[rc + cx0 + k] - s_ra[r]);
       }
       for (unsigned int i = threadIdx.x; i < total; i += kB) {
-        int r = 0;
-        while (r + 1 < nrows && s_rstart[r + 1] <= i) ++r;
+        int r = 0, rh = nrows - 1;  // the last row starting at or before i
+        while (r < rh) {
+          const int mid = (r + rh + 1) >> 1;
+          if (s_rstart[mid] <= i) r = mid; else rh = mid - 1;
+        }
         const f4 it = d.cell_pts[s_ra[r] + (i - s_rstart[r])];
         s_pxy[i] = mk2(it.x, it.y);
         s_pid[i] = __float_as_int(it.z);
@@ -1625,9 +1643,10 @@ struct dcx_pipeline {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // second stream (default priority) and the fork / join events that let the
-  // per-face record sums run beside the silhouette stages
+  // vertex normals run beside the binning stages, the per-face record sums
+  // beside the silhouette stages and the joint reduction beside the node one
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_nfork = nullptr, ev_njoin = nullptr;
 };
 
 static void pipe_free(dcx_pipeline* p) {
@@ -1635,6 +1654,8 @@ static void pipe_free(dcx_pipeline* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->ev_nfork) (void)hipEventDestroy(p->ev_nfork);
+  if (p->ev_njoin) (void)hipEventDestroy(p->ev_njoin);
   if (p->side) (void)hipStreamDestroy(p->side);
   for (void* a : p->allocs) (void)hipFree(a);
   delete p;
@@ -1847,6 +1868,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_nfork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_njoin, hipEventDisableTiming);
   if (e != hipSuccess) {
     pipe_free(p);
     return set_error(DCX_ERR_HIP, "dcx_pipeline_create: %s", hipGetErrorString(e));
@@ -1869,10 +1892,10 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 
 // Launch order of one step; dcx_pipeline_profile times each entry.
 static const char* const kStageNames[] = {
-    "clear",          "node_fwd",      "vertex_fwd",       "normals",   "project",    "tri_count",
-    "tile_scan",      "tri_scan",      "bin_scatter",      "fine",      "boundary",   "sil_point",
-    "sil_grid_count", "sil_grid_scan", "sil_grid_scatter", "sil_pixel", "face_sum",   "vertex_bwd",
-    "joint_partial",  "joint_reduce",  "node_bwd",         "finalize"};
+    "clear",          "node_fwd",      "vertex_fwd",       "normals",   "view_normals", "project",
+    "tri_count",      "tile_scan",     "tri_scan",         "bin_scatter", "fine",       "boundary",
+    "sil_point",      "sil_grid_count", "sil_grid_scan",   "sil_grid_scatter", "sil_pixel", "face_sum",
+    "vertex_bwd",     "joint_partial", "joint_reduce",     "node_bwd",  "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
@@ -1905,7 +1928,22 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d);
+  // Eager and captured steps run the vertex normals and their image-space
+  // directions (read only by the model->data residual) on the side stream
+  // beside the binning stages, the per-face record sums beside the silhouette
+  // stages and the joint-DQ reduction beside the node gradients; a profiled
+  // step keeps every stage on one stream so that its events bracket it.
+  const bool fork = ev == nullptr;
+  hipStream_t sn = s;
+  if (fork) {
+    DCX_HIP(hipEventRecord(p->ev_nfork, s));
+    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_nfork, 0));
+    sn = p->side;
+  }
+  hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, sn, d);
+  mark();
+  hipLaunchKernelGGL(view_normal_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, sn, d, io);
+  if (fork) DCX_HIP(hipEventRecord(p->ev_njoin, p->side));
   mark();
   hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
@@ -1926,11 +1964,6 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     else hipLaunchKernelGGL(fine_kernel<5>, grid, dim3(kTilePix), 0, s, d, io);
   }
   mark();
-  // Eager and captured steps run the per-face record sums (which need only
-  // the fine kernel's records) on the side stream beside the silhouette
-  // stages, and the joint-DQ reduction beside the node gradients; a profiled
-  // step keeps every stage on one stream so that its events bracket it.
-  const bool fork = ev == nullptr;
   if (fork) {
     DCX_HIP(hipEventRecord(p->ev_fork, s));
     DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
@@ -1939,6 +1972,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   }
   hipLaunchKernelGGL(boundary_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
+  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
   hipLaunchKernelGGL(sil_point_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(sil_grid_count_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
