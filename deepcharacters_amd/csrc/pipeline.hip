@@ -655,16 +655,17 @@ __device__ __forceinline__ float block_sum(float v, float* s_red) {
   return t;
 }
 
-// Workgroup id -> linear tile id such that each L2 domain (workgroups are
-// dealt to the XCDs round-robin) gets a contiguous run of tiles, so triangle
-// records shared by neighbouring tiles are fetched into one L2.
-// Bijective on [0, n): domain x receives the q + (x < r) workgroups with
-// wg % kXcd == x (q = n / kXcd, r = n % kXcd) and maps them, in order, onto
-// the run starting at x * q + min(x, r).
-__device__ __forceinline__ long long xcd_tile(long long wg, long long n) {
-  const long long q = n / kXcd, r = n % kXcd;
-  const long long x = wg % kXcd, i = wg / kXcd;
-  return x * q + (x < r ? x : r) + i;
+// Workgroup (x, view) -> tile of the view.  Workgroups are dealt to the L2
+// domains (XCDs) round-robin and the grid's x extent n is a multiple of kXcd,
+// so (x, view) runs on domain x % kXcd.  Each view is cut into kXcd strips of
+// n / kXcd consecutive tiles and domain k takes strip (k + view) % kXcd: a
+// contiguous run, so triangle records and target pixels shared by
+// neighbouring tiles are fetched into one L2, and a different strip in every
+// view, so the heavy middle strips of centred characters spread over all
+// domains (the same strip in every view left the domains holding them far
+// behind the others).  Bijective on [0, n) for every view.
+__device__ __forceinline__ int xcd_tile(unsigned int x, unsigned int n, unsigned int view) {
+  return (int)((((x & (kXcd - 1)) + view) & (kXcd - 1)) * (n / kXcd) + x / kXcd);
 }
 
 constexpr int kFast = 256;          // tiles with at most this many triangles stage their list in LDS
@@ -764,10 +765,8 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   __shared__ unsigned int s_start[kFast + 1];
   __shared__ unsigned int s_wave[kTilePix / 64];
   __shared__ float s_red[kTilePix / 64];
-  const long long lin = xcd_tile(blockIdx.x, (long long)gridDim.x);
-  if (lin >= (long long)d.V * d.maxTiles) return;
-  const int w = (int)(lin / d.maxTiles);
-  const int lt = (int)(lin - (long long)w * d.maxTiles);
+  const int w = blockIdx.y;
+  const int lt = xcd_tile(blockIdx.x, gridDim.x, (unsigned int)w);
   const ViewD vw = d.vw[w];
   if (lt >= vw.tiles) return;
   const CharD c = d.ch[vw.ch];
@@ -1801,6 +1800,10 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     delete p;
     return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: batch too large (records %lld, tiles %lld)", cap, tiles);
   }
+  if (p->V > 65535) {  // views index the grids' y extent
+    delete p;
+    return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: %d views, at most 65535", p->V);
+  }
   d.capacity = (unsigned int)cap;
   const long long aux = std::max(scan_aux_elems(tiles), std::max(scan_aux_elems(vf), scan_aux_elems(cells)));
   hipError_t e = hipSuccess;
@@ -1958,7 +1961,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
   {
-    const dim3 grid((unsigned)((long long)V * d.maxTiles));
+    const dim3 grid((unsigned)(ceil_div(d.maxTiles, kXcd) * kXcd), V);  // x extent a multiple of kXcd
     if (d.tws == 4) hipLaunchKernelGGL(fine_kernel<4>, grid, dim3(kTilePix), 0, s, d, io);
     else if (d.tws == 6) hipLaunchKernelGGL(fine_kernel<6>, grid, dim3(kTilePix), 0, s, d, io);
     else hipLaunchKernelGGL(fine_kernel<5>, grid, dim3(kTilePix), 0, s, d, io);
