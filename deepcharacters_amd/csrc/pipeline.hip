@@ -24,8 +24,9 @@
 //                    model-to-data term), kept in LDS; then the thread that
 //                    rasterised a triangle sums the pixels it won and writes
 //                    one (triangle, tile) gradient record -- no atomics
-//   boundary         per (view, vertex): silhouette-boundary flags
-//   sil_point        model->data DT residual, squared loss, its gradient
+//   sil_vertex       per (view, vertex): silhouette-boundary flag, the
+//                    model->data DT residual, squared loss, its gradient, and
+//                    the boundary vertex's count in the nearest-point grid
 //   sil grid + sil_pixel   data->model residual via the binned nearest
 //                    boundary-point search, squared loss, gradient
 //   face_sum         per face: its (triangle, tile) records summed over tiles
@@ -118,7 +119,8 @@ struct Dev {
   float* ke;              // [V,12] per view: K * R (3x3 row-major), then K * t
   f2* nrm2;               // [totVN]
   TriRec* tri;            // [totVF]
-  unsigned int* tri_cnt;  // [totVF] tiles touched per (view, face)
+  unsigned int* tri_blk;  // [V * ceil(maxF / kB)] tiles touched per binning workgroup, then their
+                          // exclusive scan (the workgroup's first record slot)
   unsigned int* tri_base; // [totVF+1] first record slot per (view, face)
   unsigned int* tile_cnt; // [totTiles]
   unsigned int* tile_off; // [totTiles+1]
@@ -205,51 +207,96 @@ __device__ __forceinline__ unsigned int block_scan256(unsigned int v, unsigned i
   return base + inc - v;
 }
 
-__global__ __launch_bounds__(kB) void scan1024_kernel(const unsigned int* in, unsigned int* out,
-                                                      unsigned int* sums, long long n) {
-  __shared__ unsigned int s_wave[kB / 64];
-  const long long base = (long long)blockIdx.x * 1024 + threadIdx.x * 4;
-  unsigned int v[4], s = 0;
+// Exclusive scan of n counts into out[0..n] (out[n] = total) in two launches
+// over chunks of kScanChunk entries (16 per lane): pass 1 writes each chunk's
+// sum; pass 2 adds up the sums of the chunks before its own (at most a few
+// thousand), scans its chunk and writes it.  No workgroup waits on another.
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kB * kScanPer;
+
+__device__ __forceinline__ void scan_load16(const unsigned int* in, long long base, long long n, unsigned int* v) {
+  if (base + kScanPer <= n) {
+    const uint4* q = reinterpret_cast<const uint4*>(in + base);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) { v[k] = base + k < n ? in[base + k] : 0u; s += v[k]; }
-  unsigned int total;
-  unsigned int ex = block_scan256(s, s_wave, total);
+    for (int k = 0; k < kScanPer / 4; ++k) {
+      const uint4 t = q[k];
+      v[4 * k] = t.x; v[4 * k + 1] = t.y; v[4 * k + 2] = t.z; v[4 * k + 3] = t.w;
+    }
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = ex;
-    ex += v[k];
+    for (int k = 0; k < kScanPer; ++k) v[k] = base + k < n ? in[base + k] : 0u;
   }
+}
+
+__global__ __launch_bounds__(kB) void scan_sum_kernel(const unsigned int* in, unsigned int* sums, long long n) {
+  __shared__ unsigned int s_wave[kB / 64];
+  unsigned int v[kScanPer], t = 0;
+  scan_load16(in, (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanPer, n, v);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) t += v[k];
+  unsigned int total;
+  (void)block_scan256(t, s_wave, total);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kB) void scan_add_kernel(unsigned int* out, const unsigned int* sums, long long n) {
-  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
-  if (i < n) out[i] += sums[i >> 10];
-}
-
-__global__ void copy1_kernel(const unsigned int* src, unsigned int* dst) { *dst = *src; }
-
-// Scratch elements scan_big needs for n inputs (block totals of every level).
-long long scan_aux_elems(long long n) {
-  long long t = 0;
-  while (n > 1) {
-    n = (n + 1023) / 1024;
-    t += n + 1;
+__global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, unsigned int* out,
+                                                        const unsigned int* sums, long long n) {
+  __shared__ unsigned int s_wave[kB / 64];
+  unsigned int pre = 0;
+  for (unsigned int i = threadIdx.x; i < blockIdx.x; i += kB) pre += sums[i];
+  unsigned int offset;
+  (void)block_scan256(pre, s_wave, offset);
+  __syncthreads();  // s_wave is reused below
+  const long long base = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanPer;
+  unsigned int v[kScanPer], t = 0;
+  scan_load16(in, base, n, v);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) t += v[k];
+  unsigned int chunk;
+  unsigned int ex = offset + block_scan256(t, s_wave, chunk);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
   }
-  return t + 2;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = offset + chunk;
 }
+
+// In-place exclusive scan of a[0..n) by one workgroup, kOnePer entries per
+// lane per pass (one pass up to 16384 entries: one launch, its loads all in
+// flight together); *total = the sum.
+constexpr int kOnePer = 64;
+__global__ __launch_bounds__(kB) void scan_one_kernel(unsigned int* a, long long n, unsigned int* total) {
+  __shared__ unsigned int s_wave[kB / 64];
+  unsigned int carry = 0;
+  for (long long b = 0; b < n; b += (long long)kB * kOnePer) {
+    const long long base = b + (long long)threadIdx.x * kOnePer;
+    unsigned int v[kOnePer], t = 0;
+#pragma unroll
+    for (int q = 0; q < kOnePer / kScanPer; ++q) scan_load16(a, base + q * kScanPer, n, v + q * kScanPer);
+#pragma unroll
+    for (int k = 0; k < kOnePer; ++k) t += v[k];
+    unsigned int chunk;
+    unsigned int ex = carry + block_scan256(t, s_wave, chunk);  // each lane writes only the entries it read
+#pragma unroll
+    for (int k = 0; k < kOnePer; ++k) {
+      if (base + k < n) a[base + k] = ex;
+      ex += v[k];
+    }
+    carry += chunk;
+    __syncthreads();  // s_wave is reused by the next pass
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// Scratch elements scan_big needs for n inputs (the chunk sums).
+long long scan_aux_elems(long long n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
 
 dcx_status scan_big(const unsigned int* in, unsigned int* out, unsigned int* aux, long long n, hipStream_t s) {
-  const long long nb = (n + 1023) / 1024;
-  hipLaunchKernelGGL(scan1024_kernel, dim3((unsigned)nb), dim3(kB), 0, s, in, out, aux, n);
-  if (nb == 1) {
-    hipLaunchKernelGGL(copy1_kernel, dim3(1), dim3(1), 0, s, aux, out + n);
-    return check_launch("scan");
-  }
-  dcx_status st = scan_big(aux, aux, aux + nb + 1, nb, s);  // aux[0..nb) offsets, aux[nb] total
-  if (st) return st;
-  hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(n, kB)), dim3(kB), 0, s, out, aux, n);
-  hipLaunchKernelGGL(copy1_kernel, dim3(1), dim3(1), 0, s, aux + nb, out + n);
+  const long long nb = (n + kScanChunk - 1) / kScanChunk;
+  if (nb < 1 || nb >= (1ll << 31)) return set_error(DCX_ERR_INVALID, "scan: bad length %lld", n);
+  hipLaunchKernelGGL(scan_sum_kernel, dim3((unsigned)nb), dim3(kB), 0, s, in, aux, n);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kB), 0, s, in, out, aux, n);
   return check_launch("scan");
 }
 
@@ -411,6 +458,8 @@ __device__ __forceinline__ int hash_find(unsigned int* keys, unsigned int tile, 
 __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   __shared__ unsigned int s_keys[kHash];
   __shared__ unsigned int s_cnt[kHash];
+  __shared__ unsigned int s_wave[kB / 64];
+  unsigned int my_cnt = 0;
   for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
   __syncthreads();
   const int w = blockIdx.y;
@@ -427,11 +476,10 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
       t.bx0_by0 = 0x7fff7fff;
       t.bx1_by1 = 0;
       d.tri[ri] = t;
-      d.tri_cnt[ri] = 0u;
     } else {
       d.tri[ri] = t;
       const TileRange r = tile_range(t, d.tws, d.ths);
-      d.tri_cnt[ri] = (unsigned int)(r.ntx * r.nty);
+      my_cnt = (unsigned int)(r.ntx * r.nty);
       const bool big = r.ntx * r.nty > kBigTri;
       for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
         for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) {
@@ -445,12 +493,18 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   __syncthreads();
   for (int i = threadIdx.x; i < kHash; i += kB)
     if (s_keys[i] != kHashEmpty) atomicAdd(cnt + s_keys[i], s_cnt[i]);
+  // record slots this workgroup's faces need (scanned over the workgroups,
+  // which run in (view, face) order, by scan_one_kernel)
+  unsigned int tot;
+  (void)block_scan256(my_cnt, s_wave, tot);
+  if (threadIdx.x == 0) d.tri_blk[(size_t)w * gridDim.x + blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
   __shared__ unsigned int s_keys[kHash];
   __shared__ unsigned int s_cnt[kHash];
   __shared__ unsigned int s_base[kHash];
+  __shared__ unsigned int s_wave[kB / 64];
   for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
   __syncthreads();
   const int w = blockIdx.y;
@@ -466,6 +520,13 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
       r = tile_range(t, d.tws, d.ths);
       big = r.ntx * r.nty > kBigTri;
     }
+  }
+  // first record slot of each face: the workgroup's offset plus the scan of
+  // its faces' tile counts (the same counts tri_count summed)
+  {
+    unsigned int tot;
+    const unsigned int ex = block_scan256((unsigned int)(r.ntx * r.nty), s_wave, tot);
+    if (f < c.F) d.tri_base[ri] = d.tri_blk[(size_t)w * gridDim.x + blockIdx.x] + ex;
   }
   // pass 1: per-tile counts of this workgroup's pairs
   if (!big)
@@ -1086,76 +1147,22 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
 // Silhouette
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kB) void boundary_kernel_p(Dev d, IO io) {
-  const int w = blockIdx.y;
-  const ViewD vw = d.vw[w];
-  const CharD c = d.ch[vw.ch];
-  const int v = blockIdx.x * kB + threadIdx.x;
-  if (v >= c.N) return;
-  const f4 p = d.proj[vw.vn0 + v];
-  unsigned char fl = 0;
+// Boundary flag of a projected vertex: inside the image with a 1-pixel
+// margin and a background pixel in its 3x3 neighbourhood of the face buffer.
+__device__ __forceinline__ bool boundary_p(const IO& io, const ViewD& vw, f4 p) {
+  bool fl = false;
   // (int)p - 1 >= 0 && (int)p + 1 < W, tested in float (NaN / huge safe)
   if (p.x >= 1.f && p.x < (float)(vw.W - 1) && p.y >= 1.f && p.y < (float)(vw.H - 1)) {
     const int u = (int)p.x, r0 = (int)p.y;
     const int* fb = io.face + vw.pix0;
     for (int cc = u - 1; cc <= u + 1; ++cc)
       for (int r = r0 - 1; r <= r0 + 1; ++r)
-        if (fb[(size_t)r * vw.W + cc] < 0) fl = 1;
+        if (fb[(size_t)r * vw.W + cc] < 0) fl = true;
   }
-  d.isb[vw.vn0 + v] = fl;
+  return fl;
 }
 
 __device__ __forceinline__ float remap_p(int v) { return v >= 127 ? (float)((v - 127) * 2) : (float)v; }
-
-// Model->data residual (MultiViewSilhouetteLossGPUOp.cu:11-138) + loss + gradient.
-__global__ __launch_bounds__(kB) void sil_point_kernel_p(Dev d, IO io) {
-  __shared__ float s_red[kB / 64];
-  const int w = blockIdx.y;
-  const ViewD vw = d.vw[w];
-  const CharD c = d.ch[vw.ch];
-  const int v = blockIdx.x * kB + threadIdx.x;
-  float lsum = 0.f;
-  if (v < c.N) {
-    const long long pi = vw.vn0 + v;
-    const float* cr = io.crops + (size_t)w * 7;
-    const f4 p = d.proj[pi];
-    const float pu = p.x, pv = p.y;
-    const bool in_view = pu - 2 > cr[3] && pu + 2 < cr[5] && pv - 2 > cr[4] && pv + 2 < cr[6];
-    const int xi = (int)((pu - cr[3]) / cr[2] + cr[0]), yi = (int)((pv - cr[4]) / cr[2] + cr[1]);
-    const int U = vw.S, Vv = vw.S;
-    float gx = 0.f, gy = 0.f;
-    if (in_view && xi - 1 >= 0 && xi + 1 < U && yi - 1 >= 0 && yi + 1 < Vv) {
-      const unsigned char* img = io.dt + vw.sil0;
-      auto at = [&](int xx, int yy) { return (int)img[(size_t)yy * U + xx]; };
-      float dv = remap_p(at(xi, yi));
-      const float f00 = remap_p(at(xi - 1, yi - 1)), f01 = remap_p(at(xi, yi - 1)), f02 = remap_p(at(xi + 1, yi - 1));
-      const float f10 = remap_p(at(xi - 1, yi)), f12 = remap_p(at(xi + 1, yi));
-      const float f20 = remap_p(at(xi - 1, yi + 1)), f21 = remap_p(at(xi, yi + 1)), f22 = remap_p(at(xi + 1, yi + 1));
-      const float du = (1.f / 8.f) * ((-1.f) * f00 + (+1.f) * f02 + (-2.f) * f10 + (+2.f) * f12 + (-1.f) * f20 + (+1.f) * f22);
-      const float dvv = (1.f / 8.f) * ((-1.f) * f00 + (-2.f) * f01 + (-1.f) * f02 + (+1.f) * f20 + (+2.f) * f21 + (+1.f) * f22);
-      float nx = du, ny = dvv, ux = du, uy = dvv;
-      const bool inside_fg = at(xi, yi) <= 127;
-      const float nonvan = (du != 0.f || dvv != 0.f) ? 1.f : 0.f;
-      const float bnd = d.isb[pi] ? 1.f : 0.f;
-      if (inside_fg) { nx = -nx; ny = -ny; }
-      const f2 nn = d.nrm2[pi];
-      if (!(nx * nn.x + ny * nn.y >= 0.f) && inside_fg) { ux = -ux; uy = -uy; }
-      dv *= bnd;
-      dv *= nonvan;
-      ux *= bnd; uy *= bnd;
-      ux *= nonvan; uy *= nonvan;
-      const float res = dv * (1.f / (255.f));
-      lsum = io.w_sil * res * res;
-      const float gres = 2.f * io.w_sil * res;
-      gx = ux * (1.f / (255.f)) * gres;
-      gy = uy * (1.f / (255.f)) * gres;
-    }
-    d.g_pts[2 * pi] = gx;
-    d.g_pts[2 * pi + 1] = gy;
-  }
-  const float t = block_sum(lsum, s_red);
-  if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
-}
 
 // Uniform grid over a view's DT crop for the nearest-boundary-point search:
 // kRing cells per search radius R = 30 * scale (cells a hair larger than
@@ -1178,15 +1185,62 @@ __device__ __forceinline__ int cell_of_p(const GridGeomP& g, int gx, int gy, flo
   return (int)fy * gx + (int)fx;
 }
 
-__global__ __launch_bounds__(kB) void sil_grid_count_p(Dev d, IO io) {
+// Per (view, vertex): the boundary flag, the model->data residual
+// (MultiViewSilhouetteLossGPUOp.cu:11-138) with its loss and gradient, and
+// the vertex's count in the nearest-point grid (boundary vertices only).
+__global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
+  __shared__ float s_red[kB / 64];
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
   const int v = blockIdx.x * kB + threadIdx.x;
-  if (v >= c.N || !d.isb[vw.vn0 + v]) return;
-  const f4 p = d.proj[vw.vn0 + v];
-  const int cl = cell_of_p(grid_geom_p(io.crops + (size_t)w * 7), vw.gx, vw.gy, p.x, p.y);
-  if (cl >= 0) atomicAdd(d.cell_cnt + vw.cell0 + cl, 1u);
+  float lsum = 0.f;
+  if (v < c.N) {
+    const long long pi = vw.vn0 + v;
+    const float* cr = io.crops + (size_t)w * 7;
+    const f4 p = d.proj[pi];
+    const bool isb = boundary_p(io, vw, p);
+    d.isb[pi] = isb ? 1 : 0;
+    if (isb) {
+      const int cl = cell_of_p(grid_geom_p(cr), vw.gx, vw.gy, p.x, p.y);
+      if (cl >= 0) atomicAdd(d.cell_cnt + vw.cell0 + cl, 1u);
+    }
+    const float pu = p.x, pv = p.y;
+    const bool in_view = pu - 2 > cr[3] && pu + 2 < cr[5] && pv - 2 > cr[4] && pv + 2 < cr[6];
+    const int xi = (int)((pu - cr[3]) / cr[2] + cr[0]), yi = (int)((pv - cr[4]) / cr[2] + cr[1]);
+    const int U = vw.S, Vv = vw.S;
+    float gx = 0.f, gy = 0.f;
+    if (in_view && xi - 1 >= 0 && xi + 1 < U && yi - 1 >= 0 && yi + 1 < Vv) {
+      const unsigned char* img = io.dt + vw.sil0;
+      auto at = [&](int xx, int yy) { return (int)img[(size_t)yy * U + xx]; };
+      float dv = remap_p(at(xi, yi));
+      const float f00 = remap_p(at(xi - 1, yi - 1)), f01 = remap_p(at(xi, yi - 1)), f02 = remap_p(at(xi + 1, yi - 1));
+      const float f10 = remap_p(at(xi - 1, yi)), f12 = remap_p(at(xi + 1, yi));
+      const float f20 = remap_p(at(xi - 1, yi + 1)), f21 = remap_p(at(xi, yi + 1)), f22 = remap_p(at(xi + 1, yi + 1));
+      const float du = (1.f / 8.f) * ((-1.f) * f00 + (+1.f) * f02 + (-2.f) * f10 + (+2.f) * f12 + (-1.f) * f20 + (+1.f) * f22);
+      const float dvv = (1.f / 8.f) * ((-1.f) * f00 + (-2.f) * f01 + (-1.f) * f02 + (+1.f) * f20 + (+2.f) * f21 + (+1.f) * f22);
+      float nx = du, ny = dvv, ux = du, uy = dvv;
+      const bool inside_fg = at(xi, yi) <= 127;
+      const float nonvan = (du != 0.f || dvv != 0.f) ? 1.f : 0.f;
+      const float bnd = isb ? 1.f : 0.f;
+      if (inside_fg) { nx = -nx; ny = -ny; }
+      const f2 nn = d.nrm2[pi];
+      if (!(nx * nn.x + ny * nn.y >= 0.f) && inside_fg) { ux = -ux; uy = -uy; }
+      dv *= bnd;
+      dv *= nonvan;
+      ux *= bnd; uy *= bnd;
+      ux *= nonvan; uy *= nonvan;
+      const float res = dv * (1.f / (255.f));
+      lsum = io.w_sil * res * res;
+      const float gres = 2.f * io.w_sil * res;
+      gx = ux * (1.f / (255.f)) * gres;
+      gy = uy * (1.f / (255.f)) * gres;
+    }
+    d.g_pts[2 * pi] = gx;
+    d.g_pts[2 * pi + 1] = gy;
+  }
+  const float t = block_sum(lsum, s_red);
+  if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
 }
 
 __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
@@ -1641,11 +1695,10 @@ struct dcx_pipeline {
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // second stream (default priority) and the fork / join events that let the
-  // vertex normals run beside the binning stages, the per-face record sums
-  // beside the silhouette stages and the joint reduction beside the node one
+  // second stream (default priority), the fork event and the join events of
+  // the work it takes beside the main stream (see pipe_enqueue)
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_nfork = nullptr, ev_njoin = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cjoin = nullptr, ev_njoin = nullptr;
 };
 
 static void pipe_free(dcx_pipeline* p) {
@@ -1653,7 +1706,7 @@ static void pipe_free(dcx_pipeline* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-  if (p->ev_nfork) (void)hipEventDestroy(p->ev_nfork);
+  if (p->ev_cjoin) (void)hipEventDestroy(p->ev_cjoin);
   if (p->ev_njoin) (void)hipEventDestroy(p->ev_njoin);
   if (p->side) (void)hipStreamDestroy(p->side);
   for (void* a : p->allocs) (void)hipFree(a);
@@ -1805,7 +1858,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     return set_error(DCX_ERR_INVALID, "dcx_pipeline_create: %d views, at most 65535", p->V);
   }
   d.capacity = (unsigned int)cap;
-  const long long aux = std::max(scan_aux_elems(tiles), std::max(scan_aux_elems(vf), scan_aux_elems(cells)));
+  const long long aux = std::max(scan_aux_elems(tiles), scan_aux_elems(cells));
   hipError_t e = hipSuccess;
   auto track = [&](void* a) { p->allocs.push_back(a); };
   auto alloc = [&](auto** dst, size_t bytes) {
@@ -1840,7 +1893,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.ke, sizeof(float) * 12 * (size_t)p->V);
   alloc(&d.nrm2, sizeof(f2) * (size_t)vn);
   alloc(&d.tri, sizeof(TriRec) * (size_t)vf);
-  alloc(&d.tri_cnt, 4 * (size_t)vf);
+  alloc(&d.tri_blk, 4 * (size_t)p->V * (size_t)ceil_div(maxF, kB));
   alloc(&d.tri_base, 4 * (size_t)(vf + 1));
   alloc(&d.tile_cnt, 4 * (size_t)tiles);
   alloc(&d.tile_off, 4 * (size_t)(tiles + 1));
@@ -1871,7 +1924,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_nfork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_cjoin, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_njoin, hipEventDisableTiming);
   if (e != hipSuccess) {
     pipe_free(p);
@@ -1896,8 +1949,8 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 // Launch order of one step; dcx_pipeline_profile times each entry.
 static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "view_normals", "project",
-    "tri_count",      "tile_scan",     "tri_scan",         "bin_scatter", "fine",       "boundary",
-    "sil_point",      "sil_grid_count", "sil_grid_scan",   "sil_grid_scatter", "sil_pixel", "face_sum",
+    "tri_count",      "tile_scan",     "tri_scan",         "bin_scatter", "fine",       "sil_vertex",
+    "sil_grid_scan",  "sil_grid_scatter", "sil_pixel",     "face_sum",
     "vertex_bwd",     "joint_partial", "joint_reduce",     "node_bwd",  "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
@@ -1924,24 +1977,34 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   int stage = 0;
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
   const int B = p->B, V = p->V;
+  // Eager and captured steps run on two streams: the side stream takes the
+  // buffer clears (needed from tri_count on), the vertex normals and their
+  // image-space directions (read only by the model->data residual) beside the
+  // forward and binning stages, the per-face record sums beside the
+  // silhouette stages, the loss totals, and the joint-DQ reduction beside the
+  // node gradients.  Each fork is an event recorded on s and waited on by the
+  // side stream right away; each join is an event of its own.  A profiled
+  // step keeps every stage on one stream so that its events bracket it.
+  const bool fork = ev == nullptr;
+  auto to_side = [&]() -> dcx_status {
+    DCX_HIP(hipEventRecord(p->ev_fork, s));
+    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    return DCX_OK;
+  };
+  hipStream_t sn = fork ? p->side : s;
   mark();
-  hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, s, L);
+  if (fork) {
+    if (dcx_status e = to_side()) return e;
+  }
+  hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, sn, L);
+  if (fork) DCX_HIP(hipEventRecord(p->ev_cjoin, p->side));
   mark();
   hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
   mark();
-  // Eager and captured steps run the vertex normals and their image-space
-  // directions (read only by the model->data residual) on the side stream
-  // beside the binning stages, the per-face record sums beside the silhouette
-  // stages and the joint-DQ reduction beside the node gradients; a profiled
-  // step keeps every stage on one stream so that its events bracket it.
-  const bool fork = ev == nullptr;
-  hipStream_t sn = s;
   if (fork) {
-    DCX_HIP(hipEventRecord(p->ev_nfork, s));
-    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_nfork, 0));
-    sn = p->side;
+    if (dcx_status e = to_side()) return e;
   }
   hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, sn, d);
   mark();
@@ -1950,13 +2013,14 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
+  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_cjoin, 0));
   hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
   dcx_status st = scan_big(d.tile_cnt, d.tile_off, d.scan_aux, d.totTiles, s);
   if (st) return st;
   mark();
-  st = scan_big(d.tri_cnt, d.tri_base, d.scan_aux, d.totVF, s);
-  if (st) return st;
+  hipLaunchKernelGGL(scan_one_kernel, dim3(1), dim3(kB), 0, s, d.tri_blk, (long long)V * ceil_div(d.maxF, kB),
+                     d.tri_base + d.totVF);
   mark();
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
@@ -1968,17 +2032,12 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   }
   mark();
   if (fork) {
-    DCX_HIP(hipEventRecord(p->ev_fork, s));
-    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    if (dcx_status e = to_side()) return e;
     hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, p->side, d);
     DCX_HIP(hipEventRecord(p->ev_join, p->side));
   }
-  hipLaunchKernelGGL(boundary_kernel_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
-  mark();
   if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
-  hipLaunchKernelGGL(sil_point_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
-  mark();
-  hipLaunchKernelGGL(sil_grid_count_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
   mark();
   st = scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, s);
   if (st) return st;
@@ -1988,7 +2047,10 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, s, d, io);
   mark();
   if (fork) {
-    DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+    // every loss partial is written: the totals go to the side stream
+    if (dcx_status e = to_side()) return e;
+    hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, p->side, d, io);
+    DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // face_sum done
   } else {
     hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
   }
@@ -1997,8 +2059,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipStream_t sj = s;
   if (fork) {
-    DCX_HIP(hipEventRecord(p->ev_fork, s));
-    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    if (dcx_status e = to_side()) return e;
     sj = p->side;
   }
   hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, sj, d,
@@ -2009,8 +2070,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
   mark();
-  hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, s, d, io);
-  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+  if (!fork) hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, s, d, io);
+  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // joint reduction (and all earlier side work) done
   mark();
   return check_launch("dcx_pipeline_step");
 }
