@@ -119,6 +119,48 @@ def cpu_baseline(works, w_col, w_sil, max_workers=16):
                        f"one character per process, 1 thread each; {cpu_s:.1f} s of CPU work, {wall:.1f} s wall")
 
 
+def hbm_copy_gbs(torch, mib=2048, reps=5):
+    """Achievable HBM bandwidth: a device-to-device copy of `mib` MiB (read +
+    write bytes over the event-timed copy)."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def atomic_rates(torch, n_addr=1 << 24, n_ops=1 << 27, reps=3):
+    """Device-scope atomics per second (dcx_probe_atomics): float atomicAdd
+    and 64-bit atomicMin, each on n_addr hashed addresses."""
+    from deepcharacters_amd import _lib
+    out = {}
+    for kind, dtype, name in ((0, torch.float32, "f32_add_per_s"), (1, torch.int64, "u64_min_per_s")):
+        buf = torch.zeros(n_addr, dtype=dtype, device="cuda")
+        s = torch.cuda.current_stream()
+        _lib.call("dcx_probe_atomics", kind, buf.data_ptr(), n_addr, n_ops, s.cuda_stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            _lib.call("dcx_probe_atomics", kind, buf.data_ptr(), n_addr, n_ops, s.cuda_stream)
+        e1.record(s)
+        torch.cuda.synchronize()
+        out[name] = n_ops * reps / (e0.elapsed_time(e1) * 1e-3)
+        del buf
+    out["addresses"] = n_addr
+    out["pattern"] = "multiplicative hash of the op index (scattered)"
+    return out
+
+
 # ----------------------------------------------------------------------------
 
 def main():
@@ -217,6 +259,11 @@ def main():
         log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['value']:.1f} char-frames/s")
         del pp
 
+    copy_gbs = hbm_copy_gbs(torch)
+    atomics = atomic_rates(torch)
+    log(f"device: copy {copy_gbs:.0f} GB/s; atomics {atomics['f32_add_per_s'] / 1e9:.1f} G float adds/s, "
+        f"{atomics['u64_min_per_s'] / 1e9:.1f} G 64-bit mins/s")
+
     fine_ms = stages["fine"]
     fine_b = fine_kernel_bytes(works)
     achieved = fine_b / (fine_ms * 1e-3) / 1e9
@@ -247,10 +294,13 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "hbm", "kernel": "fine_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
-                     "alg_bytes_per_launch": fine_b, "avg_launch_ms": fine_ms},
+                     "alg_bytes_per_launch": fine_b, "avg_launch_ms": fine_ms,
+                     "achievable": copy_gbs, "frac_of_achievable": achieved / copy_gbs,
+                     "achievable_from": "2 GiB device-to-device copy, read + write bytes"},
         "step_roofline": {"alg_bytes_per_step": sb, "achieved": sb / (ms_per_step * 1e-3) / 1e9,
                           "frac": sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms": stages,
+        "atomics": atomics,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -53,6 +53,7 @@ SIGNATURES: dict[str, str] = {
     "dcx_dt_encode": "piiippzS",
     "dcx_dtloss_fwd": "ppppiiiipppzS",
     "dcx_dtloss_bwd": "ppppiiiippS",
+    "dcx_probe_atomics": "ipllS",
 }
 
 # Functions whose return type is not dcx_status.

@@ -583,6 +583,15 @@ int dcx_pipeline_tile_width(const dcx_pipeline* pipe);
 dcx_status dcx_pipeline_autotune(dcx_pipeline* pipe, const dcx_pipeline_io* io, int reps, void* stream,
                                  int* tile_w, float* ms, long long* pairs);
 
+/* Measurement helper for the bench (no counterpart in the reference): n_ops
+ * device-scope atomics on n_addr (a power of two) addresses of buf, picked by
+ * a multiplicative hash of the op index.  kind 0: float atomicAdd of 1.0 on
+ * float buf[n_addr]; kind 1: 64-bit atomicMin on uint64 buf[n_addr].  These
+ * are the operations the reference's raster backward (float adds per covered
+ * pixel, CUDABasedRasterizationGrad.cu:114-496) and depth test (atomicMin per
+ * covered pixel, CUDABasedRasterization.cu:215-255) issue in global memory. */
+dcx_status dcx_probe_atomics(int kind, void* buf, long long n_addr, long long n_ops, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
