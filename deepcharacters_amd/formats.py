@@ -1,0 +1,680 @@
+"""Readers for the reference's on-disk character data and the embedded-graph
+build, so the released character data (not only synthetic scenes) feeds the
+GPU path.
+
+* camera calibration: ``read_calibration`` (Projects/AdditionalUtils/CameraReader.py:16-80)
+* meshes: ``read_off`` / ``write_off`` (Skeletool/Mesh/trimesh.cpp:610-723 loadOff,
+  writeOff / writeCOff) and ``read_obj`` (trimesh.cpp:337-520 loadObj)
+* skeleton: ``read_skeleton`` for "Skeletool Skeleton Definition V1.0"
+  (Skeletool/Character/skeleton.cpp:205-545 loadSkeleton10b / loadSkeleton10)
+* skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData)
+* character: ``read_character`` / ``load_character`` (skinnedcharacter.cpp:425-557 loadCharacter)
+* embedded graph: ``build_embedded_graph`` (Skeletool/Mesh/EmbeddedGraph.cpp:5-460)
+
+All of this is host-side set-up that runs once per character; the arrays it
+returns are what the GPU entry points take.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import scenes
+
+
+# ----------------------------------------------------------------------------
+# Camera calibration
+# ----------------------------------------------------------------------------
+
+@dataclass
+class Calibration:
+    extrinsics: np.ndarray   # [C, 12] float32, 3x4 [R|t] row-major per camera
+    intrinsics: np.ndarray   # [C, 9] float32, 3x3 row-major per camera (rescaled to width x height)
+    width: int
+    height: int
+    original_size: np.ndarray  # [C, 2] (U, V) of the calibration
+
+
+def read_calibration(path: str, render_u: int = -1, render_v: int = -1) -> Calibration:
+    """Cameras of a calibration file (CameraReader.py:16-80): per camera an
+    ``intrinsic`` line of 16 numbers (4x4 row-major; the top-left 3x3 is
+    kept), an ``extrinsic`` line of 16 numbers (the first 12, 3x4, are kept)
+    and a ``size U V`` line.  With a render resolution the focal lengths and
+    principal point are rescaled by render / size of the first camera, as the
+    reference does."""
+    ext, intr, su, sv = [], [], [], []
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if not t:
+                continue
+            if t[0] == "intrinsic":
+                intr.extend(float(t[i]) for i in range(1, len(t)) if i % 4 != 0 and i <= 11)
+            elif t[0] == "extrinsic":
+                ext.extend(float(t[i]) for i in range(1, len(t)) if i <= 12)
+            elif t[0] == "size":
+                su.append(float(t[1]))
+                sv.append(float(t[2]))
+    if not ext or len(ext) % 12 or len(intr) != len(ext) // 12 * 9 or not su:
+        raise ValueError(f"{path}: not a camera calibration file")
+    C = len(ext) // 12
+    e = np.asarray(ext, np.float32).reshape(C, 12)
+    k = np.asarray(intr, np.float32).reshape(C, 3, 3)
+    if render_u > 0 and render_v > 0:
+        u0, v0 = np.float32(su[0]), np.float32(sv[0])
+        for c in range(C):
+            k[c, 0, 0] = (k[c, 0, 0] / u0) * np.float32(render_u)
+            k[c, 1, 1] = (k[c, 1, 1] / v0) * np.float32(render_v)
+            k[c, 0, 2] = (k[c, 0, 2] / u0) * np.float32(render_u)
+            k[c, 1, 2] = (k[c, 1, 2] / v0) * np.float32(render_v)
+        w, h = int(render_u), int(render_v)
+    else:
+        w, h = int(su[0]), int(sv[0])
+    return Calibration(e, k.reshape(C, 9), w, h, np.stack([su, sv], 1).astype(np.float32))
+
+
+def write_calibration(path: str, extrinsics: np.ndarray, intrinsics: np.ndarray, sizes) -> None:
+    """Writes cameras in the layout read_calibration reads (4x4 rows padded
+    with the homogeneous row/column)."""
+    E = np.asarray(extrinsics, np.float64).reshape(-1, 3, 4)
+    K = np.asarray(intrinsics, np.float64).reshape(-1, 3, 3)
+    with open(path, "w") as fh:
+        for c in range(E.shape[0]):
+            k4 = np.eye(4)
+            k4[:3, :3] = K[c]
+            e4 = np.eye(4)
+            e4[:3] = E[c]
+            fh.write(f"name cam{c}\n")
+            fh.write("intrinsic " + " ".join(repr(float(x)) for x in k4.reshape(-1)) + "\n")
+            fh.write("extrinsic " + " ".join(repr(float(x)) for x in e4.reshape(-1)) + "\n")
+            fh.write(f"size {sizes[c][0]} {sizes[c][1]}\n")
+
+
+# ----------------------------------------------------------------------------
+# Meshes
+# ----------------------------------------------------------------------------
+
+@dataclass
+class Mesh:
+    verts: np.ndarray                   # [N, 3] float32
+    faces: np.ndarray                   # [F, 3] int32
+    colors: np.ndarray | None = None    # [N, 3] in [0, 1]
+    texcoords: np.ndarray | None = None  # OBJ: [T, 2] table; STOFF: [N, 2] per vertex
+    face_tindex: np.ndarray | None = None  # OBJ: [F, 3] texcoord index per face corner (-1 if none)
+
+    def face_texcoords(self) -> np.ndarray:
+        """[F, 3, 2] per-corner texture coordinates, the renderer's
+        ``texture_coordinates`` attribute layout."""
+        if self.texcoords is None:
+            raise ValueError("mesh has no texture coordinates")
+        if self.face_tindex is not None:
+            if (self.face_tindex < 0).any():
+                raise ValueError("some faces have no texture coordinates")
+            return self.texcoords[self.face_tindex].astype(np.float32)
+        return self.texcoords[self.faces].astype(np.float32)
+
+
+def read_off(path: str, flip: bool = False) -> Mesh:
+    """OFF / COFF (colours 0-255, divided by 255) / STOFF (u v per vertex)
+    triangle meshes (trimesh.cpp:610-688).  Faces are ``3 a b c`` lines;
+    ``flip`` swaps the last two indices as the reference's loader option."""
+    with open(path) as fh:
+        lines = [ln.split() for ln in fh]
+    lines = [t for t in lines if t]
+    head = lines[0]
+    if len(head) != 1 or head[0] not in ("OFF", "COFF", "STOFF"):
+        raise ValueError(f"{path}: expected an OFF header")
+    coff, stoff = head[0] == "COFF", head[0] == "STOFF"
+    nv, nf = int(lines[1][0]), int(lines[1][1])
+    vrows = lines[2:2 + nv]
+    frows = lines[2 + nv:2 + nv + nf]
+    if len(vrows) != nv or len(frows) != nf:
+        raise ValueError(f"{path}: truncated")
+    verts = np.array([[float(x) for x in t[:3]] for t in vrows], np.float32)
+    colors = texcoords = None
+    if coff:
+        colors = np.array([[np.float32(float(x)) / np.float32(255.0) for x in t[3:6]] for t in vrows], np.float32)
+    if stoff:
+        texcoords = np.array([[float(x) for x in t[3:5]] for t in vrows], np.float32)
+    for t in frows:
+        if len(t) != 4:
+            raise ValueError(f"{path}: only triangles are supported ({len(t)} tokens in a face line)")
+    faces = np.array([[int(x) for x in t[1:4]] for t in frows], np.int32).reshape(-1, 3)
+    if flip:
+        faces = faces[:, [0, 2, 1]]
+    return Mesh(verts, faces, colors, texcoords)
+
+
+def write_off(path: str, verts: np.ndarray, faces: np.ndarray, colors: np.ndarray | None = None) -> None:
+    """OFF as trimesh::writeOff; with colours, COFF as trimesh::writeCOff,
+    which writes colours as int(c * 255) and each face as (a, c, b)
+    (trimesh.cpp:697-743)."""
+    with open(path, "w") as fh:
+        fh.write("COFF\n" if colors is not None else "OFF\n")
+        fh.write(f"{len(verts)} {len(faces)} 0\n")
+        for i, v in enumerate(np.asarray(verts, np.float32)):
+            line = f"{repr(float(v[0]))} {repr(float(v[1]))} {repr(float(v[2]))}"
+            if colors is not None:
+                c = np.asarray(colors[i], np.float32)
+                line += " " + " ".join(str(int(np.float32(x) * np.float32(255.0))) for x in c)
+            fh.write(line + "\n")
+        for f in np.asarray(faces):
+            a, b, c = (int(f[0]), int(f[2]), int(f[1])) if colors is not None else (int(f[0]), int(f[1]), int(f[2]))
+            fh.write(f"3 {a} {b} {c}\n")
+
+
+def read_obj(path: str) -> Mesh:
+    """Wavefront OBJ as trimesh::loadObj (trimesh.cpp:337-520): ``v x y z
+    [r g b]``, ``vc r g b``, ``vt u v``, ``f`` with 1-based ``v[/vt[/vn]]``
+    corners; polygons are split into fans (0, d-1, d).  Texture indices are
+    kept when every corner of a face has one."""
+    verts, cols, vt, faces, tind = [], [], [], [], []
+    with open(path) as fh:
+        for line in fh:
+            t = line.rstrip("\r\n").split()
+            if not t:
+                continue
+            if t[0] == "v":
+                verts.append([float(x) for x in t[1:4]])
+                if len(t) > 5:
+                    cols.append([float(x) for x in t[4:7]])
+            elif t[0] == "vc":
+                cols.append([float(x) for x in t[1:4]])
+            elif t[0] == "vt":
+                vt.append([float(x) for x in t[1:3]])
+            elif t[0] == "f":
+                vi, ti = [], []
+                for tok in t[1:]:
+                    sub = tok.split("/")
+                    vi.append(int(sub[0]) - 1)
+                    if len(sub) > 1 and sub[1]:
+                        ti.append(int(sub[1]) - 1)
+                for d in range(2, len(vi)):
+                    faces.append([vi[0], vi[d - 1], vi[d]])
+                    tind.append([ti[0], ti[d - 1], ti[d]] if len(ti) == len(vi) else [-1, -1, -1])
+    V = np.asarray(verts, np.float32).reshape(-1, 3)
+    Fc = np.asarray(faces, np.int32).reshape(-1, 3)
+    if Fc.size and (Fc.min() < 0 or Fc.max() >= len(V)):
+        raise ValueError(f"{path}: vertex index out of range")
+    colors = np.asarray(cols, np.float32).reshape(-1, 3) if len(cols) == len(V) and cols else None
+    texc = np.asarray(vt, np.float32).reshape(-1, 2) if vt else None
+    ti = np.asarray(tind, np.int32).reshape(-1, 3) if vt else None
+    return Mesh(V, Fc, colors, texc, ti)
+
+
+# ----------------------------------------------------------------------------
+# Skeleton (V1.0), skinning, character
+# ----------------------------------------------------------------------------
+
+@dataclass
+class SkeletonFile:
+    names: list[str]
+    types: list[str]             # "revolute" | "prismatic"
+    parents: list[int]           # -1 for the root
+    offsets: np.ndarray          # [J, 3] as written
+    axes: np.ndarray             # [J, 3] (revolute axes normalised on load)
+    scales: np.ndarray           # [J]
+    markers: list[tuple]         # (name, parent joint, type, offset [3])
+    scaling: list[tuple]         # (name, [joint ids])
+    dofs: list[tuple]            # (name, limits or None, [(joint id, weight)])
+
+
+def _lines(path):
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if t:
+                yield line.strip(), t
+
+
+def read_skeleton(path: str) -> SkeletonFile:
+    """A "Skeletool Skeleton Definition V1.0" file (skeleton.cpp:246-545):
+    ``joints: J`` then ``<name> <revolute|prismatic> <parent> ox oy oz ax ay az
+    scale`` lines (a parent name that is no joint makes the root), ``markers:
+    P`` lines, ``scaling joints: S`` groups and ``dofs: D`` blocks of
+    ``<name> <count>``, ``limits lo hi`` or ``nolimits``, then ``<joint>
+    <weight>`` lines.  Joint references in markers, scaling groups and DOFs
+    match the first joint whose name starts with the given name (the
+    reference's getJointByName prefix lookup)."""
+    it = _lines(path)
+    header, _ = next(it)
+    if header != "Skeletool Skeleton Definition V1.0":
+        raise ValueError(f"{path}: only 'Skeletool Skeleton Definition V1.0' skeletons are supported")
+    _, t = next(it)
+    if len(t) != 2 or t[0] != "joints:":
+        raise ValueError(f"{path}: expected 'joints:'")
+    nj = int(t[1])
+    names, types, pnames, off, ax, sc = [], [], [], [], [], []
+    for _ in range(nj):
+        _, t = next(it)
+        if len(t) != 10 or t[1] not in ("revolute", "prismatic"):
+            raise ValueError(f"{path}: bad joint line {t}")
+        names.append(t[0])
+        types.append(t[1])
+        pnames.append(t[2])
+        off.append([float(x) for x in t[3:6]])
+        a = np.asarray([float(x) for x in t[6:9]], np.float32)
+        if t[1] == "revolute":
+            a = a / np.float32(np.sqrt(np.float32(a @ a)))
+        ax.append(a)
+        sc.append(float(t[9]))
+
+    def exact(n):
+        return names.index(n) if n in names else -1
+
+    def prefix(n):
+        for i, jn in enumerate(names):
+            if jn.startswith(n):
+                return i
+        return -1
+
+    parents = [exact(p) for p in pnames]
+    if sum(p < 0 for p in parents) != 1:
+        raise ValueError(f"{path}: the skeleton needs exactly one root joint")
+    _, t = next(it)
+    if len(t) != 2 or t[0] != "markers:":
+        raise ValueError(f"{path}: expected 'markers:'")
+    markers = []
+    for _ in range(int(t[1])):
+        _, t = next(it)
+        markers.append((t[0], prefix(t[1]), t[2], np.asarray([float(x) for x in t[3:6]], np.float32)))
+    _, t = next(it)
+    if len(t) != 3 or t[0] != "scaling" or t[1] != "joints:":
+        raise ValueError(f"{path}: expected 'scaling joints:'")
+    scaling = []
+    for _ in range(int(t[2])):
+        _, t = next(it)
+        ids = []
+        for _ in range(int(t[1])):
+            _, r = next(it)
+            j = prefix(r[0])
+            if j >= 0:
+                ids.append(j)
+        scaling.append((t[0], ids))
+    _, t = next(it)
+    if len(t) != 2 or t[0] != "dofs:":
+        raise ValueError(f"{path}: expected 'dofs:'")
+    dofs = []
+    for _ in range(int(t[1])):
+        _, t = next(it)
+        name, cnt = t[0], int(t[1])
+        _, lim = next(it)
+        limits = (float(lim[1]), float(lim[2])) if lim[0] == "limits" and len(lim) == 3 else None
+        infl = []
+        for _ in range(cnt):
+            _, r = next(it)
+            j = prefix(r[0])
+            if j < 0:
+                raise ValueError(f"{path}: unknown joint '{r[0]}' in dof '{name}'")
+            infl.append((j, float(r[1])))
+        dofs.append((name, limits, infl))
+    return SkeletonFile(names, types, parents, np.asarray(off, np.float32), np.asarray(ax, np.float32),
+                        np.asarray(sc, np.float32), markers, scaling, dofs)
+
+
+def write_skeleton(path: str, desc: dict) -> None:
+    """A V1.0 skeleton file (the layout read_skeleton reads) from a kinematic
+    description with joint ``names``: one DOF per driven joint, weight 1,
+    named after it, no limits, no markers or scaling groups."""
+    names = desc["names"]
+    J = len(names)
+    with open(path, "w") as fh:
+        fh.write("Skeletool Skeleton Definition V1.0\n")
+        fh.write(f"joints: {J}\n")
+        for j in range(J):
+            p = int(desc["parent"][j])
+            t = "revolute" if int(desc["type"][j]) == scenes.JOINT_REVOLUTE else "prismatic"
+            o, a = desc["offset"][j], desc["axis"][j]
+            fh.write(f"{names[j]} {t} {names[p] if p >= 0 else 'none'} "
+                     + " ".join(repr(float(x)) for x in (*o, *a)) + " 1.0\n")
+        fh.write("markers: 0\nscaling joints: 0\n")
+        driven = [j for j in range(J) if int(desc["dof"][j]) >= 0]
+        driven.sort(key=lambda j: int(desc["dof"][j]))
+        fh.write(f"dofs: {len(driven)}\n")
+        for j in driven:
+            fh.write(f"dof_{names[j]} 1\nnolimits\n{names[j]} 1.0\n")
+
+
+def last_joint_by_name(skel: SkeletonFile, bone: str) -> int:
+    """The last joint whose name starts with ``bone`` (skeleton.cpp:3667-3675),
+    the joint a skinning bone binds to."""
+    for i in range(len(skel.names) - 1, -1, -1):
+        if skel.names[i].startswith(bone):
+            return i
+    raise ValueError(f"no joint matches bone '{bone}'")
+
+
+def skeleton_desc(skel: SkeletonFile, bones: list[str]) -> dict:
+    """The kinematic description skinning.Skeleton takes.  Joint i's local
+    transform is translate(offset_i * scale_i) then the rotation about (or
+    translation along) its axis by its parameter, and the parameter is the
+    weighted sum of the DOFs that drive it (dof.cpp:11-13); ``dof_map``
+    [J, D] holds those weights.  When every joint is driven by at most one DOF
+    with weight 1, ``dof`` indexes the DOF vector directly; otherwise the FK
+    runs on the J joint parameters ``dofs @ dof_map.T`` (``dof`` = joint id).
+    Skinning joint b is the joint its bone name resolves to, bound at its
+    global transform with all DOFs zero."""
+    J, D = len(skel.names), len(skel.dofs)
+    dof_map = np.zeros((J, D), np.float32)
+    for d, (_, _, infl) in enumerate(skel.dofs):
+        for j, w in infl:
+            dof_map[j, d] += np.float32(w)
+    simple = all(np.count_nonzero(dof_map[j]) <= 1 and dof_map[j].max(initial=0.0) in (0.0, 1.0)
+                 and dof_map[j].min(initial=0.0) >= 0.0 for j in range(J))
+    if simple:
+        dof = np.array([int(np.argmax(dof_map[j])) if dof_map[j].any() else -1 for j in range(J)], np.int32)
+        num_dofs = D
+    else:
+        dof = np.arange(J, dtype=np.int32)
+        num_dofs = J
+    typ = np.array([scenes.JOINT_REVOLUTE if t == "revolute" else scenes.JOINT_PRISMATIC for t in skel.types],
+                   np.int32)
+    offset = skel.offsets.astype(np.float64) * skel.scales[:, None].astype(np.float64)
+    # global positions at zero parameters: rotations are identity, so the
+    # transform of each joint is the translation summed along its chain
+    order = _topological(skel.parents)
+    gpos = np.zeros((J, 3))
+    for j in order:
+        p = skel.parents[j]
+        gpos[j] = (gpos[p] if p >= 0 else 0.0) + offset[j]
+    skin_joint = np.array([last_joint_by_name(skel, b) for b in bones], np.int32)
+    bind = np.zeros((len(bones), 12))
+    for b, j in enumerate(skin_joint):
+        bind[b] = [1, 0, 0, -gpos[j, 0], 0, 1, 0, -gpos[j, 1], 0, 0, 1, -gpos[j, 2]]
+    return dict(parent=np.asarray(skel.parents, np.int32), type=typ, dof=dof, offset=offset,
+                axis=skel.axes.astype(np.float64), skin_joint=skin_joint, skin_scale=np.ones(len(bones)),
+                skin_bind=bind, num_dofs=num_dofs, names=list(skel.names), dof_map=dof_map,
+                dof_names=[d[0] for d in skel.dofs], dof_limits=[d[1] for d in skel.dofs], joint_positions=gpos,
+                joint_params=not simple)
+
+
+def joint_parameters(desc: dict, dofs: np.ndarray) -> np.ndarray:
+    """The FK input for a skeleton_desc: the DOF vector itself when each joint
+    has at most one unit-weight DOF, else the joint parameters
+    ``dofs @ dof_map.T``."""
+    dofs = np.asarray(dofs)
+    if not desc.get("joint_params", False):
+        return dofs
+    return dofs @ desc["dof_map"].T
+
+
+def _topological(parents):
+    J = len(parents)
+    done, order = [False] * J, []
+
+    def visit(j):
+        chain = []
+        while j >= 0 and not done[j]:
+            chain.append(j)
+            j = parents[j]
+        for k in reversed(chain):
+            done[k] = True
+            order.append(k)
+
+    for j in range(J):
+        visit(j)
+    return order
+
+
+@dataclass
+class SkinFile:
+    bones: list[str]
+    influences: list[list[tuple[int, float]]]  # per vertex: (bone index, weight)
+
+    def top_k(self, K: int = 4, renormalise: bool = True):
+        """[N, K] bone indices and weights: each vertex's K largest weights
+        (ties keep the file order), padded with (0, 0).  Vertices with more
+        than K influences are renormalised when ``renormalise``."""
+        N = len(self.influences)
+        idx = np.zeros((N, K), np.int32)
+        w = np.zeros((N, K), np.float32)
+        for v, inf in enumerate(self.influences):
+            srt = sorted(range(len(inf)), key=lambda i: -inf[i][1])[:K]
+            for k, i in enumerate(srt):
+                idx[v, k], w[v, k] = inf[i][0], inf[i][1]
+            if renormalise and len(inf) > K and w[v].sum() > 0:
+                w[v] /= w[v].sum()
+        return idx, w
+
+
+def read_skin(path: str) -> SkinFile:
+    """A "Skeletool character skinning file V1.0" (skinnedcharacter.cpp:154-206):
+    ``bones:`` then one line of bone names, ``vertex weights:`` then one line
+    per vertex, ``<vertex> <bone> <weight> <bone> <weight> ...`` (the first
+    token is not used)."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    lines = [ln for ln in lines if ln]
+    if lines[0] != "Skeletool character skinning file V1.0" or lines[1] != "bones:":
+        raise ValueError(f"{path}: not a skeletool skin file")
+    bones = lines[2].split()
+    if lines[3] != "vertex weights:":
+        raise ValueError(f"{path}: expected 'vertex weights:'")
+    infl = []
+    for ln in lines[4:]:
+        t = ln.split()
+        infl.append([(int(t[j]), float(t[j + 1])) for j in range(1, len(t) - 1, 2)])
+    return SkinFile(bones, infl)
+
+
+def write_skin(path: str, bones: list[str], idx: np.ndarray, w: np.ndarray) -> None:
+    with open(path, "w") as fh:
+        fh.write("Skeletool character skinning file V1.0\nbones:\n")
+        fh.write(" ".join(bones) + "\nvertex weights:\n")
+        for v in range(len(idx)):
+            pairs = " ".join(f"{int(i)} {repr(float(x))}" for i, x in zip(idx[v], w[v]) if x != 0)
+            fh.write(f"{v} {pairs}\n")
+
+
+def read_character(path: str) -> dict:
+    """Paths of a "skeletool character file" (v1.0, v0.3, v0.1;
+    skinnedcharacter.cpp:425-505): ``skeleton`` (and, before v1.0, a .dof
+    line after it), ``mesh``, ``skin`` and optional ``pose`` sections, each
+    followed by a path relative to the character file."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    lines = [ln for ln in lines if ln]
+    versions = {"skeletool character file v1.0": "v1.0", "skeletool character file v0.3": "v0.3",
+                "skeletool character file v0.1": "v0.1"}
+    if lines[0] not in versions:
+        raise ValueError(f"{path}: not a skeletool character file")
+    version = versions[lines[0]]
+    base = os.path.dirname(path)
+    out = {"version": version}
+    i = 1
+    while i < len(lines):
+        key = lines[i].split()[0]
+        if key == "skeleton":
+            out["skeleton"] = os.path.join(base, lines[i + 1].split()[0])
+            i += 2
+            if version in ("v0.1", "v0.3"):
+                out["dof"] = os.path.join(base, lines[i].split()[0])
+                i += 1
+        elif key in ("mesh", "skin", "pose"):
+            out[key] = os.path.join(base, lines[i + 1].split()[0])
+            i += 2
+        else:
+            i += 1
+    if "mesh" not in out or "skin" not in out:
+        raise ValueError(f"{path}: no mesh or skinning definition")
+    return out
+
+
+def load_character(path: str, K: int = 4, ed_graph_mesh: str | None = None):
+    """A character from its files: (scenes.Character, skeleton description).
+    Only V1.0 skeletons and .skin skinning are read; the mesh is OFF or OBJ.
+    Skinning joints are the skin file's bones in file order.  With
+    ``ed_graph_mesh`` (the decimated graph mesh) the embedded graph is built
+    as the reference builds it."""
+    paths = read_character(path)
+    if paths["version"] != "v1.0" or "skeleton" not in paths:
+        raise ValueError(f"{path}: only v1.0 characters (skeleton with its dofs) are supported")
+    skel = read_skeleton(paths["skeleton"])
+    mesh = read_obj(paths["mesh"]) if paths["mesh"].lower().endswith(".obj") else read_off(paths["mesh"])
+    if not paths["skin"].endswith(".skin"):
+        raise ValueError(f"{path}: only .skin skinning files are supported")
+    skin = read_skin(paths["skin"])
+    if len(skin.influences) != len(mesh.verts):
+        raise ValueError(f"{path}: {len(skin.influences)} skinned vertices for {len(mesh.verts)} mesh vertices")
+    desc = skeleton_desc(skel, skin.bones)
+    ji, jw = skin.top_k(K)
+    verts = mesh.verts.astype(np.float64)
+    colors = mesh.colors if mesh.colors is not None else np.full((len(verts), 3), 0.5)
+    ch = scenes.Character(verts, mesh.faces, scenes.vertex_normals(verts, mesh.faces),
+                          desc["joint_positions"][desc["skin_joint"]], ji, jw, colors)
+    if ed_graph_mesh is not None:
+        g = read_obj(ed_graph_mesh) if ed_graph_mesh.lower().endswith(".obj") else read_off(ed_graph_mesh)
+        built = build_embedded_graph(mesh.verts, mesh.faces, g.verts, g.faces)
+        ch.graph = built.graph
+        ch.meta["ed_build"] = built
+    return ch, desc
+
+
+# ----------------------------------------------------------------------------
+# Embedded graph construction
+# ----------------------------------------------------------------------------
+
+_FAR = 1000000  # the reference's "unreached" hop distance
+
+
+@dataclass
+class EDGraphBuild:
+    graph: scenes.EmbeddedGraph          # node -> base vertex, vertex -> node CSR and weights
+    radius: np.ndarray                   # [M] hop radius per node
+    node_neighbours: list[list[int]]     # node -> node (the graph mesh's edges), ascending
+    n2v_offsets: np.ndarray              # [M + 1] node -> vertex CSR (vertices ascending)
+    n2v_vertices: np.ndarray
+    n2v_weights: np.ndarray
+    rigidity: np.ndarray                 # [M] mean segmentation weight of the node's vertices
+    unconnected: int = 0                 # vertices outside every radius (tied to the nearest node)
+    meta: dict = field(default_factory=dict)
+
+
+def _adjacency(n: int, faces: np.ndarray):
+    """Vertex adjacency from the faces' edges (trimesh::computeNeighbors), as
+    a CSR matrix and the sorted neighbour lists."""
+    from scipy.sparse import csr_matrix
+    f = np.asarray(faces, np.int64)
+    a = np.concatenate([f[:, [0, 1]], f[:, [0, 2]], f[:, [1, 0]], f[:, [1, 2]], f[:, [2, 0]], f[:, [2, 1]]])
+    a = np.unique(a, axis=0)
+    m = csr_matrix((np.ones(len(a), np.int8), (a[:, 0], a[:, 1])), shape=(n, n))
+    m.sort_indices()
+    return m
+
+
+def _hops(adj, sources) -> np.ndarray:
+    """Hop distances from each source over the adjacency (trimesh::
+    computeGeodesicDistance: breadth-first, unreached = 1000000)."""
+    from scipy.sparse.csgraph import shortest_path
+    d = shortest_path(adj, method="D", unweighted=True, directed=False, indices=np.atleast_1d(sources))
+    d[~np.isfinite(d)] = _FAR
+    return d.astype(np.int64)
+
+
+def _extreme_y(verts):
+    y = np.asarray(verts, np.float32)[:, 1]
+    hi = lo = -1
+    ymax, ymin = np.float32(-10000.0), np.float32(100000.0)
+    for i, v in enumerate(y):  # strict comparisons: the first vertex reaching the extreme
+        if v > ymax:
+            ymax, hi = v, i
+        if v < ymin:
+            ymin, lo = v, i
+    return hi, lo
+
+
+def build_embedded_graph(base_verts, base_faces, graph_verts, graph_faces, seg_weights=None,
+                         refinement: bool = False) -> EDGraphBuild:
+    """The reference's embedded graph (EmbeddedGraph.cpp:34-460) from the
+    character mesh and its decimated graph mesh:
+
+    1. hop distances from the highest and lowest (y) vertex of each mesh mark
+       which of those two each vertex is connected to;
+    2. each graph-mesh vertex, in order, becomes a node at the nearest
+       (float32 Euclidean, first on ties) base vertex that is not taken yet
+       and has the same connectivity marks; its graph-mesh neighbours are its
+       node neighbours;
+    3. node radius = half (integer) the largest hop distance on the base mesh
+       to its neighbour nodes' vertices, at least 3;
+    4. vertex v joins every node i with d = hops(i, v) / radius_i <= 1 at
+       weight exp(-d^2 / 2) (nodes ascending); a vertex no node reaches joins
+       the node with the smallest hop distance (the last one on ties) at
+       weight 1; weights are normalised by their float sum;
+    5. rigidity of a node = mean segmentation weight of its vertices (with
+       ``refinement``, weights below 20 count a tenth).
+    """
+    bv = np.asarray(base_verts, np.float32)
+    gv = np.asarray(graph_verts, np.float32)
+    N, M = len(bv), len(gv)
+    adj_b = _adjacency(N, base_faces)
+    adj_g = _adjacency(M, graph_faces)
+    hb, lb = _extreme_y(bv)
+    hg, lg = _extreme_y(gv)
+    db = _hops(adj_b, [hb, lb])
+    dg = _hops(adj_g, [hg, lg])
+    mark_b = np.stack([db[0] != _FAR, db[1] != _FAR], 1)
+    mark_g = np.stack([dg[0] != _FAR, dg[1] != _FAR], 1)
+    used = np.zeros(N, bool)
+    node_vtx = np.zeros(M, np.int32)
+    for vl in range(M):
+        diff = gv[vl][None, :] - bv
+        dist = np.sqrt((diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1]) + diff[:, 2] * diff[:, 2])
+        ok = ~used & (mark_b[:, 0] == mark_g[vl, 0]) & (mark_b[:, 1] == mark_g[vl, 1])
+        if not ok.any():
+            raise ValueError(f"graph vertex {vl} has no base vertex left to attach to")
+        cand = np.where(ok, dist, np.float32(np.inf))
+        j = int(np.argmin(cand))  # first minimum: the reference's strict '<' scan
+        used[j] = True
+        node_vtx[vl] = j
+    neigh = [list(adj_g.indices[adj_g.indptr[i]:adj_g.indptr[i + 1]]) for i in range(M)]
+    if any(len(n) == 0 for n in neigh):
+        raise ValueError("every graph-mesh vertex needs at least one edge")
+    dist = _hops(adj_b, node_vtx)                       # [M, N]
+    radius = np.zeros(M, np.int64)
+    for n in range(M):
+        r = max(int(dist[n, node_vtx[j]]) for j in neigh[n])
+        radius[n] = max(r // 2, 3)
+    d = dist.astype(np.float32) / radius[:, None].astype(np.float32)   # [M, N] float32 like the reference
+    inside = d <= np.float32(1.0)
+    w = np.where(inside, np.exp(-0.5 * d.astype(np.float64) * d.astype(np.float64)).astype(np.float32),
+                 np.float32(0.0))
+    v2n_off = [0]
+    v2n_idx, v2n_w = [], []
+    unconnected = 0
+    for v in range(N):
+        nodes = np.nonzero(inside[:, v])[0]
+        if len(nodes):
+            ws = w[nodes, v]
+        else:
+            unconnected += 1
+            col = dist[:, v]
+            nodes = np.array([int(np.flatnonzero(col == col.min())[-1])])  # '<=': the last minimum
+            ws = np.array([1.0], np.float32)
+        tot = np.float32(0.0)
+        for x in ws:  # the reference's sequential float sum
+            tot = np.float32(tot + x)
+        v2n_idx.extend(int(n) for n in nodes)
+        v2n_w.extend(np.float32(x) / tot for x in ws)
+        v2n_off.append(len(v2n_idx))
+    v2n_off = np.asarray(v2n_off, np.int32)
+    v2n_idx = np.asarray(v2n_idx, np.int32)
+    v2n_w = np.asarray(v2n_w, np.float32)
+    # node -> vertex CSR in vertex order (EmbeddedGraph.cpp:364-396)
+    order = np.argsort(v2n_idx, kind="stable")
+    vert_of_entry = np.repeat(np.arange(N, dtype=np.int32), np.diff(v2n_off))
+    n2v_vtx = vert_of_entry[order]
+    n2v_w = v2n_w[order]
+    n2v_off = np.concatenate([[0], np.cumsum(np.bincount(v2n_idx, minlength=M))]).astype(np.int32)
+    sw = np.ones(N, np.float32) if seg_weights is None else np.asarray(seg_weights, np.float32)
+    rig = np.zeros(M, np.float32)
+    for n in range(M):
+        vs = n2v_vtx[n2v_off[n]:n2v_off[n + 1]]
+        acc = np.float32(0.0)
+        for v in vs:
+            s = sw[v]
+            acc = np.float32(acc + (s / np.float32(10.0) if refinement and s < 20.0 else s))
+        rig[n] = acc / np.float32(len(vs)) if len(vs) else np.float32(0.0)
+    graph = scenes.EmbeddedGraph(node_vtx, v2n_off, v2n_idx, v2n_w.astype(np.float64))
+    return EDGraphBuild(graph, radius.astype(np.int32), neigh, n2v_off, n2v_vtx, n2v_w, rig, unconnected)

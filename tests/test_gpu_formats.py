@@ -1,0 +1,54 @@
+"""Characters read from the reference's file formats drive the GPU entry
+points: a skeleton / skin / mesh / character file set through the GPU
+forward kinematics and DQ skinning, and an embedded graph built from a mesh
+and its decimated graph mesh through the GPU ED forward, against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from deepcharacters_amd import embedded_graph as edm
+from deepcharacters_amd import formats, scenes, skinning
+from oracle import oracle as orc
+
+from test_formats import _body_files
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def test_character_files_through_gpu_fk_and_skinning(tmp_path):
+    _, _, cp, _ = _body_files(tmp_path)
+    ch, desc = formats.load_character(cp)
+    sk = skinning.Skeleton(desc)
+    B = 2
+    dofs = scenes.random_dofs(np.random.default_rng(4), desc, B, 0.5, 0.03).astype(np.float32)
+    par = formats.joint_parameters(desc, dofs).astype(np.float32)
+    w = np.broadcast_to(ch.joint_w, (B,) + ch.joint_w.shape).astype(np.float32)
+    pos, nrm, jp, ja, dq, _ = skinning.dq_skinning_gpu_dofs(sk, _t(par), _t(w), _t(ch.joint_idx, torch.int32),
+                                                            _t(ch.verts), _t(ch.normals))
+    rdq, _, _ = orc.fk(desc, par.astype(np.float64))
+    rpos, rnrm = orc.dqs_fwd(rdq, w, ch.joint_idx, ch.verts, ch.normals)
+    assert _rel(pos.cpu().numpy(), rpos) < 1e-5 and _rel(nrm.cpu().numpy(), rnrm) < 1e-5
+
+
+def test_built_embedded_graph_through_gpu_ed():
+    bv, bf = scenes.capped_cylinder((0, 0, 0), (0, 1.0, 0), 0.2, 24, 30)
+    gv, gf = scenes.capped_cylinder((0, 0, 0), (0, 1.0, 0), 0.2, 8, 6)
+    b = formats.build_embedded_graph(bv, bf, gv, gf)
+    g = edm.GraphData(bv, b.graph)
+    rng = np.random.default_rng(6)
+    B, M, N = 2, b.graph.num_nodes, len(bv)
+    dT = rng.normal(0, 0.01, (B, M, 3)).astype(np.float32)
+    dA = rng.normal(0, 0.1, (B, M, 3)).astype(np.float32)
+    z = np.zeros((B, M, 3), np.float32)
+    out = edm.embedded_graph_gpu(g, _t(dT), _t(dA), _t(z), _t(z), _t(np.zeros((B, N, 3), np.float32)))
+    ref = orc.ed_fwd(b.graph, bv, dT, dA, z, z, np.zeros((B, N, 3)))
+    assert _rel(out[0].cpu().numpy(), ref[0]) < 1e-5
