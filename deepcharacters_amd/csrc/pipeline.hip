@@ -1493,31 +1493,45 @@ __global__ __launch_bounds__(kB) void face_sum_kernel(Dev d) {
   }
 }
 
-// dL/dv (per-face raster sums + projection adjoint over views) -> DQ-skinning adjoint.
+// dL/dv -> DQ-skinning adjoint, in two parts (the adjoint is linear in
+// dL/dv, so the parts add up to the adjoint of the sum):
+//   SIL = false: the raster part, the sums of the vertex's face corners
+//     (ascending face, then corner) plus any fallback accumulations; writes
+//     the colour gradient, the per-vertex joint-DQ terms, the blend weights
+//     and R^T g.  It needs only the per-face sums, so it runs beside the
+//     silhouette stages.
+//   SIL = true: the silhouette part, the camera-projection adjoint over the
+//     views (CameraProjectionGPUOpGrad.cu:12-79), added to those terms; a
+//     vertex with no silhouette gradient in any view returns at once.
+template <bool SIL>
 __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
   const CharD c = d.ch[blockIdx.y];
   for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
   __syncthreads();
   const int v = blockIdx.x * kB + threadIdx.x;
-  if (v < c.N) {
-    const int gv = c.v0 + v;
-    f3 g = ld3(d.g_v + 3 * (size_t)gv);
+  if (v >= c.N) return;
+  const int gv = c.v0 + v;
+  f3 g = mk3(0.f, 0.f, 0.f);
+  if (!SIL) {
+    g = ld3(d.g_v + 3 * (size_t)gv);
     f3 gc = ld3(io.g_col + 3 * (size_t)gv);
-    const f3 X = ld3(d.vdef + 3 * (size_t)gv);
-    // raster: the sums of the vertex's face corners (ascending face, then corner)
     for (int q = d.vf_off[gv]; q < d.vf_off[gv + 1]; ++q) {
       const float2* r = reinterpret_cast<const float2*>(d.corner_sum + 6 * (size_t)q);
       const float2 r0 = r[0], r1 = r[1], r2 = r[2];
       g += mk3(r0.x, r0.y, r1.x);
       gc += mk3(r1.y, r2.x, r2.y);
     }
+    io.g_col[3 * (size_t)gv] = gc.x; io.g_col[3 * (size_t)gv + 1] = gc.y; io.g_col[3 * (size_t)gv + 2] = gc.z;
+  } else {
+    const f3 X = ld3(d.vdef + 3 * (size_t)gv);
+    bool any = false;
     for (int k = 0; k < c.C; ++k) {
       const int w = c.view0 + k;
       const ViewD vw = d.vw[w];
-      // silhouette: camera-projection adjoint (CameraProjectionGPUOpGrad.cu:12-79)
       const f2 gp = ((const f2*)d.g_pts)[vw.vn0 + v];
       if (gp.x == 0.f && gp.y == 0.f) continue;
+      any = true;
       const float* E = io.ext + (size_t)w * 12;
       const float* K = io.intr + (size_t)w * 9;
       const float xc = E[0] * X.x + E[1] * X.y + E[2] * X.z + E[3];
@@ -1529,39 +1543,46 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
       const float g0 = gp.x * j00, g1 = gp.x * j01 + gp.y * j11, g2 = gp.x * j02 + gp.y * j12;
       g = g + mk3(E[0] * g0 + E[4] * g1 + E[8] * g2, E[1] * g0 + E[5] * g1 + E[9] * g2, E[2] * g0 + E[6] * g1 + E[10] * g2);
     }
-    io.g_col[3 * (size_t)gv] = gc.x; io.g_col[3 * (size_t)gv + 1] = gc.y; io.g_col[3 * (size_t)gv + 2] = gc.z;
-    const f3 vnr = ed_inner(d, c, gv);
-    int idx[KS];
-    float w[KS];
-    for (int k = 0; k < KS; ++k) { idx[k] = d.jidx[(size_t)gv * KS + k]; w[k] = d.jw[(size_t)gv * KS + k]; }
-    const DqBlend bl = dq_blend(s_tab, idx, w, KS);
-    const m3 R = quat_to_mat(bl.r);
-    const float G[9] = {g.x * vnr.x, g.x * vnr.y, g.x * vnr.z, g.y * vnr.x, g.y * vnr.y, g.y * vnr.z,
-                        g.z * vnr.x, g.z * vnr.y, g.z * vnr.z};
-    const f4 r = bl.r, dd = bl.d;
-    f4 gr = dR_dq_contract(r, G);
-    gr.x += 2.f * (-dd.w * g.x - dd.z * g.y + dd.y * g.z);
-    gr.y += 2.f * (dd.z * g.x - dd.w * g.y - dd.x * g.z);
-    gr.z += 2.f * (-dd.y * g.x + dd.x * g.y - dd.w * g.z);
-    gr.w += 2.f * (dd.x * g.x + dd.y * g.y + dd.z * g.z);
-    const f4 gd = mk4(2.f * (r.w * g.x + r.z * g.y - r.y * g.z), 2.f * (-r.z * g.x + r.w * g.y + r.x * g.z),
-                      2.f * (r.y * g.x - r.x * g.y + r.w * g.z), 2.f * (-r.x * g.x - r.y * g.y - r.z * g.z));
-    f4 gbr, gbd;
-    if (bl.small) {
-      gbr = gr; gbd = gd;
-    } else {
-      const float proj = dot(gr, r) + dot(gd, dd);
-      gbr = (gr - r * proj) * bl.scale;
-      gbd = gd * bl.scale;
-    }
-    // the joint DQ gradients sum w_k * s_k * (gbr, gbd) over vertices: kept per
-    // vertex and reduced per joint in a fixed order by joint_reduce_kernel
-    float* q8 = d.vq + 8 * (size_t)gv;
+    if (!any) return;
+  }
+  const f3 vnr = ed_inner(d, c, gv);
+  int idx[KS];
+  float w[KS];
+  for (int k = 0; k < KS; ++k) { idx[k] = d.jidx[(size_t)gv * KS + k]; w[k] = d.jw[(size_t)gv * KS + k]; }
+  const DqBlend bl = dq_blend(s_tab, idx, w, KS);
+  const m3 R = quat_to_mat(bl.r);
+  const float G[9] = {g.x * vnr.x, g.x * vnr.y, g.x * vnr.z, g.y * vnr.x, g.y * vnr.y, g.y * vnr.z,
+                      g.z * vnr.x, g.z * vnr.y, g.z * vnr.z};
+  const f4 r = bl.r, dd = bl.d;
+  f4 gr = dR_dq_contract(r, G);
+  gr.x += 2.f * (-dd.w * g.x - dd.z * g.y + dd.y * g.z);
+  gr.y += 2.f * (dd.z * g.x - dd.w * g.y - dd.x * g.z);
+  gr.z += 2.f * (-dd.y * g.x + dd.x * g.y - dd.w * g.z);
+  gr.w += 2.f * (dd.x * g.x + dd.y * g.y + dd.z * g.z);
+  const f4 gd = mk4(2.f * (r.w * g.x + r.z * g.y - r.y * g.z), 2.f * (-r.z * g.x + r.w * g.y + r.x * g.z),
+                    2.f * (r.y * g.x - r.x * g.y + r.w * g.z), 2.f * (-r.x * g.x - r.y * g.y - r.z * g.z));
+  f4 gbr, gbd;
+  if (bl.small) {
+    gbr = gr; gbd = gd;
+  } else {
+    const float proj = dot(gr, r) + dot(gd, dd);
+    gbr = (gr - r * proj) * bl.scale;
+    gbd = gd * bl.scale;
+  }
+  // the joint DQ gradients sum w_k * s_k * (gbr, gbd) over vertices: kept per
+  // vertex and reduced per joint in a fixed order by joint_reduce_kernel
+  float* q8 = d.vq + 8 * (size_t)gv;
+  const f3 u = mulT(R, g);
+  float* up = d.u + 3 * (size_t)gv;
+  if (!SIL) {
     q8[0] = gbr.x; q8[1] = gbr.y; q8[2] = gbr.z; q8[3] = gbr.w;
     q8[4] = gbd.x; q8[5] = gbd.y; q8[6] = gbd.z; q8[7] = gbd.w;
     for (int k = 0; k < KS; ++k) d.vws[KS * (size_t)gv + k] = w[k] * bl.sgn[k];
-    const f3 u = mulT(R, g);
-    d.u[3 * (size_t)gv] = u.x; d.u[3 * (size_t)gv + 1] = u.y; d.u[3 * (size_t)gv + 2] = u.z;
+    up[0] = u.x; up[1] = u.y; up[2] = u.z;
+  } else {
+    q8[0] += gbr.x; q8[1] += gbr.y; q8[2] += gbr.z; q8[3] += gbr.w;
+    q8[4] += gbd.x; q8[5] += gbd.y; q8[6] += gbd.z; q8[7] += gbd.w;
+    up[0] += u.x; up[1] += u.y; up[2] += u.z;
   }
 }
 
@@ -1950,8 +1971,8 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "view_normals", "project",
     "tri_count",      "tile_scan",     "tri_scan",         "bin_scatter", "fine",       "sil_vertex",
-    "sil_grid_scan",  "sil_grid_scatter", "sil_pixel",     "face_sum",
-    "vertex_bwd",     "joint_partial", "joint_reduce",     "node_bwd",  "finalize"};
+    "sil_grid_scan",  "sil_grid_scatter", "sil_pixel",     "face_sum",  "vertex_bwd_raster",
+    "vertex_bwd_sil", "joint_partial", "joint_reduce",     "node_bwd",  "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // ev == nullptr: plain enqueue.  Otherwise ev[i] is recorded before stage i
@@ -2032,8 +2053,10 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   }
   mark();
   if (fork) {
+    // the raster part of the vertex backward follows the per-face sums there
     if (dcx_status e = to_side()) return e;
     hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, p->side, d);
+    hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, p->side, d, io);
     DCX_HIP(hipEventRecord(p->ev_join, p->side));
   }
   if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
@@ -2050,12 +2073,14 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     // every loss partial is written: the totals go to the side stream
     if (dcx_status e = to_side()) return e;
     hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, p->side, d, io);
-    DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // face_sum done
+    DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // face_sum and the raster vertex part done
   } else {
     hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
+    mark();
+    hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
   }
   mark();
-  hipLaunchKernelGGL(vertex_bwd_kernel, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(vertex_bwd_kernel<true>, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
   mark();
   hipStream_t sj = s;
   if (fork) {
