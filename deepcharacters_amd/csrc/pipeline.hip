@@ -1719,7 +1719,7 @@ struct dcx_pipeline {
   // second stream (default priority), the fork event and the join events of
   // the work it takes beside the main stream (see pipe_enqueue)
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cjoin = nullptr, ev_njoin = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_njoin = nullptr;
 };
 
 static void pipe_free(dcx_pipeline* p) {
@@ -1727,7 +1727,6 @@ static void pipe_free(dcx_pipeline* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-  if (p->ev_cjoin) (void)hipEventDestroy(p->ev_cjoin);
   if (p->ev_njoin) (void)hipEventDestroy(p->ev_njoin);
   if (p->side) (void)hipStreamDestroy(p->side);
   for (void* a : p->allocs) (void)hipFree(a);
@@ -1945,7 +1944,6 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_cjoin, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_njoin, hipEventDisableTiming);
   if (e != hipSuccess) {
     pipe_free(p);
@@ -1999,11 +1997,12 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
   const int B = p->B, V = p->V;
   // Eager and captured steps run on two streams: the side stream takes the
-  // buffer clears (needed from tri_count on), the vertex normals and their
-  // image-space directions (read only by the model->data residual) beside the
-  // forward and binning stages, the per-face record sums beside the
-  // silhouette stages, the loss totals, and the joint-DQ reduction beside the
-  // node gradients.  Each fork is an event recorded on s and waited on by the
+  // vertex normals and their image-space directions (read only by the
+  // model->data residual) beside the forward and binning stages, the
+  // per-face record sums and the raster part of the vertex backward beside
+  // the silhouette stages, the loss totals, and the joint-DQ reduction beside
+  // the node gradients.  (A join costs ~10-20 us in a replayed graph: the
+  // buffer clears, ~8 us, measured faster on the main stream.)  Each fork is an event recorded on s and waited on by the
   // side stream right away; each join is an event of its own.  A profiled
   // step keeps every stage on one stream so that its events bracket it.
   const bool fork = ev == nullptr;
@@ -2014,11 +2013,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   };
   hipStream_t sn = fork ? p->side : s;
   mark();
-  if (fork) {
-    if (dcx_status e = to_side()) return e;
-  }
-  hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, sn, L);
-  if (fork) DCX_HIP(hipEventRecord(p->ev_cjoin, p->side));
+  hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, s, L);
   mark();
   hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
   mark();
@@ -2034,7 +2029,6 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
   mark();
-  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_cjoin, 0));
   hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
   dcx_status st = scan_big(d.tile_cnt, d.tile_off, d.scan_aux, d.totTiles, s);
