@@ -115,6 +115,7 @@ def cpu_baseline(works, w_col, w_sil, max_workers=16):
             os.environ["OMP_NUM_THREADS"] = old
     cpu_s = sum(r[0] for r in res)
     return dict(value=len(works) / wall, unit="char-frames/s", cores=workers, kind="port",
+                single_thread_value=len(works) / cpu_s, host_cpus=os.cpu_count(),
                 sample=f"one full step of the same {len(works)} characters (fwd+bwd, 4 views at 1024x1024), "
                        f"one character per process, 1 thread each; {cpu_s:.1f} s of CPU work, {wall:.1f} s wall")
 
