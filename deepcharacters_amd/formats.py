@@ -6,7 +6,9 @@ GPU path.
 * meshes: ``read_off`` / ``write_off`` (Skeletool/Mesh/trimesh.cpp:610-723 loadOff,
   writeOff / writeCOff) and ``read_obj`` (trimesh.cpp:337-520 loadObj)
 * skeleton: ``read_skeleton`` for "Skeletool Skeleton Definition V1.0"
-  (Skeletool/Character/skeleton.cpp:205-545 loadSkeleton10b / loadSkeleton10)
+  (Skeletool/Character/skeleton.cpp:205-545 loadSkeleton10b / loadSkeleton10) and
+  "Skeleton v0.9" / "v0.10" (skeleton.cpp:533-872 loadSkeleton09), whose DOFs
+  ``read_dof`` takes from a .dof file (skeleton.cpp:1679-2095 loadDof / loadDof05)
 * skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData)
 * character: ``read_character`` / ``load_character`` (skinnedcharacter.cpp:425-557 loadCharacter)
 * embedded graph: ``build_embedded_graph`` (Skeletool/Mesh/EmbeddedGraph.cpp:5-460)
@@ -16,6 +18,7 @@ returns are what the GPU entry points take.
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 from dataclasses import dataclass, field
 
@@ -205,7 +208,7 @@ def read_obj(path: str) -> Mesh:
 
 
 # ----------------------------------------------------------------------------
-# Skeleton (V1.0), skinning, character
+# Skeleton (V1.0; v0.9 + .dof), skinning, character
 # ----------------------------------------------------------------------------
 
 @dataclass
@@ -219,6 +222,8 @@ class SkeletonFile:
     markers: list[tuple]         # (name, parent joint, type, offset [3])
     scaling: list[tuple]         # (name, [joint ids])
     dofs: list[tuple]            # (name, limits or None, [(joint id, weight)])
+    joint_limits: np.ndarray | None = None  # v0.9: [J, 2] per-joint limits (lo == hi: none)
+    dof_file_required: bool = False         # v0.9: the DOFs come from a .dof file (read_dof)
 
 
 def _lines(path):
@@ -237,11 +242,15 @@ def read_skeleton(path: str) -> SkeletonFile:
     ``<name> <count>``, ``limits lo hi`` or ``nolimits``, then ``<joint>
     <weight>`` lines.  Joint references in markers, scaling groups and DOFs
     match the first joint whose name starts with the given name (the
-    reference's getJointByName prefix lookup)."""
+    reference's getJointByName prefix lookup).  "Skeleton v0.9" / "v0.10"
+    files are read by the older layout's reader (no DOFs: see read_dof)."""
     it = _lines(path)
     header, _ = next(it)
+    if header in ("Skeleton v0.9", "Skeleton v0.10"):
+        return _read_skeleton09(path)
     if header != "Skeletool Skeleton Definition V1.0":
-        raise ValueError(f"{path}: only 'Skeletool Skeleton Definition V1.0' skeletons are supported")
+        raise ValueError(f"{path}: '{header}' skeletons are not supported "
+                         "(V1.0, and v0.9 / v0.10 with a .dof file, are)")
     _, t = next(it)
     if len(t) != 2 or t[0] != "joints:":
         raise ValueError(f"{path}: expected 'joints:'")
@@ -337,6 +346,198 @@ def write_skeleton(path: str, desc: dict) -> None:
             fh.write(f"dof_{names[j]} 1\nnolimits\n{names[j]} 1.0\n")
 
 
+def _token_rows(path):
+    with open(path) as fh:
+        return [ln.split() for ln in fh]
+
+
+def _first_prefix(names, n):
+    for i, jn in enumerate(names):
+        if jn.startswith(n):
+            return i
+    return -1
+
+
+def _read_skeleton09(path: str) -> SkeletonFile:
+    """A "Skeleton v0.9" / "Skeleton v0.10" file (skeleton.cpp:533-872), the
+    layout whose DOFs live in a separate .dof file (read_dof).
+    ``BEGIN joints global`` is followed by joint lines ``<name> <parent>
+    <r|t|s> ox oy oz ax ay az ...`` (at least 10 tokens; revolute, prismatic,
+    scaling) whose offsets are global: each is stored relative to the global
+    offset of the earlier joint named exactly as its parent, and a parent
+    name that is no earlier joint makes the root.  ``BEGIN limits`` lines
+    ``<joint> <weight> lo hi`` set per-joint limits (exact names; unknown ones
+    skipped).  ``BEGIN proxies global level 0`` lines ``<name> <parent> ox oy
+    oz size`` are markers, their parent found by prefix and their global
+    offset made relative like the joints'.  The marker colour block after
+    them is not read."""
+    rows = _token_rows(path)
+    n, i = len(rows), 0
+    while i < n and not rows[i]:
+        i += 1
+    i += 1
+    while i < n and len(rows[i]) < 2:
+        i += 1
+    if i >= n or rows[i] != ["BEGIN", "joints", "global"]:
+        raise ValueError(f"{path}: expected 'BEGIN joints global'")
+    i += 1
+    kinds = {"r": "revolute", "t": "prismatic", "s": "scaling"}
+    names, types, parents, glob, off, ax = [], [], [], [], [], []
+    while i < n and len(rows[i]) >= 10:
+        t = rows[i]
+        i += 1
+        if t[2] not in kinds:
+            raise ValueError(f"{path}: unknown joint type '{t[2]}' of joint '{t[0]}'")
+        g = np.asarray([float(x) for x in t[3:6]], np.float32)
+        a = np.asarray([float(x) for x in t[6:9]], np.float32)
+        if t[2] == "r":
+            a = a / np.float32(np.sqrt(np.float32(a @ a)))
+        p = names.index(t[1]) if t[1] in names else -1
+        names.append(t[0])
+        types.append(kinds[t[2]])
+        parents.append(p)
+        glob.append(g)
+        off.append(g - glob[p] if p >= 0 else g)
+        ax.append(a)
+    J = len(names)
+    if sum(p < 0 for p in parents) != 1:
+        raise ValueError(f"{path}: the skeleton needs exactly one root joint")
+    limits = np.zeros((J, 2), np.float32)
+    while i < n and len(rows[i]) < 2:
+        i += 1
+    if i < n and rows[i] == ["BEGIN", "limits"]:
+        i += 1
+        while i < n and (not rows[i] or rows[i][0] != "BEGIN"):
+            t = rows[i]
+            i += 1
+            if len(t) >= 4 and t[0] in names:
+                limits[names.index(t[0])] = (float(t[2]), float(t[3]))
+    markers = []
+    while i < n and len(rows[i]) < 5:
+        i += 1
+    if i < n and rows[i] == ["BEGIN", "proxies", "global", "level", "0"]:
+        i += 1
+        while i < n and len(rows[i]) == 6:
+            t = rows[i]
+            i += 1
+            o = np.asarray([float(x) for x in t[2:5]], np.float32)
+            if t[1] in names:
+                o = o - glob[names.index(t[1])]
+            markers.append((t[0], _first_prefix(names, t[1]), "proxy", o))
+    return SkeletonFile(names, types, parents, np.asarray(off, np.float32).reshape(J, 3),
+                        np.asarray(ax, np.float32).reshape(J, 3), np.ones(J, np.float32), markers, [], [],
+                        joint_limits=limits, dof_file_required=True)
+
+
+def read_dof(path: str, skel: SkeletonFile) -> SkeletonFile:
+    """The DOFs of a v0.9 skeleton from its .dof file (skeleton.cpp:1679-2095),
+    as a copy of ``skel`` with ``dofs`` filled in.
+
+    - "Dofs v0.3": ``dofs: D`` then ``<name> <value> limits lo hi`` or
+      ``<name> <value> nolimits`` lines.
+    - "Dofs v0.1": ``dofs: D`` then ``[<id>] <name> <value>`` lines; limits
+      come from the skeleton's per-joint limits.
+    - "Dofs v0.5": ``BEGIN dofs`` (``<name> <value>`` lines), ``BEGIN
+      scalingpose`` (initial values of scaling DOFs, not kept: a scaling joint
+      driven by a DOF is rejected by skeleton_desc), ``BEGIN map`` up to the
+      first empty line; limits from the skeleton, then DOFs that drive no
+      joint are dropped.
+
+    The joint map (after ``joints: J`` in v0.1 / v0.3) has lines ``<joint> =
+    0 + w1 <dof1> + w2 <dof2> ...``; the joint is found by prefix, each DOF by
+    exact name, and unknown joints or DOF names are skipped.  A per-joint
+    limit (lo != hi) goes to the first DOF that drives the joint, later
+    joints overriding earlier ones."""
+    if not skel.names:
+        raise ValueError("read_dof needs a loaded skeleton")
+    rows = _token_rows(path)
+    n, i = len(rows), 0
+    while i < n and not rows[i]:
+        i += 1
+    header = " ".join(rows[i]) if i < n else ""
+    if header not in ("Dofs v0.5", "Dofs v0.3", "Dofs v0.1"):
+        raise ValueError(f"{path}: expected a 'Dofs v0.1', 'Dofs v0.3' or 'Dofs v0.5' header")
+    version = header.split()[1]
+    i += 1
+    names, lims = [], []
+    if version == "v0.5":
+        while i < n and len(rows[i]) < 2:
+            i += 1
+        if i >= n or rows[i] != ["BEGIN", "dofs"]:
+            raise ValueError(f"{path}: expected 'BEGIN dofs'")
+        i += 1
+        while i < n and len(rows[i]) >= 2 and rows[i][0] != "BEGIN":
+            names.append(rows[i][0])
+            lims.append(None)
+            i += 1
+        while i < n and rows[i] != ["BEGIN", "scalingpose"]:
+            i += 1
+        i += 1
+        while i < n and len(rows[i]) >= 2 and rows[i][0] != "BEGIN":
+            i += 1
+        while i < n and rows[i] != ["BEGIN", "map"]:
+            i += 1
+        i += 1
+        map_rows = []
+        while i < n and rows[i]:
+            map_rows.append(rows[i])
+            i += 1
+    else:
+        while i < n and len(rows[i]) < 2:
+            i += 1
+        if i >= n or len(rows[i]) != 2 or rows[i][0] != "dofs:":
+            raise ValueError(f"{path}: expected 'dofs:'")
+        nd = int(rows[i][1])
+        i += 1
+        for _ in range(nd):
+            t = rows[i] if i < n else []
+            i += 1
+            if version == "v0.3":
+                if len(t) == 5 and t[2] == "limits":
+                    lims.append((float(t[3]), float(t[4])))
+                elif len(t) == 3 and t[2] == "nolimits":
+                    lims.append(None)
+                else:
+                    raise ValueError(f"{path}: bad dof line {t}")
+                names.append(t[0])
+            else:
+                if len(t) < 2:
+                    raise ValueError(f"{path}: bad dof line {t}")
+                names.append(t[1] if len(t) == 3 else t[0])
+                lims.append(None)
+        while i < n and len(rows[i]) < 2:
+            i += 1
+        if i >= n or len(rows[i]) != 2 or rows[i][0] != "joints:":
+            raise ValueError(f"{path}: expected 'joints:'")
+        nj = int(rows[i][1])
+        map_rows = rows[i + 1:i + 1 + nj]
+        if len(map_rows) != nj or any(len(t) < 3 for t in map_rows):
+            raise ValueError(f"{path}: expected {nj} joint lines of at least 3 values")
+    infl = [[] for _ in names]
+    for t in map_rows:
+        j = _first_prefix(skel.names, t[0])
+        if j < 0:
+            continue
+        for d in range(3, len(t), 3):
+            if t[d] != "+":
+                break
+            if d + 2 >= len(t):
+                raise ValueError(f"{path}: truncated dof influence in {t}")
+            if t[d + 2] in names:
+                infl[names.index(t[d + 2])].append((j, float(t[d + 1])))
+    if version in ("v0.1", "v0.5") and skel.joint_limits is not None:
+        for j, (lo, hi) in enumerate(skel.joint_limits):
+            if lo != hi:
+                for q in range(len(names)):
+                    if any(jj == j for jj, _ in infl[q]):
+                        lims[q] = (float(lo), float(hi))
+                        break
+    dofs = [(nm, lm, inf) for nm, lm, inf in zip(names, lims, infl)]
+    if version == "v0.5":
+        dofs = [d for d in dofs if d[2]]
+    return dataclasses.replace(skel, dofs=dofs, dof_file_required=False)
+
+
 def last_joint_by_name(skel: SkeletonFile, bone: str) -> int:
     """The last joint whose name starts with ``bone`` (skeleton.cpp:3667-3675),
     the joint a skinning bone binds to."""
@@ -369,8 +570,16 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str]) -> dict:
     else:
         dof = np.arange(J, dtype=np.int32)
         num_dofs = J
+    # a scaling joint (v0.9 type "s", a 2^p scaling) is the identity at p = 0:
+    # undriven, it is a prismatic joint with no axis; driven, the FK cannot
+    # represent it
+    for j, t in enumerate(skel.types):
+        if t == "scaling" and dof_map[j].any():
+            raise ValueError(f"scaling joint '{skel.names[j]}' is driven by a DOF; the FK has no scaling joints")
     typ = np.array([scenes.JOINT_REVOLUTE if t == "revolute" else scenes.JOINT_PRISMATIC for t in skel.types],
                    np.int32)
+    axes = skel.axes.astype(np.float64)
+    axes[[t == "scaling" for t in skel.types]] = 0.0
     offset = skel.offsets.astype(np.float64) * skel.scales[:, None].astype(np.float64)
     # global positions at zero parameters: rotations are identity, so the
     # transform of each joint is the translation summed along its chain
@@ -384,7 +593,7 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str]) -> dict:
     for b, j in enumerate(skin_joint):
         bind[b] = [1, 0, 0, -gpos[j, 0], 0, 1, 0, -gpos[j, 1], 0, 0, 1, -gpos[j, 2]]
     return dict(parent=np.asarray(skel.parents, np.int32), type=typ, dof=dof, offset=offset,
-                axis=skel.axes.astype(np.float64), skin_joint=skin_joint, skin_scale=np.ones(len(bones)),
+                axis=axes, skin_joint=skin_joint, skin_scale=np.ones(len(bones)),
                 skin_bind=bind, num_dofs=num_dofs, names=list(skel.names), dof_map=dof_map,
                 dof_names=[d[0] for d in skel.dofs], dof_limits=[d[1] for d in skel.dofs], joint_positions=gpos,
                 joint_params=not simple)
@@ -504,14 +713,20 @@ def read_character(path: str) -> dict:
 
 def load_character(path: str, K: int = 4, ed_graph_mesh: str | None = None):
     """A character from its files: (scenes.Character, skeleton description).
-    Only V1.0 skeletons and .skin skinning are read; the mesh is OFF or OBJ.
+    Skeletons are V1.0, or v0.9 / v0.10 completed by the character's .dof file
+    (skinnedcharacter.cpp:515-518); skinning is a .skin file; the mesh is OFF
+    or OBJ.
     Skinning joints are the skin file's bones in file order.  With
     ``ed_graph_mesh`` (the decimated graph mesh) the embedded graph is built
     as the reference builds it."""
     paths = read_character(path)
-    if paths["version"] != "v1.0" or "skeleton" not in paths:
-        raise ValueError(f"{path}: only v1.0 characters (skeleton with its dofs) are supported")
+    if "skeleton" not in paths:
+        raise ValueError(f"{path}: no skeleton")
     skel = read_skeleton(paths["skeleton"])
+    if skel.dof_file_required:
+        if "dof" not in paths:
+            raise ValueError(f"{path}: the skeleton needs a .dof file, which only v0.1 / v0.3 characters name")
+        skel = read_dof(paths["dof"], skel)
     mesh = read_obj(paths["mesh"]) if paths["mesh"].lower().endswith(".obj") else read_off(paths["mesh"])
     if not paths["skin"].endswith(".skin"):
         raise ValueError(f"{path}: only .skin skinning files are supported")

@@ -140,6 +140,166 @@ def test_character_file_loads_mesh_skeleton_and_skin(tmp_path):
                                atol=1e-7)
 
 
+def _write_skel09(path, desc):
+    """The kinematic description as a v0.9 skeleton: global joint offsets,
+    parents before children, no limits or proxies."""
+    names, par = desc["names"], [int(p) for p in desc["parent"]]
+    g = np.zeros((len(names), 3))
+    with open(path, "w") as fh:
+        fh.write("Skeleton v0.9\n\nBEGIN joints global\n")
+        for j, nm in enumerate(names):
+            assert par[j] < j
+            g[j] = (g[par[j]] if par[j] >= 0 else 0.0) + desc["offset"][j]
+            t = "r" if int(desc["type"][j]) == scenes.JOINT_REVOLUTE else "t"
+            fh.write(f"{nm} {names[par[j]] if par[j] >= 0 else 'none'} {t} "
+                     + " ".join(repr(float(x)) for x in (*g[j], *desc["axis"][j])) + " 0\n")
+        fh.write("\nBEGIN limits\n\nBEGIN proxies global level 0\n")
+
+
+def _write_dof(path, desc, version):
+    """One unit-weight DOF per driven joint, in DOF order, in a .dof layout."""
+    names = desc["names"]
+    driven = sorted((j for j in range(len(names)) if int(desc["dof"][j]) >= 0), key=lambda j: int(desc["dof"][j]))
+    rows = [f"{names[j]} = 0 + 1.0 dof_{names[j]}" for j in driven]
+    with open(path, "w") as fh:
+        if version == "v0.5":
+            fh.write("Dofs v0.5\nBEGIN dofs\n" + "".join(f"dof_{names[j]} 0\n" for j in driven))
+            fh.write("BEGIN scalingpose\nBEGIN map\n" + "\n".join(rows) + "\n\n")
+            return
+        fh.write(f"Dofs {version}\ndofs: {len(driven)}\n")
+        for k, j in enumerate(driven):
+            fh.write(f"dof_{names[j]} 0 nolimits\n" if version == "v0.3" else f"{k} dof_{names[j]} 0\n")
+        fh.write(f"\njoints: {len(rows)}\n" + "\n".join(rows) + "\n")
+
+
+def _body_files09(tmp_path, version="v0.3"):
+    """The body character as a v0.3 character file naming a v0.9 skeleton and
+    a .dof file of the given version."""
+    skel, ch, _, _ = _body_files(tmp_path)
+    _write_skel09(str(tmp_path / "body09.skeleton"), skel)
+    _write_dof(str(tmp_path / "body.dof"), skel, version)
+    cp = str(tmp_path / "body09.character")
+    with open(cp, "w") as fh:
+        fh.write("skeletool character file v0.3\nskeleton\nbody09.skeleton\nbody.dof\n"
+                 "mesh\nbody.off\nskin\nbody.skin\n")
+    return skel, ch, cp
+
+
+@pytest.mark.parametrize("version", ["v0.1", "v0.3", "v0.5"])
+def test_v09_skeleton_with_dof_file_matches_the_v10_file(tmp_path, version):
+    skel, ch, cp = _body_files09(tmp_path, version)
+    sf = formats.read_skeleton(str(tmp_path / "body09.skeleton"))
+    assert sf.dof_file_required and not sf.dofs and len(sf.names) == 57
+    sf = formats.read_dof(str(tmp_path / "body.dof"), sf)
+    assert len(sf.dofs) == 57 and all(len(d[2]) == 1 and d[2][0][1] == 1.0 for d in sf.dofs)
+    bones = ["root_"] + [f"j{b}_" for b in range(1, 18)]
+    ref = formats.skeleton_desc(formats.read_skeleton(str(tmp_path / "body.skeleton")), bones)
+    desc = formats.skeleton_desc(sf, bones)
+    for k in ("parent", "type", "dof", "skin_joint"):
+        np.testing.assert_array_equal(desc[k], ref[k], err_msg=k)
+    # global offsets back to relative ones in float32
+    np.testing.assert_allclose(desc["offset"], ref["offset"], atol=2e-6 * np.abs(ref["offset"]).max())
+    np.testing.assert_allclose(desc["axis"], ref["axis"], atol=1e-7)
+    dofs = scenes.random_dofs(np.random.default_rng(7), skel, 2)
+    a = orc.fk(ref, formats.joint_parameters(ref, dofs))
+    b = orc.fk(desc, formats.joint_parameters(desc, dofs))
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, atol=1e-5 * max(1.0, np.abs(y).max()))
+    # the v0.3 character file names the .dof file after the skeleton
+    paths = formats.read_character(cp)
+    assert paths["version"] == "v0.3" and os.path.basename(paths["dof"]) == "body.dof"
+    ch2, d2 = formats.load_character(cp)
+    np.testing.assert_array_equal(d2["skin_joint"], ref["skin_joint"])
+    np.testing.assert_allclose(ch2.joint_centres, ref["joint_positions"][ref["skin_joint"]],
+                               atol=2e-6 * np.abs(ref["offset"]).max())
+
+
+_SKEL09 = """Skeleton v0.10
+
+BEGIN joints global
+root none t 0 0 0 1 0 0 0
+hip root r 0 1 0 0 0 2 0
+knee hip r 1 1 0 0 0 1 0
+sc knee s 1 1 0 1 1 1 0
+
+BEGIN limits
+hip 1 -0.5 0.5
+knee 1 0 0
+nosuch 1 -1 1
+BEGIN proxies global level 0
+m1 knee 1 2 0 0.05
+m2 hi 1 1 1 0.05
+BEGIN textures histogram level 0
+0 0 0
+"""
+
+_MAP = ("root = 0 + 1.0 slide\nhip = 0 + 0.5 bend + 2.0 slide\nkn = 0 + 1.0 bend + 1.0 nosuchdof\n"
+        "ghost = 0 + 1.0 bend\n")
+
+
+@pytest.mark.parametrize("version", ["v0.1", "v0.3", "v0.5"])
+def test_dof_file_weights_limits_and_unknown_names(tmp_path, version):
+    sp = tmp_path / "s.skeleton"
+    sp.write_text(_SKEL09)
+    sf = formats.read_skeleton(str(sp))
+    assert sf.parents == [-1, 0, 1, 2] and sf.types == ["prismatic", "revolute", "revolute", "scaling"]
+    np.testing.assert_array_equal(sf.offsets, [[0, 0, 0], [0, 1, 0], [1, 0, 0], [0, 0, 0]])  # relative
+    np.testing.assert_allclose(sf.axes[1], [0, 0, 1])                                      # normalised
+    np.testing.assert_array_equal(sf.joint_limits, [[0, 0], [-0.5, 0.5], [0, 0], [0, 0]])
+    (n1, p1, _, o1), (n2, p2, _, o2) = sf.markers
+    assert (n1, p1, n2, p2) == ("m1", 2, "m2", 1)                 # m2's parent found by prefix
+    np.testing.assert_array_equal(o1, [0, 1, 0])                  # relative to knee's global offset
+    np.testing.assert_array_equal(o2, [1, 1, 1])                  # no exact parent name: kept as written
+    dp = tmp_path / "s.dof"
+    if version == "v0.5":
+        dp.write_text("Dofs v0.5\nBEGIN dofs\nslide 0\nbend 0\nunused 0\nBEGIN scalingpose\nbend 0.25\n"
+                      "BEGIN map\n" + _MAP + "\nknee = 0 + 3.0 slide\n")  # after the map's end
+    elif version == "v0.3":
+        dp.write_text("Dofs v0.3\ndofs: 3\nslide 0 limits -1 1\nbend 0 nolimits\nunused 0 nolimits\n"
+                      "joints: 4\n" + _MAP)
+    else:
+        dp.write_text("Dofs v0.1\n\ndofs: 3\n0 slide 0\n1 bend 0\nunused 0\n\njoints: 4\n" + _MAP)
+    sf = formats.read_dof(str(dp), sf)
+    d = {nm: (lim, infl) for nm, lim, infl in sf.dofs}
+    assert d["slide"][1] == [(0, 1.0), (1, 2.0)] and d["bend"][1] == [(1, 0.5), (2, 1.0)]
+    # v0.3 limits come from the dof lines; v0.1 / v0.5 from the skeleton, onto
+    # the first DOF that drives the limited joint (hip: slide)
+    assert d["slide"][0] == ((-1.0, 1.0) if version == "v0.3" else (-0.5, 0.5)) and d["bend"][0] is None
+    assert ("unused" in d) == (version != "v0.5")                  # v0.5 drops DOFs that drive nothing
+    assert list(d)[:2] == ["slide", "bend"]
+    desc = formats.skeleton_desc(sf, ["knee"])
+    assert desc["joint_params"] and list(desc["type"]) == [scenes.JOINT_PRISMATIC, scenes.JOINT_REVOLUTE,
+                                                           scenes.JOINT_REVOLUTE, scenes.JOINT_PRISMATIC]
+    np.testing.assert_array_equal(desc["axis"][3], [0, 0, 0])     # undriven scaling joint: identity
+    dofs = np.array([[0.3, 0.4] + [0.0] * (len(sf.dofs) - 2)])
+    par = formats.joint_parameters(desc, dofs)
+    np.testing.assert_allclose(par[0, :3], [0.3, 0.2 + 0.6, 0.4], atol=1e-7)
+    _, jp, _, _ = orc.fk(desc, par, transforms=True)
+    # knee: 1 along x from the hip, turned 0.8 rad about z by the hip
+    np.testing.assert_allclose(jp[0, 2], [0.3 + np.cos(0.8), 1 + np.sin(0.8), 0], atol=1e-6)
+    np.testing.assert_allclose(jp[0, 3], jp[0, 2], atol=1e-12)
+
+
+def test_driven_scaling_joints_and_missing_dof_files_are_rejected(tmp_path):
+    sp = tmp_path / "s.skeleton"
+    sp.write_text(_SKEL09)
+    dp = tmp_path / "s.dof"
+    dp.write_text("Dofs v0.3\ndofs: 1\ngrow 0 nolimits\njoints: 1\nsc = 0 + 1.0 grow\n")
+    sf = formats.read_dof(str(dp), formats.read_skeleton(str(sp)))
+    with pytest.raises(ValueError, match="scaling joint"):
+        formats.skeleton_desc(sf, ["knee"])
+    cp = tmp_path / "c.character"
+    (tmp_path / "m.off").write_text("OFF\n3 1 0\n0 0 0\n1 0 0\n0 1 0\n3 0 1 2\n")
+    (tmp_path / "m.skin").write_text("")
+    cp.write_text("skeletool character file v1.0\nskeleton\ns.skeleton\nmesh\nm.off\nskin\nm.skin\n")
+    with pytest.raises(ValueError, match=".dof file"):
+        formats.load_character(str(cp))
+    bad = tmp_path / "b.dof"
+    bad.write_text("Dofs v0.3\ndofs: 1\nx 0 limits 1\njoints: 0\n")
+    with pytest.raises(ValueError, match="bad dof line"):
+        formats.read_dof(str(bad), formats.read_skeleton(str(sp)))
+
+
 def test_skin_top_k_keeps_the_largest_and_renormalises():
     sk = formats.SkinFile(["a", "b"], [[(0, 0.5), (1, 0.1), (2, 0.3), (3, 0.05), (4, 0.05)], [(1, 1.0)]])
     idx, w = sk.top_k(3)

@@ -1,6 +1,7 @@
 """Characters read from the reference's file formats drive the GPU entry
-points: a skeleton / skin / mesh / character file set through the GPU
-forward kinematics and DQ skinning, and an embedded graph built from a mesh
+points: a skeleton / skin / mesh / character file set (a V1.0 skeleton, or
+a v0.9 one with its .dof file) through the GPU forward kinematics and DQ
+skinning, and an embedded graph built from a mesh
 and its decimated graph mesh through the GPU ED forward, against the oracle."""
 import numpy as np
 import pytest
@@ -10,7 +11,7 @@ from deepcharacters_amd import embedded_graph as edm
 from deepcharacters_amd import formats, scenes, skinning
 from oracle import oracle as orc
 
-from test_formats import _body_files
+from test_formats import _body_files, _body_files09
 
 pytestmark = pytest.mark.gpu
 
@@ -24,8 +25,9 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def test_character_files_through_gpu_fk_and_skinning(tmp_path):
-    _, _, cp, _ = _body_files(tmp_path)
+@pytest.mark.parametrize("layout", ["v1.0", "v0.9+dof v0.5"])
+def test_character_files_through_gpu_fk_and_skinning(tmp_path, layout):
+    cp = _body_files(tmp_path)[2] if layout == "v1.0" else _body_files09(tmp_path, "v0.5")[2]
     ch, desc = formats.load_character(cp)
     sk = skinning.Skeleton(desc)
     B = 2
