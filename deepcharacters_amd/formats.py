@@ -9,7 +9,8 @@ GPU path.
   (Skeletool/Character/skeleton.cpp:205-545 loadSkeleton10b / loadSkeleton10) and
   "Skeleton v0.9" / "v0.10" (skeleton.cpp:533-872 loadSkeleton09), whose DOFs
   ``read_dof`` takes from a .dof file (skeleton.cpp:1679-2095 loadDof / loadDof05)
-* skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData)
+* skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData),
+  ``read_pskin`` (:210-268 loadPinocchioSkinning), ``read_maya_skin`` (:272-362 loadMayaSkinningData)
 * character: ``read_character`` / ``load_character`` (skinnedcharacter.cpp:425-557 loadCharacter)
 * embedded graph: ``build_embedded_graph`` (Skeletool/Mesh/EmbeddedGraph.cpp:5-460)
 
@@ -547,7 +548,7 @@ def last_joint_by_name(skel: SkeletonFile, bone: str) -> int:
     raise ValueError(f"no joint matches bone '{bone}'")
 
 
-def skeleton_desc(skel: SkeletonFile, bones: list[str]) -> dict:
+def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | None = None) -> dict:
     """The kinematic description skinning.Skeleton takes.  Joint i's local
     transform is translate(offset_i * scale_i) then the rotation about (or
     translation along) its axis by its parameter, and the parameter is the
@@ -555,8 +556,9 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str]) -> dict:
     [J, D] holds those weights.  When every joint is driven by at most one DOF
     with weight 1, ``dof`` indexes the DOF vector directly; otherwise the FK
     runs on the J joint parameters ``dofs @ dof_map.T`` (``dof`` = joint id).
-    Skinning joint b is the joint its bone name resolves to, bound at its
-    global transform with all DOFs zero."""
+    Skinning joint b is the joint its bone name resolves to (or
+    ``joint_ids[b]``, the Maya export's joint index), bound at its global
+    transform with all DOFs zero."""
     J, D = len(skel.names), len(skel.dofs)
     dof_map = np.zeros((J, D), np.float32)
     for d, (_, _, infl) in enumerate(skel.dofs):
@@ -588,7 +590,12 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str]) -> dict:
     for j in order:
         p = skel.parents[j]
         gpos[j] = (gpos[p] if p >= 0 else 0.0) + offset[j]
-    skin_joint = np.array([last_joint_by_name(skel, b) for b in bones], np.int32)
+    if joint_ids is not None:
+        if len(joint_ids) != len(bones) or not all(0 <= j < J for j in joint_ids):
+            raise ValueError(f"skinning joint indices {joint_ids} do not fit {J} joints")
+        skin_joint = np.asarray(joint_ids, np.int32)
+    else:
+        skin_joint = np.array([last_joint_by_name(skel, b) for b in bones], np.int32)
     bind = np.zeros((len(bones), 12))
     for b, j in enumerate(skin_joint):
         bind[b] = [1, 0, 0, -gpos[j, 0], 0, 1, 0, -gpos[j, 1], 0, 0, 1, -gpos[j, 2]]
@@ -631,6 +638,7 @@ def _topological(parents):
 class SkinFile:
     bones: list[str]
     influences: list[list[tuple[int, float]]]  # per vertex: (bone index, weight)
+    joint_ids: list[int] | None = None         # Maya export: each bone's skeleton joint index
 
     def top_k(self, K: int = 4, renormalise: bool = True):
         """[N, K] bone indices and weights: each vertex's K largest weights
@@ -666,6 +674,68 @@ def read_skin(path: str) -> SkinFile:
         t = ln.split()
         infl.append([(int(t[j]), float(t[j + 1])) for j in range(1, len(t) - 1, 2)])
     return SkinFile(bones, infl)
+
+
+def read_pskin(path: str) -> SkinFile:
+    """A Pinocchio skinning file (.pskin, skinnedcharacter.cpp:210-268): the
+    .skin header, ``bones:`` and its line of bone names, ``vertex weights:``,
+    then one line per vertex holding a weight for every bone in bone order;
+    weights > 0 are kept.  Empty lines are skipped."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    lines = [ln for ln in lines if ln]
+    if len(lines) < 4 or lines[0] != "Skeletool character skinning file V1.0" or lines[1] != "bones:":
+        raise ValueError(f"{path}: not a skeletool skin file")
+    bones = lines[2].split()
+    if lines[3] != "vertex weights:":
+        raise ValueError(f"{path}: expected 'vertex weights:'")
+    infl = []
+    for ln in lines[4:]:
+        w = [float(x) for x in ln.split()]
+        if len(w) > len(bones):
+            raise ValueError(f"{path}: {len(w)} weights for {len(bones)} bones")
+        infl.append([(j, x) for j, x in enumerate(w) if x > 0.0])
+    return SkinFile(bones, infl)
+
+
+def read_maya_skin(path: str, num_vertices: int) -> SkinFile:
+    """The Maya skinning export (any skin extension other than .skin and
+    .pskin, skinnedcharacter.cpp:272-362): a ``<label> B`` line, B lines ``<joint
+    index> <bone name>`` (each bone binds to the skeleton joint of that index),
+    a ``:``-separated header whose fields after the first name the bone of
+    each weight column, then ``num_vertices`` ``:``-separated lines whose
+    fields after the first are the weights; weights > 0 are kept.  A column
+    naming no bone is an error here (the reference reports it and then
+    misaligns the later columns)."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    lines = [ln for ln in lines if ln]
+    t = lines[0].split()
+    if len(t) != 2:
+        raise ValueError(f"{path}: expected the number of bones")
+    nb = int(t[1])
+    bones, ids = [], []
+    for ln in lines[1:1 + nb]:
+        t = ln.split()
+        if len(t) != 2:
+            raise ValueError(f"{path}: expected '<joint index> <bone name>', got {t}")
+        ids.append(int(t[0]))
+        bones.append(t[1])
+    cols = [c.strip() for c in lines[1 + nb].split(":")][1:]
+    for c in cols:
+        if c not in bones:
+            raise ValueError(f"{path}: weight column '{c}' names no bone")
+    col_bone = [bones.index(c) for c in cols]
+    rows = lines[2 + nb:2 + nb + num_vertices]
+    if len(rows) != num_vertices:
+        raise ValueError(f"{path}: {len(rows)} weight lines for {num_vertices} vertices")
+    infl = []
+    for ln in rows:
+        w = [float(x) for x in ln.split(":")[1:]]
+        if len(w) > len(cols):
+            raise ValueError(f"{path}: {len(w)} weights for {len(cols)} columns")
+        infl.append([(col_bone[j], x) for j, x in enumerate(w) if x > 0.0])
+    return SkinFile(bones, infl, ids)
 
 
 def write_skin(path: str, bones: list[str], idx: np.ndarray, w: np.ndarray) -> None:
@@ -714,8 +784,9 @@ def read_character(path: str) -> dict:
 def load_character(path: str, K: int = 4, ed_graph_mesh: str | None = None):
     """A character from its files: (scenes.Character, skeleton description).
     Skeletons are V1.0, or v0.9 / v0.10 completed by the character's .dof file
-    (skinnedcharacter.cpp:515-518); skinning is a .skin file; the mesh is OFF
-    or OBJ.
+    (skinnedcharacter.cpp:515-518); skinning is chosen by extension as the
+    reference does (:542-547): .skin, .pskin, else the Maya export; the mesh
+    is OFF or OBJ.
     Skinning joints are the skin file's bones in file order.  With
     ``ed_graph_mesh`` (the decimated graph mesh) the embedded graph is built
     as the reference builds it."""
@@ -728,12 +799,16 @@ def load_character(path: str, K: int = 4, ed_graph_mesh: str | None = None):
             raise ValueError(f"{path}: the skeleton needs a .dof file, which only v0.1 / v0.3 characters name")
         skel = read_dof(paths["dof"], skel)
     mesh = read_obj(paths["mesh"]) if paths["mesh"].lower().endswith(".obj") else read_off(paths["mesh"])
-    if not paths["skin"].endswith(".skin"):
-        raise ValueError(f"{path}: only .skin skinning files are supported")
-    skin = read_skin(paths["skin"])
+    ext = os.path.splitext(paths["skin"])[1]
+    if ext == ".skin":
+        skin = read_skin(paths["skin"])
+    elif ext == ".pskin":
+        skin = read_pskin(paths["skin"])
+    else:
+        skin = read_maya_skin(paths["skin"], len(mesh.verts))
     if len(skin.influences) != len(mesh.verts):
         raise ValueError(f"{path}: {len(skin.influences)} skinned vertices for {len(mesh.verts)} mesh vertices")
-    desc = skeleton_desc(skel, skin.bones)
+    desc = skeleton_desc(skel, skin.bones, skin.joint_ids)
     ji, jw = skin.top_k(K)
     verts = mesh.verts.astype(np.float64)
     colors = mesh.colors if mesh.colors is not None else np.full((len(verts), 3), 0.5)

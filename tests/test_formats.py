@@ -300,6 +300,64 @@ def test_driven_scaling_joints_and_missing_dof_files_are_rejected(tmp_path):
         formats.read_dof(str(bad), formats.read_skeleton(str(sp)))
 
 
+def _character_with_skin(tmp_path, skin_name):
+    with open(tmp_path / f"{skin_name}.character", "w") as fh:
+        fh.write(f"skeletool character file v1.0\nskeleton\nbody.skeleton\nmesh\nbody.off\nskin\n{skin_name}\n")
+    return str(tmp_path / f"{skin_name}.character")
+
+
+def test_pinocchio_and_maya_skinning_files_load_the_same_character(tmp_path):
+    skel, ch, cp, _ = _body_files(tmp_path)
+    ref_ch, ref = formats.load_character(cp)
+    bones = ["root_"] + [f"j{b}_" for b in range(1, 18)]
+    dense = np.zeros((ch.N, len(bones)))
+    for v in range(ch.N):
+        for i, w in zip(ch.joint_idx[v], ch.joint_w[v]):
+            dense[v, i] += w
+    with open(tmp_path / "body.pskin", "w") as fh:
+        fh.write("Skeletool character skinning file V1.0\nbones:\n" + " ".join(bones) + "\nvertex weights:\n")
+        for v in range(ch.N):
+            fh.write(" ".join(repr(float(x)) for x in dense[v]) + "\n")
+    # Maya export: bones bound by joint index, weight columns in another order
+    perm = np.random.default_rng(3).permutation(len(bones))
+    with open(tmp_path / "body.weights", "w") as fh:
+        fh.write(f"bones {len(bones)}\n")
+        for b, nm in enumerate(bones):
+            fh.write(f"{int(skel['skin_joint'][b])} {nm}\n")
+        fh.write("vertex:" + ":".join(bones[i] for i in perm) + "\n")
+        for v in range(ch.N):
+            fh.write(f"{v}:" + ":".join(repr(float(dense[v, i])) for i in perm) + "\n")
+    for name in ("body.pskin", "body.weights"):
+        c2, d2 = formats.load_character(_character_with_skin(tmp_path, name))
+        np.testing.assert_array_equal(d2["skin_joint"], ref["skin_joint"], err_msg=name)
+        np.testing.assert_array_equal(c2.joint_idx[c2.joint_w > 0], ref_ch.joint_idx[ref_ch.joint_w > 0])
+        np.testing.assert_allclose(c2.joint_w, ref_ch.joint_w, atol=1e-7)
+
+
+def test_pinocchio_and_maya_skinning_hand_written(tmp_path):
+    p = tmp_path / "s.pskin"
+    p.write_text("Skeletool character skinning file V1.0\nbones:\na b c\nvertex weights:\n"
+                 "0.5 0 0.5\n\n0 1 0\n0 -0.1 1.1\n")
+    sk = formats.read_pskin(str(p))
+    assert sk.bones == ["a", "b", "c"] and sk.joint_ids is None
+    assert sk.influences == [[(0, 0.5), (2, 0.5)], [(1, 1.0)], [(2, 1.1)]]   # weights > 0 only
+    m = tmp_path / "s.weights"
+    m.write_text("joints: 2\n4 hip\n1 spine\nvertex : spine : hip\n0 : 0.25 : 0.75\n1 : 1.0 : 0\n")
+    sk = formats.read_maya_skin(str(m), 2)
+    assert sk.bones == ["hip", "spine"] and sk.joint_ids == [4, 1]
+    assert sk.influences == [[(1, 0.25), (0, 0.75)], [(1, 1.0)]]             # columns mapped by name
+    m.write_text("joints: 1\n4 hip\nvertex:spine\n0:1\n")
+    with pytest.raises(ValueError, match="names no bone"):
+        formats.read_maya_skin(str(m), 1)
+    sf = formats.SkeletonFile(["r", "c"], ["revolute", "revolute"], [-1, 0], np.array([[0, 0, 0], [0, 1, 0]], np.float32),
+                              np.array([[0, 0, 1], [0, 0, 1]], np.float32), np.ones(2, np.float32), [], [], [])
+    d = formats.skeleton_desc(sf, ["hip"], [1])
+    np.testing.assert_array_equal(d["skin_joint"], [1])
+    np.testing.assert_allclose(d["skin_bind"][0, [3, 7, 11]], [0, -1, 0])
+    with pytest.raises(ValueError, match="do not fit"):
+        formats.skeleton_desc(sf, ["hip"], [2])
+
+
 def test_skin_top_k_keeps_the_largest_and_renormalises():
     sk = formats.SkinFile(["a", "b"], [[(0, 0.5), (1, 0.1), (2, 0.3), (3, 0.05), (4, 0.05)], [(1, 1.0)]])
     idx, w = sk.top_k(3)
