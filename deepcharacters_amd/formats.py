@@ -7,7 +7,7 @@ GPU path.
   writeOff / writeCOff) and ``read_obj`` (trimesh.cpp:337-520 loadObj)
 * skeleton: ``read_skeleton`` for "Skeletool Skeleton Definition V1.0"
   (Skeletool/Character/skeleton.cpp:205-545 loadSkeleton10b / loadSkeleton10) and
-  "Skeleton v0.9" / "v0.10" (skeleton.cpp:533-872 loadSkeleton09), whose DOFs
+  "Skeleton v0.9" / "v0.10" / "v0.3" / "v0.1" (skeleton.cpp:533-1354), whose DOFs
   ``read_dof`` takes from a .dof file (skeleton.cpp:1679-2095 loadDof / loadDof05)
 * skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData),
   ``read_pskin`` (:210-268 loadPinocchioSkinning), ``read_maya_skin`` (:272-362 loadMayaSkinningData)
@@ -209,7 +209,7 @@ def read_obj(path: str) -> Mesh:
 
 
 # ----------------------------------------------------------------------------
-# Skeleton (V1.0; v0.9 + .dof), skinning, character
+# Skeleton (V1.0; older layouts + .dof), skinning, character
 # ----------------------------------------------------------------------------
 
 @dataclass
@@ -223,8 +223,8 @@ class SkeletonFile:
     markers: list[tuple]         # (name, parent joint, type, offset [3])
     scaling: list[tuple]         # (name, [joint ids])
     dofs: list[tuple]            # (name, limits or None, [(joint id, weight)])
-    joint_limits: np.ndarray | None = None  # v0.9: [J, 2] per-joint limits (lo == hi: none)
-    dof_file_required: bool = False         # v0.9: the DOFs come from a .dof file (read_dof)
+    joint_limits: np.ndarray | None = None  # pre-V1.0: [J, 2] per-joint limits (lo == hi: none)
+    dof_file_required: bool = False         # pre-V1.0: the DOFs come from a .dof file (read_dof)
 
 
 def _lines(path):
@@ -243,15 +243,18 @@ def read_skeleton(path: str) -> SkeletonFile:
     ``<name> <count>``, ``limits lo hi`` or ``nolimits``, then ``<joint>
     <weight>`` lines.  Joint references in markers, scaling groups and DOFs
     match the first joint whose name starts with the given name (the
-    reference's getJointByName prefix lookup).  "Skeleton v0.9" / "v0.10"
-    files are read by the older layout's reader (no DOFs: see read_dof)."""
+    reference's getJointByName prefix lookup).  "Skeleton v0.9" / "v0.10" /
+    "v0.3" / "v0.1" files go to the older layouts' readers (no DOFs: see
+    read_dof)."""
     it = _lines(path)
     header, _ = next(it)
     if header in ("Skeleton v0.9", "Skeleton v0.10"):
         return _read_skeleton09(path)
+    if header in ("Skeleton v0.3", "Skeleton v0.1"):
+        return _read_skeleton_v01_v03(path, header.split()[1])
     if header != "Skeletool Skeleton Definition V1.0":
         raise ValueError(f"{path}: '{header}' skeletons are not supported "
-                         "(V1.0, and v0.9 / v0.10 with a .dof file, are)")
+                         "(V1.0, and v0.9 / v0.10 / v0.3 / v0.1 with a .dof file, are)")
     _, t = next(it)
     if len(t) != 2 or t[0] != "joints:":
         raise ValueError(f"{path}: expected 'joints:'")
@@ -430,8 +433,112 @@ def _read_skeleton09(path: str) -> SkeletonFile:
                         joint_limits=limits, dof_file_required=True)
 
 
+def _read_skeleton_v01_v03(path: str, version: str) -> SkeletonFile:
+    """"Skeleton v0.3" (skeleton.cpp:874-1019) and "Skeleton v0.1"
+    (:1022-1354) files, whose DOFs also come from a .dof file.  ``joints: J``
+    then J joint lines ``<name> <parent> <r|t|s> ox oy oz ax ay az`` (parent:
+    an earlier joint of that exact name, else the root; v0.3 has no scaling
+    joints, a v0.1 scaling joint has zero offset and no axis).
+
+    - v0.3: offsets are parent-relative; ``proxies: P`` lines ``<parent> ox oy
+      oz size -1 r g b`` (parent by prefix, offset as written, named by index).
+    - v0.1 with limits inline (the first joint line has 11 values): offsets
+      relative, ``lo hi`` after the axis; ``proxies: P`` lines ``<name>
+      <joint index> ox oy oz size -1 h s v`` (the first proxy line's length
+      picks the form for all of them), offsets as written.
+    - v0.1 without: offsets global (made relative to the exact-named parent's
+      global offset; a scaling joint's is zero, placing it at the origin), then ``BEGIN
+      limits`` lines ``<joint> <weight> lo hi`` up to ``End::``; 6-value
+      proxy lines ``<tag> <parent> ox oy oz size`` (exact parent name, global
+      offsets made relative, named by index)."""
+    rows = _token_rows(path)
+    n, i = len(rows), 0
+    while i < n and not rows[i]:
+        i += 1
+    i += 1
+    while i < n and len(rows[i]) < 2:
+        i += 1
+    if i >= n or len(rows[i]) != 2 or rows[i][0] != "joints:":
+        raise ValueError(f"{path}: expected 'joints:'")
+    nj = int(rows[i][1])
+    jrows = rows[i + 1:i + 1 + nj]
+    i += 1 + nj
+    if len(jrows) != nj or nj == 0:
+        raise ValueError(f"{path}: expected {nj} joint lines")
+    inline = version == "v0.1" and len(jrows[0]) == 11
+    need = 11 if inline else 9
+    kinds = {"r": "revolute", "t": "prismatic"} if version == "v0.3" else \
+        {"r": "revolute", "t": "prismatic", "s": "scaling"}
+    global_offsets = version == "v0.1" and not inline
+    names, types, parents, glob, off, ax = [], [], [], [], [], []
+    limits = np.zeros((nj, 2), np.float32)
+    for j, t in enumerate(jrows):
+        if len(t) != need:
+            raise ValueError(f"{path}: joint line {t} has {len(t)} values, expected {need}")
+        if t[2] not in kinds:
+            raise ValueError(f"{path}: unknown joint type '{t[2]}' of joint '{t[0]}'")
+        o = np.asarray([float(x) for x in t[3:6]], np.float32)
+        a = np.asarray([float(x) for x in t[6:9]], np.float32)
+        if t[2] == "r":
+            a = a / np.float32(np.sqrt(np.float32(a @ a)))
+        elif t[2] == "s":
+            o = np.zeros(3, np.float32)
+            a = np.zeros(3, np.float32)
+        p = names.index(t[1]) if t[1] in names else -1
+        names.append(t[0])
+        types.append(kinds[t[2]])
+        parents.append(p)
+        glob.append(o)
+        off.append(o - glob[p] if global_offsets and p >= 0 else o)
+        ax.append(a)
+        if inline:
+            limits[j] = (float(t[9]), float(t[10]))
+    if sum(p < 0 for p in parents) != 1:
+        raise ValueError(f"{path}: the skeleton needs exactly one root joint")
+    if global_offsets:
+        while i < n and len(rows[i]) < 2:
+            i += 1
+        if i >= n or rows[i] != ["BEGIN", "limits"]:
+            raise ValueError(f"{path}: expected 'BEGIN limits'")
+        i += 1
+        while i < n and (not rows[i] or rows[i][0] != "End::"):
+            t = rows[i]
+            i += 1
+            if len(t) >= 4 and t[0] in names:
+                limits[names.index(t[0])] = (float(t[2]), float(t[3]))
+        i += 1
+    while i < n and len(rows[i]) < 2:
+        i += 1
+    if i >= n or len(rows[i]) != 2 or rows[i][0] != "proxies:":
+        raise ValueError(f"{path}: expected 'proxies:'")
+    npx = int(rows[i][1])
+    prows = rows[i + 1:i + 1 + npx]
+    if len(prows) != npx:
+        raise ValueError(f"{path}: expected {npx} proxy lines")
+    markers = []
+    coloured = bool(prows) and len(prows[0]) == 10
+    for k, t in enumerate(prows):
+        want = 9 if version == "v0.3" else (10 if coloured else 6)
+        if len(t) != want:
+            raise ValueError(f"{path}: proxy line {t} has {len(t)} values, expected {want}")
+        if version == "v0.3":
+            o = np.asarray([float(x) for x in t[1:4]], np.float32)
+            markers.append((str(k), _first_prefix(names, t[0]), "proxy", o))
+        elif coloured:
+            pj = int(t[1])
+            markers.append((t[0], pj if 0 <= pj < nj else -1, "proxy",
+                            np.asarray([float(x) for x in t[2:5]], np.float32)))
+        else:
+            pj = names.index(t[1]) if t[1] in names else -1
+            o = np.asarray([float(x) for x in t[2:5]], np.float32)
+            markers.append((str(k), pj, "proxy", o - glob[pj] if pj >= 0 else o))
+    return SkeletonFile(names, types, parents, np.asarray(off, np.float32).reshape(nj, 3),
+                        np.asarray(ax, np.float32).reshape(nj, 3), np.ones(nj, np.float32), markers, [], [],
+                        joint_limits=limits, dof_file_required=True)
+
+
 def read_dof(path: str, skel: SkeletonFile) -> SkeletonFile:
-    """The DOFs of a v0.9 skeleton from its .dof file (skeleton.cpp:1679-2095),
+    """The DOFs of a pre-V1.0 skeleton from its .dof file (skeleton.cpp:1679-2095),
     as a copy of ``skel`` with ``dofs`` filled in.
 
     - "Dofs v0.3": ``dofs: D`` then ``<name> <value> limits lo hi`` or
@@ -572,7 +679,7 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | N
     else:
         dof = np.arange(J, dtype=np.int32)
         num_dofs = J
-    # a scaling joint (v0.9 type "s", a 2^p scaling) is the identity at p = 0:
+    # a scaling joint (type "s" of the pre-V1.0 layouts, a 2^p scaling) is the identity at p = 0:
     # undriven, it is a prismatic joint with no axis; driven, the FK cannot
     # represent it
     for j, t in enumerate(skel.types):

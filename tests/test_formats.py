@@ -140,20 +140,29 @@ def test_character_file_loads_mesh_skeleton_and_skin(tmp_path):
                                atol=1e-7)
 
 
-def _write_skel09(path, desc):
-    """The kinematic description as a v0.9 skeleton: global joint offsets,
-    parents before children, no limits or proxies."""
+def _write_skel_old(path, desc, layout="v0.9"):
+    """The kinematic description as a pre-V1.0 skeleton, parents before
+    children and no proxies: "v0.9" and "v0.1g" with global offsets (and an
+    empty limits block), "v0.3" with relative ones, "v0.1" with relative ones
+    and limits -1 1 inline."""
     names, par = desc["names"], [int(p) for p in desc["parent"]]
     g = np.zeros((len(names), 3))
+    head = {"v0.9": "Skeleton v0.9\n\nBEGIN joints global\n", "v0.3": "Skeleton v0.3\njoints: %d\n",
+            "v0.1": "Skeleton v0.1\njoints: %d\n", "v0.1g": "Skeleton v0.1\n\njoints: %d\n"}[layout]
+    tail = {"v0.9": "\nBEGIN limits\n\nBEGIN proxies global level 0\n", "v0.3": "proxies: 0\n",
+            "v0.1": "\nproxies: 0\n", "v0.1g": "BEGIN limits\nEnd::\nproxies: 0\n"}[layout]
     with open(path, "w") as fh:
-        fh.write("Skeleton v0.9\n\nBEGIN joints global\n")
+        fh.write(head.replace("%d", str(len(names))))
         for j, nm in enumerate(names):
             assert par[j] < j
             g[j] = (g[par[j]] if par[j] >= 0 else 0.0) + desc["offset"][j]
+            o = g[j] if layout in ("v0.9", "v0.1g") else desc["offset"][j]
             t = "r" if int(desc["type"][j]) == scenes.JOINT_REVOLUTE else "t"
-            fh.write(f"{nm} {names[par[j]] if par[j] >= 0 else 'none'} {t} "
-                     + " ".join(repr(float(x)) for x in (*g[j], *desc["axis"][j])) + " 0\n")
-        fh.write("\nBEGIN limits\n\nBEGIN proxies global level 0\n")
+            pn = names[par[j]] if par[j] >= 0 else "none"
+            vals = " ".join(repr(float(x)) for x in (*o, *desc["axis"][j]))
+            line = f"{nm} {pn} {t} {vals}" if layout != "v0.9" else f"{nm} {pn} {t} {vals} 0"
+            fh.write(line + (" -1 1\n" if layout == "v0.1" else "\n"))
+        fh.write(tail)
 
 
 def _write_dof(path, desc, version):
@@ -172,11 +181,11 @@ def _write_dof(path, desc, version):
         fh.write(f"\njoints: {len(rows)}\n" + "\n".join(rows) + "\n")
 
 
-def _body_files09(tmp_path, version="v0.3"):
-    """The body character as a v0.3 character file naming a v0.9 skeleton and
-    a .dof file of the given version."""
+def _body_files09(tmp_path, version="v0.3", layout="v0.9"):
+    """The body character as a v0.3 character file naming a pre-V1.0 skeleton
+    and a .dof file of the given version."""
     skel, ch, _, _ = _body_files(tmp_path)
-    _write_skel09(str(tmp_path / "body09.skeleton"), skel)
+    _write_skel_old(str(tmp_path / "body09.skeleton"), skel, layout)
     _write_dof(str(tmp_path / "body.dof"), skel, version)
     cp = str(tmp_path / "body09.character")
     with open(cp, "w") as fh:
@@ -185,13 +194,16 @@ def _body_files09(tmp_path, version="v0.3"):
     return skel, ch, cp
 
 
-@pytest.mark.parametrize("version", ["v0.1", "v0.3", "v0.5"])
-def test_v09_skeleton_with_dof_file_matches_the_v10_file(tmp_path, version):
-    skel, ch, cp = _body_files09(tmp_path, version)
+@pytest.mark.parametrize("layout,version", [("v0.9", "v0.1"), ("v0.9", "v0.3"), ("v0.9", "v0.5"),
+                                            ("v0.3", "v0.3"), ("v0.1", "v0.1"), ("v0.1g", "v0.1")])
+def test_older_skeletons_with_dof_files_match_the_v10_file(tmp_path, layout, version):
+    skel, ch, cp = _body_files09(tmp_path, version, layout)
     sf = formats.read_skeleton(str(tmp_path / "body09.skeleton"))
     assert sf.dof_file_required and not sf.dofs and len(sf.names) == 57
     sf = formats.read_dof(str(tmp_path / "body.dof"), sf)
     assert len(sf.dofs) == 57 and all(len(d[2]) == 1 and d[2][0][1] == 1.0 for d in sf.dofs)
+    # the inline v0.1 limits reach each DOF through the joint it drives
+    assert all(d[1] == ((-1.0, 1.0) if layout == "v0.1" else None) for d in sf.dofs)
     bones = ["root_"] + [f"j{b}_" for b in range(1, 18)]
     ref = formats.skeleton_desc(formats.read_skeleton(str(tmp_path / "body.skeleton")), bones)
     desc = formats.skeleton_desc(sf, bones)
@@ -278,6 +290,40 @@ def test_dof_file_weights_limits_and_unknown_names(tmp_path, version):
     # knee: 1 along x from the hip, turned 0.8 rad about z by the hip
     np.testing.assert_allclose(jp[0, 2], [0.3 + np.cos(0.8), 1 + np.sin(0.8), 0], atol=1e-6)
     np.testing.assert_allclose(jp[0, 3], jp[0, 2], atol=1e-12)
+
+
+def test_v03_and_v01_skeleton_layouts_hand_written(tmp_path):
+    p = tmp_path / "a.skeleton"
+    p.write_text("Skeleton v0.3\njoints: 2\nroot none t 0 0 0 2 0 0\narm root r 0 1 0 0 0 3\n"
+                 "proxies: 1\nar 0.1 0.2 0.3 0.05 -1 0 0 0\n")
+    sf = formats.read_skeleton(str(p))
+    assert sf.dof_file_required and sf.parents == [-1, 0] and sf.markers[0][:3] == ("0", 1, "proxy")  # parent by prefix
+    np.testing.assert_array_equal(sf.axes, [[2, 0, 0], [0, 0, 1]])       # prismatic kept, revolute normalised
+    np.testing.assert_allclose(sf.markers[0][3], [0.1, 0.2, 0.3])           # relative, as written
+    # v0.1, global offsets: a scaling joint sits at the origin (its offset is
+    # forced to zero before being made relative), so its children keep theirs
+    p.write_text("Skeleton v0.1\njoints: 4\nroot none t 1 0 0 1 0 0\nsc root s 5 5 5 1 1 1\n"
+                 "arm sc r 1 2 0 0 0 1\nhand arm r 1 3 0 0 0 1\nBEGIN limits\narm 1 -2 2\nnope 1 0 1\nEnd::\n"
+                 "proxies: 2\nblob hand 1 3 1 0.05\nM002 nosuch 0 0 1 0.05\nBEGIN textures color level 0\n")
+    sf = formats.read_skeleton(str(p))
+    assert sf.types == ["prismatic", "scaling", "revolute", "revolute"]
+    np.testing.assert_array_equal(sf.offsets, [[1, 0, 0], [-1, 0, 0], [1, 2, 0], [0, 1, 0]])
+    np.testing.assert_array_equal(sf.joint_limits[:, 0], [0, 0, -2, 0])
+    (n0, p0, _, o0), (n1, p1, _, o1) = sf.markers
+    assert (n0, p0, n1, p1) == ("0", 3, "1", -1)
+    np.testing.assert_array_equal(o0, [0, 0, 1])                            # minus hand's global offset
+    np.testing.assert_array_equal(o1, [0, 0, 1])
+    # v0.1 with inline limits and coloured proxies (joint index parents)
+    p.write_text("Skeleton v0.1\njoints: 2\nroot none t 0 0 0 1 0 0 -1 1\narm root r 0 1 0 0 0 1 0 0\n"
+                 "proxies: 1\nm7 1 0.5 0 0 0.05 -1 0.1 0.2 0.3\n")
+    sf = formats.read_skeleton(str(p))
+    np.testing.assert_array_equal(sf.offsets, [[0, 0, 0], [0, 1, 0]])
+    np.testing.assert_array_equal(sf.joint_limits, [[-1, 1], [0, 0]])
+    assert sf.markers[0][:3] == ("m7", 1, "proxy")
+    dp = tmp_path / "a.dof"
+    dp.write_text("Dofs v0.1\ndofs: 2\nslide 0\nturn 0\njoints: 2\nroot = 0 + 1 slide\narm = 0 + 1 turn\n")
+    d = formats.read_dof(str(dp), sf).dofs
+    assert d == [("slide", (-1.0, 1.0), [(0, 1.0)]), ("turn", None, [(1, 1.0)])]
 
 
 def test_driven_scaling_joints_and_missing_dof_files_are_rejected(tmp_path):
