@@ -8,7 +8,8 @@ GPU path.
 * skeleton: ``read_skeleton`` for "Skeletool Skeleton Definition V1.0"
   (Skeletool/Character/skeleton.cpp:205-545 loadSkeleton10b / loadSkeleton10) and
   "Skeleton v0.9" / "v0.10" / "v0.3" / "v0.1" (skeleton.cpp:533-1354), whose DOFs
-  ``read_dof`` takes from a .dof file (skeleton.cpp:1679-2095 loadDof / loadDof05)
+  ``read_dof`` takes from a .dof file (skeleton.cpp:1679-2095 loadDof / loadDof05),
+  Pinocchio .pskel (:1357-1455) and BVH hierarchies (:1459-1676)
 * skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData),
   ``read_pskin`` (:210-268 loadPinocchioSkinning), ``read_maya_skin`` (:272-362 loadMayaSkinningData)
 * character: ``read_character`` / ``load_character`` (skinnedcharacter.cpp:425-557 loadCharacter)
@@ -245,7 +246,13 @@ def read_skeleton(path: str) -> SkeletonFile:
     match the first joint whose name starts with the given name (the
     reference's getJointByName prefix lookup).  "Skeleton v0.9" / "v0.10" /
     "v0.3" / "v0.1" files go to the older layouts' readers (no DOFs: see
-    read_dof)."""
+    read_dof); .pskel (Pinocchio) and .bvh files are told by their extension
+    first, as skeleton.cpp:2097-2130 does."""
+    ext = os.path.splitext(path)[1]
+    if ext == ".pskel":
+        return _read_skeleton_pinocchio(path)
+    if ext == ".bvh":
+        return _read_skeleton_bvh(path)
     it = _lines(path)
     header, _ = next(it)
     if header in ("Skeleton v0.9", "Skeleton v0.10"):
@@ -535,6 +542,140 @@ def _read_skeleton_v01_v03(path: str, version: str) -> SkeletonFile:
     return SkeletonFile(names, types, parents, np.asarray(off, np.float32).reshape(nj, 3),
                         np.asarray(ax, np.float32).reshape(nj, 3), np.ones(nj, np.float32), markers, [], [],
                         joint_limits=limits, dof_file_required=True)
+
+
+def _read_skeleton_pinocchio(path: str) -> SkeletonFile:
+    """A Pinocchio skeleton (.pskel, skeleton.cpp:1357-1455): one line per
+    joint, ``<name> ax ay az x y z <parent>``, all revolute (axes
+    normalised).  The parent is the earlier joint of that exact name (none:
+    the root); the position is global with its last two components swapped,
+    (x, z, y), and is made relative to the position of the joint whose index
+    the parent field reads as.  Each joint gets its own DOF (weight 1, no
+    limits)."""
+    rows = [t for t in _token_rows(path) if t]
+    names, parents, off, ax, glob, dofs = [], [], [], [], [], []
+    for t in rows:
+        if len(t) != 8:
+            raise ValueError(f"{path}: joint line {t} has {len(t)} values, expected 8")
+        a = np.asarray([float(x) for x in t[1:4]], np.float32)
+        a = a / np.float32(np.sqrt(np.float32(a @ a)))
+        g = np.asarray([float(t[4]), float(t[6]), float(t[5])], np.float32)
+        try:
+            ip = int(t[7])
+        except ValueError:
+            raise ValueError(f"{path}: parent field '{t[7]}' is not a joint index") from None
+        p = names.index(t[7]) if t[7] in names else -1
+        glob.append(g)
+        off.append(g - glob[ip] if 0 <= ip < len(glob) else g)
+        dofs.append((t[0], None, [(len(names), 1.0)]))
+        names.append(t[0])
+        parents.append(p)
+        ax.append(a)
+    J = len(names)
+    if J == 0 or sum(p < 0 for p in parents) != 1:
+        raise ValueError(f"{path}: the skeleton needs exactly one root joint")
+    return SkeletonFile(names, ["revolute"] * J, parents, np.asarray(off, np.float32).reshape(J, 3),
+                        np.asarray(ax, np.float32).reshape(J, 3), np.ones(J, np.float32), [], [], dofs)
+
+
+def _bone_name(joint_name: str) -> str:
+    k = joint_name.rfind("_")
+    return joint_name if k < 0 else joint_name[:k]
+
+
+_BVH_CHANNELS = (("Xposition", "_tx", "prismatic", (1, 0, 0)), ("Yposition", "_ty", "prismatic", (0, 1, 0)),
+                 ("Zposition", "_tz", "prismatic", (0, 0, 1)), ("Xrotation", "_rx", "revolute", (1, 0, 0)),
+                 ("Yrotation", "_ry", "revolute", (0, 1, 0)), ("Zrotation", "_rz", "revolute", (0, 0, 1)))
+
+
+def _read_skeleton_bvh(path: str) -> SkeletonFile:
+    """The HIERARCHY part of a BVH file (skeleton.cpp:1459-1676).  Every
+    channel of a ROOT (bone name "Root") or JOINT becomes a joint named
+    ``<bone>_tx|ty|tz|rx|ry|rz`` (prismatic / revolute about the unit axis),
+    chained in channel order under the previous joint, with a DOF of its own
+    (weight 1, no limits).  A channel joint's offset is the bone's OFFSET
+    minus the offset of its parent joint, or, when that parent's own parents
+    are channels of the same bone, of the first of them; the root's first
+    channel keeps the OFFSET.  An End Site is a revolute joint
+    ``<parent bone>_end`` about x with the OFFSET as written and no DOF.
+    Each closing brace pops the innermost bone's channel chain.  Bone names
+    are joint names up to their last '_'.  The MOTION part is not read."""
+    rows = [t for t in _token_rows(path) if t]
+    if not rows or rows[0] != ["HIERARCHY"]:
+        raise ValueError(f"{path}: expected 'HIERARCHY'")
+    names, types, parents, off, ax, dofs = [], [], [], [], [], []
+    stack = []
+    i, n = 1, len(rows)
+
+    def row(k, what):
+        if k >= n:
+            raise ValueError(f"{path}: ended before {what}")
+        return rows[k]
+
+    t = row(i, "MOTION")
+    while "MOTION" not in t[0]:
+        is_root, is_joint, is_end = "ROOT" in t[0], "JOINT" in t[0], "End" in t[0]
+        if not (is_root or is_joint or is_end) or (is_joint and len(t) < 2):
+            raise ValueError(f"{path}: expected ROOT, JOINT or End Site, got {t}")
+        bone = "Root" if is_root else (t[1] if is_joint else "End")
+        i += 1
+        if "{" not in row(i, "'{'")[0]:
+            raise ValueError(f"{path}: expected '{{' after {t}")
+        i += 1
+        o = row(i, "OFFSET")
+        if "OFFSET" not in o[0] or len(o) < 4:
+            raise ValueError(f"{path}: expected 'OFFSET x y z', got {o}")
+        offset = np.asarray([float(x) for x in o[1:4]], np.float32)
+        if is_end:
+            if not stack:
+                raise ValueError(f"{path}: End Site outside a joint")
+            p = stack[-1]
+            names.append(_bone_name(names[p]) + "_end")
+            types.append("revolute")
+            parents.append(p)
+            off.append(offset)
+            ax.append(np.asarray((1, 0, 0), np.float32))
+            i += 1
+            if "}" not in row(i, "'}'")[0]:
+                raise ValueError(f"{path}: expected '}}' closing an End Site")
+        else:
+            i += 1
+            c = row(i, "CHANNELS")
+            if "CHANNELS" not in c[0] or len(c) < 2 or len(c) < int(c[1]) + 2:
+                raise ValueError(f"{path}: expected 'CHANNELS n ...', got {c}")
+            for ch in c[2:2 + int(c[1])]:
+                spec = next((s for s in _BVH_CHANNELS if s[0] in ch), None)
+                if spec is None:
+                    raise ValueError(f"{path}: unknown channel '{ch}'")
+                jn = bone + spec[1]
+                if not stack:
+                    p, rel = -1, offset
+                else:
+                    p = stack[-1]
+                    father, fj = off[p], parents[p]
+                    while fj >= 0 and _bone_name(names[fj]) == _bone_name(jn):
+                        father, fj = off[fj], parents[fj]
+                    rel = offset - father
+                dofs.append((jn, None, [(len(names), 1.0)]))
+                stack.append(len(names))
+                names.append(jn)
+                types.append(spec[2])
+                parents.append(p)
+                off.append(rel)
+                ax.append(np.asarray(spec[3], np.float32))
+        i += 1
+        while i < n and "}" in rows[i][0]:
+            if stack:
+                last = _bone_name(names[stack.pop()])
+                while stack and _bone_name(names[stack[-1]]) == last:
+                    stack.pop()
+            i += 1
+        t = row(i, "MOTION")
+    J = len(names)
+    if J == 0 or sum(p < 0 for p in parents) != 1:
+        raise ValueError(f"{path}: the skeleton needs exactly one root joint")
+    return SkeletonFile(names, types, parents, np.asarray(off, np.float32).reshape(J, 3),
+                        np.asarray(ax, np.float32).reshape(J, 3), np.ones(J, np.float32), [], [], dofs)
 
 
 def read_dof(path: str, skel: SkeletonFile) -> SkeletonFile:

@@ -326,6 +326,75 @@ def test_v03_and_v01_skeleton_layouts_hand_written(tmp_path):
     assert d == [("slide", (-1.0, 1.0), [(0, 1.0)]), ("turn", None, [(1, 1.0)])]
 
 
+_BVH = """HIERARCHY
+ROOT Hips
+{
+\tOFFSET 1 2 3
+\tCHANNELS 6 Xposition Yposition Zposition Zrotation Xrotation Yrotation
+\tJOINT Spine
+\t{
+\t\tOFFSET 0 10 0
+\t\tCHANNELS 3 Zrotation Xrotation Yrotation
+\t\tJOINT Arm
+\t\t{
+\t\t\tOFFSET 5 0 0
+\t\t\tCHANNELS 1 Zrotation
+\t\t\tEnd Site
+\t\t\t{
+\t\t\t\tOFFSET 4 0 0
+\t\t\t}
+\t\t}
+\t}
+\tJOINT Leg
+\t{
+\t\tOFFSET 0 -8 0
+\t\tCHANNELS 1 Xrotation
+\t}
+}
+MOTION
+Frames: 1
+Frame Time: 0.033
+0 0 0 0 0 0 0 0 0 0 0
+"""
+
+
+def test_bvh_hierarchy_channel_joints_and_kinematics(tmp_path):
+    p = tmp_path / "a.bvh"
+    p.write_text(_BVH)
+    sf = formats.read_skeleton(str(p))
+    root = ["Root_tx", "Root_ty", "Root_tz", "Root_rz", "Root_rx", "Root_ry"]
+    assert sf.names == root + ["Spine_rz", "Spine_rx", "Spine_ry", "Arm_rz", "Arm_end", "Leg_rx"]
+    # channels chain in order; the braces pop Arm and Spine, so Leg hangs off the root's last channel
+    assert sf.parents == [-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 5]
+    exp = np.zeros((12, 3))
+    exp[0], exp[6], exp[9], exp[10], exp[11] = (1, 2, 3), (0, 10, 0), (5, 0, 0), (4, 0, 0), (0, -8, 0)
+    np.testing.assert_array_equal(sf.offsets, exp)
+    assert sf.types[:3] == ["prismatic"] * 3 and sf.types[3:] == ["revolute"] * 9
+    np.testing.assert_array_equal(sf.axes[[0, 3, 4, 5, 10]], [[1, 0, 0], [0, 0, 1], [1, 0, 0], [0, 1, 0], [1, 0, 0]])
+    assert [d[0] for d in sf.dofs] == [nm for nm in sf.names if nm != "Arm_end"]
+    desc = formats.skeleton_desc(sf, ["Arm_rz"])
+    assert not desc["joint_params"] and desc["dof"][10] == -1 and desc["num_dofs"] == 11
+    th = 0.7
+    d = np.zeros((1, 11))
+    d[0, 0], d[0, 6] = 0.5, th                                     # Root_tx, Spine_rz
+    _, jp, _, _ = orc.fk(desc, d, transforms=True)
+    c, s_ = np.cos(th), np.sin(th)
+    np.testing.assert_allclose(jp[0, 9], [1.5 + 5 * c, 12 + 5 * s_, 3], atol=1e-5)
+    np.testing.assert_allclose(jp[0, 10], [1.5 + 9 * c, 12 + 9 * s_, 3], atol=1e-5)
+    np.testing.assert_allclose(jp[0, 11], [1.5, -6, 3], atol=1e-5)
+
+
+def test_pinocchio_skeleton_swaps_y_and_z_and_makes_offsets_relative(tmp_path):
+    p = tmp_path / "a.pskel"
+    p.write_text("0 0 0 2 1 2 3 -1\n1 3 0 0 1 5 3 0\n\n2 0 2 0 4 5 3 1\n")
+    sf = formats.read_skeleton(str(p))
+    assert sf.names == ["0", "1", "2"] and sf.parents == [-1, 0, 1] and sf.types == ["revolute"] * 3
+    np.testing.assert_array_equal(sf.offsets, [[1, 3, 2], [0, 0, 3], [3, 0, 0]])   # (x, z, y), relative
+    np.testing.assert_array_equal(sf.axes, [[0, 0, 1], [1, 0, 0], [0, 1, 0]])
+    assert sf.dofs == [("0", None, [(0, 1.0)]), ("1", None, [(1, 1.0)]), ("2", None, [(2, 1.0)])]
+    assert not sf.dof_file_required
+
+
 def test_driven_scaling_joints_and_missing_dof_files_are_rejected(tmp_path):
     sp = tmp_path / "s.skeleton"
     sp.write_text(_SKEL09)
