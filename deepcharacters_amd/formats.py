@@ -12,6 +12,7 @@ GPU path.
   Pinocchio .pskel (:1357-1455) and BVH hierarchies (:1459-1676)
 * skinning: ``read_skin`` (Skeletool/Character/skinnedcharacter.cpp:154-206 loadSkinningData),
   ``read_pskin`` (:210-268 loadPinocchioSkinning), ``read_maya_skin`` (:272-362 loadMayaSkinningData)
+* motion: ``read_motion`` (Skeletool/Parameters/parameter_container.cpp:129-190,330-404)
 * character: ``read_character`` / ``load_character`` (skinnedcharacter.cpp:425-557 loadCharacter)
 * embedded graph: ``build_embedded_graph`` (Skeletool/Mesh/EmbeddedGraph.cpp:5-460)
 
@@ -787,6 +788,56 @@ def read_dof(path: str, skel: SkeletonFile) -> SkeletonFile:
     return dataclasses.replace(skel, dofs=dofs, dof_file_required=False)
 
 
+_DEG2RAD = np.float32(np.float32(np.pi) / np.float32(180.0))
+
+
+def read_motion(path: str, skel: SkeletonFile):
+    """Per-frame DOF vectors of a motion file (parameter_container.cpp:129-190,
+    330-404): (frame numbers [F], DOFs [F, D] float32) for a skeleton with D
+    DOFs.  "Skeletool Motion File V1.0" has lines ``<frame> p_0 ... p_D-1``
+    (extra values ignored); "V0.1" the same with the sign of every DOF after
+    the first three flipped; a BVH file (header HIERARCHY) has ``Frames: F``,
+    ``Frame Time: t`` and F lines of D values after its MOTION line, in
+    degrees for every DOF whose first joint is not prismatic (converted with
+    the single-precision pi / 180).  .proj motion files are not read."""
+    if os.path.splitext(path)[1] == ".proj":
+        raise ValueError(f"{path}: .proj motion files are not supported")
+    D = len(skel.dofs)
+    rows = [t for t in _token_rows(path) if t]
+    if not rows:
+        raise ValueError(f"{path}: empty motion file")
+    header = " ".join(rows[0])
+    frames, params = [], []
+    if header in ("Skeletool Motion File V1.0", "Skeletool Motion File V0.1"):
+        for t in rows[1:]:
+            if len(t) < D + 1:
+                raise ValueError(f"{path}: {len(t) - 1} parameters in a frame line, the skeleton has {D}")
+            p = np.asarray([float(x) for x in t[1:D + 1]], np.float32)
+            if header.endswith("V0.1"):
+                p[3:] = -p[3:]
+            frames.append(int(t[0]))
+            params.append(p)
+    elif header == "HIERARCHY":
+        i = next((k for k, t in enumerate(rows) if "MOTION" in t[0]), None)
+        if i is None or i + 2 >= len(rows) or rows[i + 1][0] != "Frames:" or rows[i + 2][:2] != ["Frame", "Time:"]:
+            raise ValueError(f"{path}: expected MOTION, 'Frames: F' and 'Frame Time: t'")
+        nf = int(rows[i + 1][1])
+        deg = np.array([skel.types[d[2][0][0]] != "prismatic" if d[2] else True for d in skel.dofs])
+        lines = rows[i + 3:i + 3 + nf]
+        if len(lines) != nf:
+            raise ValueError(f"{path}: {len(lines)} frame lines, expected {nf}")
+        for f, t in enumerate(lines):
+            if len(t) != D:
+                raise ValueError(f"{path}: {len(t)} values in a frame line, the skeleton has {D} DOFs")
+            p = np.asarray([float(x) for x in t], np.float32)
+            p[deg] = p[deg] * _DEG2RAD
+            frames.append(f)
+            params.append(p)
+    else:
+        raise ValueError(f"{path}: not a skeletool motion file or BVH")
+    return np.asarray(frames, np.int64), np.asarray(params, np.float32).reshape(len(params), D)
+
+
 def last_joint_by_name(skel: SkeletonFile, bone: str) -> int:
     """The last joint whose name starts with ``bone`` (skeleton.cpp:3667-3675),
     the joint a skinning bone binds to."""
@@ -796,7 +847,28 @@ def last_joint_by_name(skel: SkeletonFile, bone: str) -> int:
     raise ValueError(f"no joint matches bone '{bone}'")
 
 
-def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | None = None) -> dict:
+def _global_transforms(parents, types, offset, axis, params) -> np.ndarray:
+    """[J, 4, 4] joint transforms at joint parameters ``params`` [J]: G_j =
+    G_parent translate(offset_j) M_j(p_j), M a rotation by p about the unit
+    axis (Rodrigues) or a translation by p along it."""
+    J = len(parents)
+    G = np.zeros((J, 4, 4))
+    for j in _topological(list(parents)):
+        L = np.eye(4)
+        L[:3, 3] = offset[j]
+        M = np.eye(4)
+        a, t = np.asarray(axis[j], np.float64), float(params[j])
+        if int(types[j]) == scenes.JOINT_REVOLUTE:
+            K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+            M[:3, :3] = np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * (K @ K)
+        else:
+            M[:3, 3] = a * t
+        G[j] = (G[parents[j]] if parents[j] >= 0 else np.eye(4)) @ L @ M
+    return G
+
+
+def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | None = None,
+                  bind_dofs: np.ndarray | None = None) -> dict:
     """The kinematic description skinning.Skeleton takes.  Joint i's local
     transform is translate(offset_i * scale_i) then the rotation about (or
     translation along) its axis by its parameter, and the parameter is the
@@ -806,7 +878,9 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | N
     runs on the J joint parameters ``dofs @ dof_map.T`` (``dof`` = joint id).
     Skinning joint b is the joint its bone name resolves to (or
     ``joint_ids[b]``, the Maya export's joint index), bound at its global
-    transform with all DOFs zero."""
+    transform with all DOFs zero, or at the DOF vector ``bind_dofs`` (a
+    character's initial pose, which the reference applies before it reads
+    the skinning, skinnedcharacter.cpp:519-547)."""
     J, D = len(skel.names), len(skel.dofs)
     dof_map = np.zeros((J, D), np.float32)
     for d, (_, _, infl) in enumerate(skel.dofs):
@@ -838,6 +912,13 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | N
     for j in order:
         p = skel.parents[j]
         gpos[j] = (gpos[p] if p >= 0 else 0.0) + offset[j]
+    G = None
+    if bind_dofs is not None:
+        bind_dofs = np.asarray(bind_dofs, np.float64).reshape(-1)
+        if bind_dofs.shape[0] != D:
+            raise ValueError(f"bind pose has {bind_dofs.shape[0]} DOFs, the skeleton {D}")
+        G = _global_transforms(skel.parents, typ, offset, axes, dof_map.astype(np.float64) @ bind_dofs)
+        gpos = G[:, :3, 3].copy()
     if joint_ids is not None:
         if len(joint_ids) != len(bones) or not all(0 <= j < J for j in joint_ids):
             raise ValueError(f"skinning joint indices {joint_ids} do not fit {J} joints")
@@ -846,7 +927,10 @@ def skeleton_desc(skel: SkeletonFile, bones: list[str], joint_ids: list[int] | N
         skin_joint = np.array([last_joint_by_name(skel, b) for b in bones], np.int32)
     bind = np.zeros((len(bones), 12))
     for b, j in enumerate(skin_joint):
-        bind[b] = [1, 0, 0, -gpos[j, 0], 0, 1, 0, -gpos[j, 1], 0, 0, 1, -gpos[j, 2]]
+        if G is None:
+            bind[b] = [1, 0, 0, -gpos[j, 0], 0, 1, 0, -gpos[j, 1], 0, 0, 1, -gpos[j, 2]]
+        else:
+            bind[b] = np.linalg.inv(G[j])[:3].reshape(-1)
     return dict(parent=np.asarray(skel.parents, np.int32), type=typ, dof=dof, offset=offset,
                 axis=axes, skin_joint=skin_joint, skin_scale=np.ones(len(bones)),
                 skin_bind=bind, num_dofs=num_dofs, names=list(skel.names), dof_map=dof_map,
@@ -1034,7 +1118,8 @@ def load_character(path: str, K: int = 4, ed_graph_mesh: str | None = None):
     Skeletons are V1.0, or v0.9 / v0.10 completed by the character's .dof file
     (skinnedcharacter.cpp:515-518); skinning is chosen by extension as the
     reference does (:542-547): .skin, .pskin, else the Maya export; the mesh
-    is OFF or OBJ.
+    is OFF or OBJ.  A ``pose`` file's frame 0 is the pose the mesh is bound
+    in.
     Skinning joints are the skin file's bones in file order.  With
     ``ed_graph_mesh`` (the decimated graph mesh) the embedded graph is built
     as the reference builds it."""
@@ -1056,7 +1141,13 @@ def load_character(path: str, K: int = 4, ed_graph_mesh: str | None = None):
         skin = read_maya_skin(paths["skin"], len(mesh.verts))
     if len(skin.influences) != len(mesh.verts):
         raise ValueError(f"{path}: {len(skin.influences)} skinned vertices for {len(mesh.verts)} mesh vertices")
-    desc = skeleton_desc(skel, skin.bones, skin.joint_ids)
+    bind = None
+    if "pose" in paths:
+        frames, poses = read_motion(paths["pose"], skel)
+        if not (frames == 0).any():
+            raise ValueError(f"{paths['pose']}: no frame 0 for the initial pose")
+        bind = poses[int(np.flatnonzero(frames == 0)[-1])]
+    desc = skeleton_desc(skel, skin.bones, skin.joint_ids, bind)
     ji, jw = skin.top_k(K)
     verts = mesh.verts.astype(np.float64)
     colors = mesh.colors if mesh.colors is not None else np.full((len(verts), 3), 0.5)

@@ -395,6 +395,66 @@ def test_pinocchio_skeleton_swaps_y_and_z_and_makes_offsets_relative(tmp_path):
     assert not sf.dof_file_required
 
 
+def test_motion_files_v10_v01_and_bvh(tmp_path):
+    sf = formats.read_skeleton(str(_write(tmp_path / "a.bvh", _BVH)))
+    m = _write(tmp_path / "a.motion", "Skeletool Motion File V1.0\n0 " + " ".join(str(k) for k in range(11))
+               + "\n\n7 " + " ".join("0.5" for _ in range(12)) + "\n")
+    fr, P = formats.read_motion(str(m), sf)
+    assert list(fr) == [0, 7] and P.shape == (2, 11) and P.dtype == np.float32
+    np.testing.assert_array_equal(P[0], np.arange(11))
+    m.write_text(m.read_text().replace("V1.0", "V0.1"))
+    _, P1 = formats.read_motion(str(m), sf)
+    np.testing.assert_array_equal(P1[0], [0, 1, 2] + [-k for k in range(3, 11)])   # rotations flip sign
+    # BVH motion: degrees for DOFs of revolute joints, translations as written
+    b = _write(tmp_path / "b.bvh", _BVH.replace("0 0 0 0 0 0 0 0 0 0 0", "1 2 3 90 180 45 30 0 0 60 -90"))
+    fr, P = formats.read_motion(str(b), sf)
+    assert list(fr) == [0]
+    d2r = np.float32(np.float32(np.pi) / np.float32(180))
+    np.testing.assert_array_equal(P[0, :3], [1, 2, 3])
+    np.testing.assert_array_equal(P[0, 3:], np.float32([90, 180, 45, 30, 0, 0, 60, -90]) * d2r)
+    np.testing.assert_allclose(P[0, 3], np.pi / 2, rtol=1e-7)
+    with pytest.raises(ValueError, match="parameters in a frame line"):
+        formats.read_motion(str(_write(tmp_path / "c.motion", "Skeletool Motion File V1.0\n0 1 2\n")), sf)
+
+
+def test_initial_pose_file_sets_the_skinning_bind(tmp_path):
+    skel, ch, cp, _ = _body_files(tmp_path)
+    sf = formats.read_skeleton(str(tmp_path / "body.skeleton"))
+    bones = ["root_"] + [f"j{b}_" for b in range(1, 18)]
+    init = scenes.random_dofs(np.random.default_rng(9), skel, 1, 0.5, 0.03)[0]
+    _write(tmp_path / "init.motion", "Skeletool Motion File V1.0\n0 " + " ".join(repr(float(x)) for x in init) + "\n")
+    with open(cp, "a") as fh:
+        fh.write("pose\ninit.motion\n")
+    ch2, desc = formats.load_character(cp)
+    ref = formats.skeleton_desc(sf, bones)
+    init = init.astype(np.float32).astype(np.float64)                       # as the motion file is read
+    # at the initial pose every skinning transform is the identity; at any
+    # other pose it is G(pose) G(init)^-1, i.e. the zero-bound one times G(0) G(init)^-1
+    dq, jp, _, T = orc.fk(desc, init[None], transforms=True)
+    np.testing.assert_allclose(dq[0, :, :4], np.tile([0, 0, 0, 1], (18, 1)) * np.sign(dq[0, :, 3:4]), atol=1e-6)
+    np.testing.assert_allclose(dq[0, :, 4:], 0, atol=1e-6)                  # real part (x, y, z, w), dual 0
+    np.testing.assert_allclose(ch2.joint_centres, jp[0][desc["skin_joint"]], atol=1e-9)
+    pose = scenes.random_dofs(np.random.default_rng(10), skel, 1, 0.5, 0.03)
+    _, _, _, Ta = orc.fk(desc, pose, transforms=True)
+    _, _, _, Tb = orc.fk(ref, pose, transforms=True)
+    _, _, _, T0 = orc.fk(ref, init[None], transforms=True)
+    def h(m):  # R row-major, then t
+        out = np.eye(4)
+        out[:3, :3], out[:3, 3] = m[:9].reshape(3, 3), m[9:12]
+        return out
+    for b in range(18):
+        np.testing.assert_allclose(h(Ta[0, b]), h(Tb[0, b]) @ np.linalg.inv(h(T0[0, b])), atol=1e-7)
+    # a pose file without frame 0 is rejected
+    _write(tmp_path / "init.motion", "Skeletool Motion File V1.0\n3 " + " ".join("0" for _ in init) + "\n")
+    with pytest.raises(ValueError, match="no frame 0"):
+        formats.load_character(cp)
+
+
+def _write(path, text):
+    path.write_text(text)
+    return path
+
+
 def test_driven_scaling_joints_and_missing_dof_files_are_rejected(tmp_path):
     sp = tmp_path / "s.skeleton"
     sp.write_text(_SKEL09)

@@ -1,8 +1,9 @@
 """Characters read from the reference's file formats drive the GPU entry
-points: a skeleton / skin / mesh / character file set (a V1.0 skeleton, or
-a v0.9 one with its .dof file) through the GPU forward kinematics and DQ
-skinning, and an embedded graph built from a mesh
-and its decimated graph mesh through the GPU ED forward, against the oracle."""
+points: a skeleton / skin / mesh / character file set (a V1.0 skeleton, a
+v0.9 one with its .dof file, or a V1.0 one with an initial-pose file, whose
+bind carries rotations) through the GPU forward kinematics and DQ skinning,
+and an embedded graph built from a mesh and its decimated graph mesh
+through the GPU ED forward, against the oracle."""
 import numpy as np
 import pytest
 import torch
@@ -25,9 +26,18 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-@pytest.mark.parametrize("layout", ["v1.0", "v0.9+dof v0.5"])
+@pytest.mark.parametrize("layout", ["v1.0", "v0.9+dof v0.5", "v1.0+initial pose"])
 def test_character_files_through_gpu_fk_and_skinning(tmp_path, layout):
-    cp = _body_files(tmp_path)[2] if layout == "v1.0" else _body_files09(tmp_path, "v0.5")[2]
+    if layout == "v0.9+dof v0.5":
+        cp = _body_files09(tmp_path, "v0.5")[2]
+    else:
+        skel, _, cp, _ = _body_files(tmp_path)
+        if layout == "v1.0+initial pose":  # the bind then carries rotations
+            init = scenes.random_dofs(np.random.default_rng(12), skel, 1, 0.5, 0.03)[0]
+            (tmp_path / "init.motion").write_text("Skeletool Motion File V1.0\n0 "
+                                                  + " ".join(repr(float(x)) for x in init) + "\n")
+            with open(cp, "a") as fh:
+                fh.write("pose\ninit.motion\n")
     ch, desc = formats.load_character(cp)
     sk = skinning.Skeleton(desc)
     B = 2
