@@ -459,12 +459,18 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   __shared__ unsigned int s_keys[kHash];
   __shared__ unsigned int s_cnt[kHash];
   __shared__ unsigned int s_wave[kB / 64];
-  unsigned int my_cnt = 0;
-  for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
-  __syncthreads();
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
+  // the grid is sized for the largest character: workgroups past this one's
+  // faces (uniform over the block) only record that they need no slots
+  if ((long long)blockIdx.x * kB >= c.F) {
+    if (threadIdx.x == 0) d.tri_blk[(size_t)w * gridDim.x + blockIdx.x] = 0u;
+    return;
+  }
+  unsigned int my_cnt = 0;
+  for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
+  __syncthreads();
   const int f = blockIdx.x * kB + threadIdx.x;
   unsigned int* cnt = d.tile_cnt + vw.tile0;
   if (f < c.F) {
@@ -505,11 +511,12 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
   __shared__ unsigned int s_cnt[kHash];
   __shared__ unsigned int s_base[kHash];
   __shared__ unsigned int s_wave[kB / 64];
-  for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
-  __syncthreads();
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
+  if ((long long)blockIdx.x * kB >= c.F) return;  // no faces here (uniform over the block)
+  for (int i = threadIdx.x; i < kHash; i += kB) { s_keys[i] = kHashEmpty; s_cnt[i] = 0u; }
+  __syncthreads();
   const int f = blockIdx.x * kB + threadIdx.x;
   const long long ri = vw.vf0 + f;
   TileRange r{0, 0, 0, 0};
