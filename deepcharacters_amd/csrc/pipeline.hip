@@ -335,6 +335,7 @@ __device__ __forceinline__ f3 ed_inner(const Dev& d, const CharD& c, int gv) {
 __global__ __launch_bounds__(kB) void vertex_fwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
   const CharD c = d.ch[blockIdx.y];
+  if ((long long)blockIdx.x * kB >= c.N) return;  // no vertices here (uniform over the block)
   for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
   __syncthreads();
   const int v = blockIdx.x * kB + threadIdx.x;
@@ -1200,6 +1201,10 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
+  if ((long long)blockIdx.x * kB >= c.N) {  // no vertices here (uniform over the block)
+    if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = 0.f;
+    return;
+  }
   const int v = blockIdx.x * kB + threadIdx.x;
   float lsum = 0.f;
   if (v < c.N) {
@@ -1514,6 +1519,7 @@ template <bool SIL>
 __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
   const CharD c = d.ch[blockIdx.y];
+  if ((long long)blockIdx.x * kB >= c.N) return;  // no vertices here (uniform over the block)
   for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
   __syncthreads();
   const int v = blockIdx.x * kB + threadIdx.x;
