@@ -263,9 +263,9 @@ __global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, 
 }
 
 // In-place exclusive scan of a[0..n) by one workgroup, kOnePer entries per
-// lane per pass (one pass up to 16384 entries: one launch, its loads all in
+// lane per pass (one pass up to 32768 entries: one launch, its loads all in
 // flight together); *total = the sum.
-constexpr int kOnePer = 64;
+constexpr int kOnePer = 128;
 __global__ __launch_bounds__(kB) void scan_one_kernel(unsigned int* a, long long n, unsigned int* total) {
   __shared__ unsigned int s_wave[kB / 64];
   unsigned int carry = 0;
@@ -278,10 +278,23 @@ __global__ __launch_bounds__(kB) void scan_one_kernel(unsigned int* a, long long
     for (int k = 0; k < kOnePer; ++k) t += v[k];
     unsigned int chunk;
     unsigned int ex = carry + block_scan256(t, s_wave, chunk);  // each lane writes only the entries it read
+    if (base + kOnePer <= n) {
+      uint4* q = reinterpret_cast<uint4*>(a + base);
 #pragma unroll
-    for (int k = 0; k < kOnePer; ++k) {
-      if (base + k < n) a[base + k] = ex;
-      ex += v[k];
+      for (int k = 0; k < kOnePer / 4; ++k) {
+        uint4 o;
+        o.x = ex; ex += v[4 * k];
+        o.y = ex; ex += v[4 * k + 1];
+        o.z = ex; ex += v[4 * k + 2];
+        o.w = ex; ex += v[4 * k + 3];
+        q[k] = o;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kOnePer; ++k) {
+        if (base + k < n) a[base + k] = ex;
+        ex += v[k];
+      }
     }
     carry += chunk;
     __syncthreads();  // s_wave is reused by the next pass
