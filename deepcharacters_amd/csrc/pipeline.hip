@@ -4,8 +4,9 @@
 //
 // Per step (all characters at once, indexed through per-character and
 // per-view descriptor tables):
-//   clear            zero the tile / grid counters and the fallback accumulators
-//                    (fill kernel, no memset, so the sequence is graph-safe)
+//   clear            (an empty stage: the projection kernel zeroes the tile /
+//                    grid counters and the fallback accumulators with plain
+//                    stores, no memset, so the sequence is graph-safe)
 //   node_fwd         per ED node: Euler delta rotation -> 15-float node row
 //   vertex_fwd       per vertex: ED inner sum (4 nodes) then DQ skinning
 //   normals          per vertex: normalised face-normal sum of the skinned mesh
@@ -175,12 +176,6 @@ struct ClearList {
   long long n[8];
   int count;
 };
-
-__global__ __launch_bounds__(kB) void clear_kernel(ClearList L) {
-  const long long stride = (long long)gridDim.x * kB;
-  for (int k = 0; k < L.count; ++k)
-    for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < L.n[k]; i += stride) L.p[k][i] = 0u;
-}
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of any length: 1024-element block scans, a recursive scan of
@@ -389,7 +384,16 @@ __device__ __forceinline__ f2 proj2(const float* K, f3 x) {
   return mk2(h0 / h2, h1 / h2);
 }
 
-__global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
+// Also clears the step's accumulators and counters (grid-stride over the
+// whole grid, before any early return): the first of them is read by the
+// binning that follows, so the clear needs no launch of its own.
+__global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io, ClearList L) {
+  {
+    const long long stride = (long long)gridDim.x * gridDim.y * kB;
+    const long long i0 = ((long long)blockIdx.y * gridDim.x + blockIdx.x) * kB + threadIdx.x;
+    for (int k = 0; k < L.count; ++k)
+      for (long long i = i0; i < L.n[k]; i += stride) L.p[k][i] = 0u;
+  }
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
@@ -2027,8 +2031,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   // model->data residual) beside the forward and binning stages, the
   // per-face record sums and the raster part of the vertex backward beside
   // the silhouette stages, the loss totals, and the joint-DQ reduction beside
-  // the node gradients.  (A join costs ~10-20 us in a replayed graph: the
-  // buffer clears, ~8 us, measured faster on the main stream.)  Each fork is an event recorded on s and waited on by the
+  // the node gradients.  (A join costs ~10-20 us in a replayed graph, so the
+  // buffer clears run inside the projection kernel on s.)  Each fork is an event recorded on s and waited on by the
   // side stream right away; each join is an event of its own.  A profiled
   // step keeps every stage on one stream so that its events bracket it.
   const bool fork = ev == nullptr;
@@ -2039,8 +2043,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   };
   hipStream_t sn = fork ? p->side : s;
   mark();
-  hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(kB), 0, s, L);
-  mark();
+  mark();  // the clears run inside project_kernel
   hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
   mark();
   hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
@@ -2053,7 +2056,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   hipLaunchKernelGGL(view_normal_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, sn, d, io);
   if (fork) DCX_HIP(hipEventRecord(p->ev_njoin, p->side));
   mark();
-  hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io, L);
   mark();
   hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
