@@ -223,15 +223,19 @@ __device__ __forceinline__ void scan_load16(const unsigned int* in, long long ba
   }
 }
 
-__global__ __launch_bounds__(kB) void scan_sum_kernel(const unsigned int* in, unsigned int* sums, long long n) {
+__device__ __forceinline__ void scan_sum_body(const unsigned int* in, unsigned int* sums, long long n, int block) {
   __shared__ unsigned int s_wave[kB / 64];
   unsigned int v[kScanPer], t = 0;
-  scan_load16(in, (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanPer, n, v);
+  scan_load16(in, (long long)block * kScanChunk + (long long)threadIdx.x * kScanPer, n, v);
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) t += v[k];
   unsigned int total;
   (void)block_scan256(t, s_wave, total);
-  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) sums[block] = total;
+}
+
+__global__ __launch_bounds__(kB) void scan_sum_kernel(const unsigned int* in, unsigned int* sums, long long n) {
+  scan_sum_body(in, sums, n, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, unsigned int* out,
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, 
 // lane per pass (one pass up to 32768 entries: one launch, its loads all in
 // flight together); *total = the sum.
 constexpr int kOnePer = 128;
-__global__ __launch_bounds__(kB) void scan_one_kernel(unsigned int* a, long long n, unsigned int* total) {
+__device__ __forceinline__ void scan_one_body(unsigned int* a, long long n, unsigned int* total) {
   __shared__ unsigned int s_wave[kB / 64];
   unsigned int carry = 0;
   for (long long b = 0; b < n; b += (long long)kB * kOnePer) {
@@ -295,6 +299,17 @@ __global__ __launch_bounds__(kB) void scan_one_kernel(unsigned int* a, long long
     __syncthreads();  // s_wave is reused by the next pass
   }
   if (threadIdx.x == 0) *total = carry;
+}
+
+// The chunk sums of one scan plus, in one more workgroup, the single-group
+// scan of another array made by the same producer: one launch for both.
+__global__ __launch_bounds__(kB) void scan_sum_with_one_kernel(const unsigned int* in, unsigned int* sums, long long n,
+                                                               unsigned int* a1, long long n1, unsigned int* total1) {
+  if (blockIdx.x == gridDim.x - 1) {  // uniform over the block
+    scan_one_body(a1, n1, total1);
+    return;
+  }
+  scan_sum_body(in, sums, n, blockIdx.x);
 }
 
 // Scratch elements scan_big needs for n inputs (the chunk sums).
@@ -518,7 +533,7 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   for (int i = threadIdx.x; i < kHash; i += kB)
     if (s_keys[i] != kHashEmpty) atomicAdd(cnt + s_keys[i], s_cnt[i]);
   // record slots this workgroup's faces need (scanned over the workgroups,
-  // which run in (view, face) order, by scan_one_kernel)
+  // which run in (view, face) order, by the single-group record-slot scan)
   unsigned int tot;
   (void)block_scan256(my_cnt, s_wave, tot);
   if (threadIdx.x == 0) d.tri_blk[(size_t)w * gridDim.x + blockIdx.x] = tot;
@@ -2060,12 +2075,19 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   mark();
   hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
-  dcx_status st = scan_big(d.tile_cnt, d.tile_off, d.scan_aux, d.totTiles, s);
-  if (st) return st;
+  {
+    // tile counts -> tile offsets, and in the same first launch the
+    // per-workgroup record-slot counts -> record-slot offsets
+    const long long nt = d.totTiles, nb = (nt + kScanChunk - 1) / kScanChunk;
+    if (nb < 1 || nb >= (1ll << 31) - 1) return set_error(DCX_ERR_INVALID, "scan: bad length %lld", nt);
+    hipLaunchKernelGGL(scan_sum_with_one_kernel, dim3((unsigned)nb + 1), dim3(kB), 0, s, d.tile_cnt, d.scan_aux, nt,
+                       d.tri_blk, (long long)V * ceil_div(d.maxF, kB), d.tri_base + d.totVF);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kB), 0, s, d.tile_cnt, d.tile_off, d.scan_aux, nt);
+    if (dcx_status e = check_launch("scan")) return e;
+  }
   mark();
-  hipLaunchKernelGGL(scan_one_kernel, dim3(1), dim3(kB), 0, s, d.tri_blk, (long long)V * ceil_div(d.maxF, kB),
-                     d.tri_base + d.totVF);
-  mark();
+  mark();  // the record-slot scan ran inside the tile scan's first launch
+  dcx_status st = DCX_OK;
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
   {
