@@ -12,7 +12,9 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdcx.so")
+# DCX_LIBRARY names another build of the same library (e.g. to time two builds
+# of the kernels side by side on one device); the default is the in-tree one.
+LIB_PATH = os.environ.get("DCX_LIBRARY") or os.path.join(_HERE, "libdcx.so")
 
 # Signature codes: p = pointer, i = int, z = size_t, s = stream, f = float
 # Every function returns dcx_status (int) unless listed in _SPECIAL.
@@ -54,6 +56,7 @@ SIGNATURES: dict[str, str] = {
     "dcx_dtloss_fwd": "ppppiiiipppzS",
     "dcx_dtloss_bwd": "ppppiiiippS",
     "dcx_probe_atomics": "ipllS",
+    "dcx_probe_copy": "ppliiS",
 }
 
 # Functions whose return type is not dcx_status.

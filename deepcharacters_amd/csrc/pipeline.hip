@@ -2212,6 +2212,10 @@ extern "C" dcx_status dcx_pipeline_set_tile(dcx_pipeline* p, int tile_w) {
     if (tile_w == (1 << k)) tws = k;
   DCX_REQUIRE(tws >= 0, "dcx_pipeline_set_tile: tile width %d is not 16, 32 or 64", tile_w);
   if (tws == p->d.tws) return DCX_OK;
+  // A step queued on any stream (the caller's, the side stream, a graph
+  // replay) still reads the view table and runs kernels built for the old
+  // shape: wait for all of it before the table or the graph changes.
+  DCX_HIP(hipDeviceSynchronize());
   int mt = 0;
   const long long tot = layout_tiles(p->vw_host, tws, mt);
   DCX_HIP(hipMemcpy((void*)p->d.vw, p->vw_host.data(), sizeof(ViewD) * p->vw_host.size(), hipMemcpyHostToDevice));

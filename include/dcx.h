@@ -532,7 +532,11 @@ typedef struct {
   const float* extrinsics;  /* [views,12] */
   const float* intrinsics;  /* [views,9] */
   const float* target;      /* [sum over views of H*W, 3] */
-  const uint8_t* dt;        /* [sum over views of S*S] */
+  const uint8_t* dt;        /* [sum over views of S*S]: exactly S*S uint8 per view
+                               (S = that character's dt_size, square), views in
+                               order.  The step indexes it as S x S and cannot
+                               check the caller's buffer: a shorter or non-square
+                               DT image reads out of bounds. */
   const float* crops;       /* [views,7] */
   float w_color, w_sil;
   /* DEVICE outputs */
@@ -571,7 +575,9 @@ dcx_status dcx_pipeline_profile(dcx_pipeline* pipe, const dcx_pipeline_io* io, f
  * (default 32).  The tile-binned rasterizer the north-star design asks for
  * replaces the reference's per-triangle bounding-box loop
  * (CUDABasedRasterization.cu:215-451), which has no tiles.  Changing the
- * shape synchronises with the device and drops a captured graph. */
+ * shape waits for all queued device work (hipDeviceSynchronize: any stream,
+ * replays included) before it rewrites the view table, and drops a captured
+ * graph. */
 dcx_status dcx_pipeline_set_tile(dcx_pipeline* pipe, int tile_w);
 int dcx_pipeline_tile_width(const dcx_pipeline* pipe);
 /* Autotuner: runs one warm-up step and `reps` timed steps per tile shape on
@@ -591,6 +597,14 @@ dcx_status dcx_pipeline_autotune(dcx_pipeline* pipe, const dcx_pipeline_io* io, 
  * pixel, CUDABasedRasterizationGrad.cu:114-496) and depth test (atomicMin per
  * covered pixel, CUDABasedRasterization.cu:215-255) issue in global memory. */
 dcx_status dcx_probe_atomics(int kind, void* buf, long long n_addr, long long n_ops, void* stream);
+
+/* Measurement helper for the bench (no counterpart in the reference): copies
+ * `bytes` (a multiple of 16; both pointers 16-byte aligned) from src to dst
+ * with 16-byte vector loads and stores, grid-stride over `blocks` workgroups
+ * of 256 lanes (0 = 8192), plain or nontemporal.  Its read + write bytes over
+ * its time are the achievable HBM bandwidth the roofline is also quoted
+ * against. */
+dcx_status dcx_probe_copy(const void* src, void* dst, long long bytes, int nontemporal, int blocks, void* stream);
 
 #ifdef __cplusplus
 }

@@ -54,3 +54,36 @@ def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None):
     gT, gA, _, _, _ = orc.ed_bwd(g, ch.verts, g_rest, wk.dA[None], zM, dtype=dtype)
     return dict(loss=np.array([loss_col, loss_sil]), g_dq=g_dq[0], g_dT=gT[0], g_dA=gA[0], g_col=g_col[0],
                 face=face[0], render=render[0], pos=pos[0])
+
+
+def _pool_step(args):
+    wk, w_col, w_sil, positions, dtype, keys = args
+    r = step(wk, w_col, w_sil, dtype=dtype, positions=positions)
+    return {k: r[k] for k in keys}
+
+
+def step_many(works, w_col: float, w_sil: float, positions=None, dtype=np.float64,
+              keys=("loss", "g_dq", "g_dT", "g_dA", "g_col"), max_workers=16):
+    """step() for every character in a pool of spawned processes of one
+    thread each (largest characters handed out first); results in the order
+    of `works`, holding only `keys` (the image buffers are large to return)."""
+    import multiprocessing as mp
+    import os
+    positions = positions if positions is not None else [None] * len(works)
+    order = sorted(range(len(works)), key=lambda i: -works[i].ch.N * works[i].ext.shape[0])
+    tasks = [(works[i], w_col, w_sil, positions[i], dtype, tuple(keys)) for i in order]
+    workers = max(1, min(max_workers, os.cpu_count() or 1, len(works)))
+    old = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        with mp.get_context("spawn").Pool(workers) as pool:
+            res = pool.map(_pool_step, tasks, chunksize=1)
+    finally:
+        if old is None:
+            os.environ.pop("OMP_NUM_THREADS", None)
+        else:
+            os.environ["OMP_NUM_THREADS"] = old
+    out = [None] * len(works)
+    for i, r in zip(order, res):
+        out[i] = r
+    return out

@@ -235,3 +235,32 @@ def test_autotune_picks_a_shape_and_grows_the_bins(works):
     torch.cuda.synchronize()
     assert torch.equal(bp.face, want[0]) and torch.equal(bp.render, want[2])
     assert _rel(bp.g_dq, want[5]) < 1e-5
+
+
+def test_set_tile_waits_for_queued_replays(works):
+    """A shape change right after replays queued on another stream (no host
+    synchronisation in between) must not let those replays read the new view
+    table: the queued replays and the step after the change both reproduce
+    the eager results."""
+    ref = pipeline.BatchedPipeline(works)
+    ref.step()
+    torch.cuda.synchronize()
+    want = _outputs(ref)
+    bp = pipeline.BatchedPipeline(works)
+    bp.set_tile(64)
+    bp.step()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    bp.capture(s)
+    for _ in range(4):
+        bp.replay()
+    bp.set_tile(16)  # queued replays still running on s
+    after_replays = _outputs(bp)  # set_tile waited for them
+    bp.step()
+    torch.cuda.synchronize()
+    got = _outputs(bp)
+    for res in (after_replays, got):
+        for k in range(3):
+            assert torch.equal(res[k], want[k]), k
+        for k in range(3, 8):
+            assert _rel(res[k], want[k]) < 1e-5, k

@@ -30,3 +30,20 @@ def test_probe_rejects_sizes_that_are_not_powers_of_two():
     buf = torch.zeros(10, dtype=torch.float32, device="cuda")
     with pytest.raises(_lib.DcxError):
         _lib.call("dcx_probe_atomics", 0, buf.data_ptr(), 10, 100, torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("nontemporal", [0, 1])
+def test_probe_copy_copies_every_byte(nontemporal):
+    n = (1 << 20) + 4 * 37  # not a multiple of the grid's stride
+    src = torch.randint(-2**31, 2**31 - 1, (n,), dtype=torch.int32, device="cuda")
+    dst = torch.zeros_like(src)
+    _lib.call("dcx_probe_copy", src.data_ptr(), dst.data_ptr(), n * 4, nontemporal, 64,
+              torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+
+
+def test_probe_copy_rejects_unaligned_sizes():
+    buf = torch.zeros(64, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.DcxError):
+        _lib.call("dcx_probe_copy", buf.data_ptr(), buf.data_ptr(), 10, 0, 0, torch.cuda.current_stream().cuda_stream)

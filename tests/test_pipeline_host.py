@@ -96,3 +96,15 @@ def test_degenerate_batch_is_valid_input():
     ref = pipeline_ref.step(ws[-1], 1.0, 1.0)
     for key in ("loss", "g_dq", "g_dT", "g_dA", "g_col"):
         assert not np.any(ref[key]), key
+
+
+def test_oracle_step_many_matches_step(small_works):
+    """The process-pool helper the headline GPU test uses returns, per
+    character and in order, what step() returns."""
+    from oracle import pipeline_ref
+    w_col, w_sil = 1e-4, 1.0
+    many = pipeline_ref.step_many(small_works, w_col, w_sil, max_workers=2)
+    for wk, r in zip(small_works, many):
+        one = pipeline_ref.step(wk, w_col, w_sil)
+        for k in ("loss", "g_dq", "g_dT", "g_dA", "g_col"):
+            np.testing.assert_array_equal(r[k], one[k])
