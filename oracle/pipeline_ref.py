@@ -13,31 +13,36 @@ import numpy as np
 from . import oracle as orc
 
 
-def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None):
+def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None, fwd_dtype=np.float32, colors=None):
     """Returns dict(loss=[colour, silhouette], g_dq, g_dT, g_dA, g_col, face, render, pos).
 
     positions: optional skinned positions [N,3] at which the raster, losses
     and their adjoints are evaluated instead of the oracle's own skinning
-    output (so that discrete decisions match a given producer exactly)."""
+    output (so that discrete decisions match a given producer exactly).
+    fwd_dtype: precision of the forward (float32 like the product; float64
+    for finite differences of the whole step).
+    colors: optional vertex colours [N,3] in place of the character's own."""
     ch = wk.ch
     g = ch.graph
     M, N = g.num_nodes, ch.N
     ext = wk.ext.reshape(1, -1)
     intr = wk.intr.reshape(1, -1)
     zM = np.zeros((1, M, 3))
+    col = ch.colors if colors is None else np.asarray(colors)
     # forward
     # the forward runs in float32 like the product so that the discrete
     # decisions (tri ids, boundary flags, nearest points) see the same inputs
-    v_nr = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], zM, zM, np.zeros((1, N, 3)), dtype=np.float32)[0]
-    pos, _ = orc.dqs_fwd(wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=np.float32)
+    fd = fwd_dtype
+    v_nr = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], zM, zM, np.zeros((1, N, 3)), dtype=fd)[0]
+    pos, _ = orc.dqs_fwd(wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=fd)
     if positions is not None:
-        pos = np.asarray(positions, np.float32).reshape(pos.shape)
-    face, bary, render, vn = orc.raster_fwd(ch.faces, pos.astype(np.float32), ext, intr, wk.H, wk.W,
-                                            vcol=ch.colors[None], dtype=np.float32)
+        pos = np.asarray(positions, fd).reshape(pos.shape)
+    face, bary, render, vn = orc.raster_fwd(ch.faces, pos.astype(fd), ext, intr, wk.H, wk.W,
+                                            vcol=col[None], dtype=fd)
     nrm = vn[:, 0].astype(np.float64)
     nrm = nrm / np.maximum(np.linalg.norm(nrm, axis=-1, keepdims=True), 1e-30)
-    pts = orc.camproj(pos, None, ext, intr, True, dtype=np.float32)
-    nrm2 = orc.camproj(pos, nrm, ext, intr, False, dtype=np.float32)
+    pts = orc.camproj(pos, None, ext, intr, True, dtype=fd)
+    nrm2 = orc.camproj(pos, nrm, ext, intr, False, dtype=fd)
     isb = orc.boundary_flags(pts, (face >= 0).astype(np.uint8))
     res, res1, dtg, cid = orc.mvsil_fwd(pts, nrm2, isb, wk.dt[None], wk.crops[None])
     diff = render.astype(np.float64) - wk.target[None]
@@ -47,7 +52,7 @@ def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None):
     g_render = w_col * np.sign(diff) * (face >= 0)[..., None]
     g_pts = orc.mvsil_bwd(2 * w_sil * res, 2 * w_sil * res1, dtg, cid, wk.crops[None])
     g_pos = orc.camproj_bwd(g_pts, pos, ext, intr)
-    g_pos_r, g_col, _, _ = orc.raster_bwd(ch.faces, g_render, pos, ext, intr, face, bary, vn, vcol=ch.colors[None],
+    g_pos_r, g_col, _, _ = orc.raster_bwd(ch.faces, g_render, pos, ext, intr, face, bary, vn, vcol=col[None],
                                           target=wk.target[None], dtype=dtype)
     g_pos = g_pos + g_pos_r
     g_dq, _, g_rest = orc.dqs_bwd(g_pos, wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=dtype)

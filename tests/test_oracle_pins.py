@@ -1,0 +1,358 @@
+"""Pins the parts of the raster oracle's backward that the masked finite
+differences of test_oracle_raster.py leave open, and the composed step as a
+whole (CPU only):
+
+* the SH-coefficient gradient (CUDABasedRasterizationGrad.cu:317-354): the
+  shaded render is linear in the SH coefficients, so the gradient equals the
+  exact change of the render per unit coefficient;
+* the texture gradient (Grad.cu:270-310): the textured/shadeless render copies
+  the nearest texel, so the gradient of a texel is the sum of g over the
+  pixels that sample it -- and zero where the pixel normal was flipped toward
+  the camera, which the reference skips;
+* the textured position chain (Grad.cu:211-243, RendererUtil.h:566-644): on a
+  texture linear in its flat texel index the reference's image gradient is
+  the ramp's slope, so the chain equals the derivative of the UV render;
+* the swapped-weight "bilinear" albedo the reference's textured backward uses
+  (Grad.cu:216-243), in closed form on that texture, through the SH gradient;
+* textured+shaded with a constant texture equals vertexColor+shaded with a
+  constant colour;
+* the composed step (oracle/pipeline_ref.step: ED -> DQS -> raster -> L1
+  colour loss -> adjoints) against FP64 central differences of its own
+  forward, for the joint DQs, node translations and rotations and colours.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from deepcharacters_amd import pipeline, scenes
+from oracle import oracle as orc
+from oracle import pipeline_ref
+
+
+def cam(W, H, f, dist=2.0):
+    """Camera at (0, 0, -dist) looking down +z: x right, y down."""
+    ext = np.array([[1.0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, dist]])
+    intr = np.array([[f, 0, W / 2.0, 0, f, H / 2.0, 0, 0, 1]])
+    return ext, intr
+
+
+def two_cams(W, H, f):
+    """Two cameras at z = -2, one of them rotated a little about y."""
+    e0, k0 = cam(W, H, f)
+    a = 0.25
+    R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    E1 = np.concatenate([R, np.array([[0.1], [0.05], [2.1]])], 1).reshape(1, 12)
+    return np.concatenate([e0, E1], 1), np.concatenate([k0, k0], 1)
+
+
+def heightfield(n=9, seed=0, amp=0.06, facing=True):
+    """A jittered grid over [-0.6, 0.6]^2 with a gentle bump in z, its
+    triangles wound so that the face normals point toward -z (the camera)
+    when `facing`, away from it otherwise; per-face texture coordinates map
+    (x, y) linearly into [0.1, 0.9]^2."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(-0.6, 0.6, n)
+    X, Y = np.meshgrid(g, g)
+    X = X + rng.uniform(-0.03, 0.03, X.shape)
+    Y = Y + rng.uniform(-0.03, 0.03, Y.shape)
+    Z = amp * np.sin(2.0 * X + 0.7) * np.cos(1.5 * Y)
+    V = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    faces = []
+    for i in range(n - 1):
+        for j in range(n - 1):
+            a, b, c, d = i * n + j, i * n + j + 1, (i + 1) * n + j, (i + 1) * n + j + 1
+            faces += [(a, c, b), (b, c, d)] if facing else [(a, b, c), (b, d, c)]
+    faces = np.array(faces, np.int32)
+    uv = 0.1 + 0.8 * (V[:, :2] + 0.6) / 1.2
+    tc = uv[faces].reshape(-1, 6)
+    return V, faces, tc
+
+
+def flat_index_texture(TH, TW, gamma=(1.0, -0.5, 0.25)):
+    """texel (iv, iu) of channel d holds gamma_d * (iv * TW + iu)."""
+    k = np.arange(TH * TW, dtype=np.float64).reshape(TH, TW)
+    return (k[..., None] * np.asarray(gamma))[None]
+
+
+def _random_sh(rng, C):
+    sh = np.zeros((1, C, 27))
+    for d in range(3):
+        sh[0, :, 9 * d] = 0.8
+        sh[0, :, 9 * d + 1:9 * d + 9] = rng.normal(scale=0.25, size=(C, 8))
+    return sh
+
+
+def test_sh_gradient_equals_the_change_of_the_shaded_render():
+    """vertexColor + shaded: render = albedo * sum_k sh_k basis_k(n) is linear
+    in each SH coefficient, so dL/dsh_j (per camera) is exactly the change of
+    sum(g * render) for a unit change of sh_j.  All 27 coefficients of both
+    cameras, on a closed mesh whose silhouette pixels carry flipped normals."""
+    ch = scenes.ellipsoid_character(2, scale=(0.6, 0.45, 0.35), seed=1)
+    H, W = 36, 40
+    ext, intr = two_cams(W, H, 45.0)
+    rng = np.random.default_rng(7)
+    sh = _random_sh(rng, 2)
+    kw = dict(vcol=ch.colors[None], albedo="vertexColor", shading="shaded")
+    face, bary, render, vn = orc.raster_fwd(ch.faces, ch.verts[None], ext, intr, H, W, sh=sh, dtype=np.float64, **kw)
+    assert (face >= 0).mean() > 0.2
+    g = rng.normal(size=render.shape)
+    _, _, _, gsh = orc.raster_bwd(ch.faces, g, ch.verts[None], ext, intr, face, bary, vn, sh=sh, **kw)
+    for j in range(27):
+        sp = sh.copy()
+        sp[0, :, j] += 1.0
+        _, _, rp, _ = orc.raster_fwd(ch.faces, ch.verts[None], ext, intr, H, W, sh=sp, dtype=np.float64, **kw)
+        for c in range(2):
+            exp = float(np.sum(g[0, c] * (rp[0, c] - render[0, c])))
+            assert abs(gsh[0, c, j] - exp) <= 1e-10 * max(1.0, abs(exp)), (c, j, gsh[0, c, j], exp)
+    # every basis function contributes on this mesh
+    assert (np.abs(gsh) > 1e-3).all()
+
+
+@pytest.mark.parametrize("facing", [True, False])
+def test_texture_gradient_is_the_sum_of_g_over_sampling_pixels(facing):
+    """textured + shadeless: each pixel's colour is the nearest texel, so
+    dL/dtexel = sum of g over the pixels that sample it (read back from a
+    render of a texture holding each texel's own index).  With the triangles
+    wound away from the camera every pixel normal is flipped and the
+    reference's backward adds nothing to the texture (Grad.cu:283-310)."""
+    V, faces, tc = heightfield(facing=facing)
+    H, W, TH, TW = 48, 56, 64, 64
+    ext, intr = two_cams(W, H, 70.0)
+    idx_tex = np.repeat(np.arange(TH * TW, dtype=np.float64).reshape(1, TH, TW, 1), 3, -1)
+    face, bary, render, vn = orc.raster_fwd(faces, V[None], ext, intr, H, W, albedo="textured", tex=idx_tex,
+                                            texcoords=tc, dtype=np.float32)
+    cov = face >= 0
+    assert cov.mean() > 0.3
+    texel = render[..., 0][cov].astype(np.int64)
+    assert np.array_equal(texel.astype(np.float32), render[..., 0][cov])  # whole texel indices
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=render.shape)
+    tex = rng.uniform(size=(1, TH, TW, 3))
+    _, _, gtex, _ = orc.raster_bwd(faces, g, V[None], ext, intr, face, bary.astype(np.float64), vn, albedo="textured",
+                                   tex=tex, texcoords=tc.astype(np.float32))
+    if facing:
+        exp = np.stack([np.bincount(texel, weights=g[..., d][cov], minlength=TH * TW) for d in range(3)], -1)
+        np.testing.assert_allclose(gtex.reshape(-1, 3), exp, rtol=1e-12, atol=1e-12)
+        assert (np.abs(exp).sum(-1) > 0).sum() > 100
+    else:
+        assert not np.any(gtex)
+
+
+def test_textured_position_chain_is_the_derivative_of_the_uv_render():
+    """textured + shadeless on a texture linear in its flat texel index
+    (gamma_d * (iv*TW + iu)): with filter size 1 the reference's texture image
+    gradient (RendererUtil.h:566-620, sampled at the float texel position)
+    returns du = gamma and dv = gamma*TW, and dAlbedo/dbary = du * TW*u_i +
+    dv * TH*(1 - v_i) (RendererUtil.h:627-644).  The position gradient is then
+    d/dV of sum_p s_p * (TW * U_p + TW*TH * V_p), s_p = sum_d g_d gamma_d, with
+    (U, V) the UV render: checked by FP64 central differences of the UV
+    render over pixels whose triangle id is unchanged at +-h."""
+    V, faces, tc = heightfield(seed=2)
+    H, W, TH, TW = 48, 56, 64, 64
+    gamma = np.array([1.0, -0.5, 0.25])
+    ext, intr = two_cams(W, H, 70.0)
+    tex = flat_index_texture(TH, TW, gamma)
+    face, bary, uvr, vn = orc.raster_fwd(faces, V[None], ext, intr, H, W, albedo="uv", texcoords=tc, dtype=np.float64)
+    rng = np.random.default_rng(5)
+    g = rng.normal(size=uvr.shape)
+    # The reference takes the tap index as int(y*TW + x) of the float texel
+    # position: leave out pixels within 1e-3 of an index step, where a tap
+    # could round across it.
+    U0 = uvr[..., 0] * TW
+    V0 = uvr[..., 1] * TH
+    fr = np.modf(V0 * TW + U0)[0]
+    g = g * ((fr > 1e-3) & (fr < 1 - 1e-3))[..., None]
+    s = g @ gamma
+
+    def chain(gm):
+        gv, _, _, _ = orc.raster_bwd(faces, gm, V[None], ext, intr, face, bary, vn, albedo="textured", tex=tex,
+                                     texcoords=tc, texture_filter=1)
+        return gv
+
+    gvp = chain(g)
+    assert np.abs(gvp).max() > 1.0
+    h = 1e-6
+    cand = np.unique(faces[np.unique(face[face >= 0])].reshape(-1))
+    for v in rng.choice(cand, 10, replace=False):
+        for d in range(3):
+            i = 3 * int(v) + d
+            Vp = V.copy(); Vp.reshape(-1)[i] += h
+            Vm = V.copy(); Vm.reshape(-1)[i] -= h
+            fp, _, rp, _ = orc.raster_fwd(faces, Vp[None], ext, intr, H, W, albedo="uv", texcoords=tc, dtype=np.float64)
+            fm, _, rm, _ = orc.raster_fwd(faces, Vm[None], ext, intr, H, W, albedo="uv", texcoords=tc, dtype=np.float64)
+            stable = (fp == face) & (fm == face)
+            gm = g * stable[..., None]
+            an = (chain(gm) if not stable.all() else gvp).reshape(-1)[i]
+            sm = s * stable
+            fd = np.sum(sm * (TW * (rp[..., 0] - rm[..., 0]) + TW * TH * (rp[..., 1] - rm[..., 1]))) / (2 * h)
+            assert abs(fd - an) <= 1e-5 * max(1.0, abs(fd)), (int(v), d, fd, an)
+
+
+def test_swapped_bilinear_albedo_in_closed_form():
+    """The reference's textured backward forms its albedo from the four texels
+    around the float texel position (U0, V0) with each pair's weights swapped
+    (Grad.cu:216-243): on the flat-index texture, with iu = int(U0 - 0.5) and
+    iv = int(V0 - 0.5), that is gamma * ((2 iu + 1.5 - U0) + TW (2 iv + 1.5 - V0))
+    where true bilinear interpolation gives gamma * ((U0 - 0.5) + TW (V0 - 0.5)).
+    It reaches the outputs through the SH gradient of textured + shaded:
+    dL/dsh_{9d+k} = sum_p g_d * albedo_d * basis_k(n_p), with basis_k read from
+    the shaded render of unit colours under a unit SH coefficient."""
+    V, faces, tc = heightfield(seed=4)
+    H, W, TH, TW = 48, 56, 64, 64
+    gamma = np.array([1.0, -0.5, 0.25])
+    ext, intr = two_cams(W, H, 70.0)
+    tex = flat_index_texture(TH, TW, gamma)
+    rng = np.random.default_rng(9)
+    sh = _random_sh(rng, 2)
+    # FP32 forward: the backward recomputes the float texel position from
+    # these barycentrics exactly as the UV render does (U0 = U * TW, TW = 2^6)
+    face, bary, uv32, vn = orc.raster_fwd(faces, V[None], ext, intr, H, W, albedo="uv", texcoords=tc,
+                                          dtype=np.float32)
+    cov = face >= 0
+    U0 = uv32[..., 0].astype(np.float64) * TW
+    V0 = uv32[..., 1].astype(np.float64) * TH
+    iu = np.floor(U0 - 0.5)
+    iv = np.floor(V0 - 0.5)
+    alb = ((2 * iu + 1.5 - U0) + TW * (2 * iv + 1.5 - V0))[..., None] * gamma
+    ones = np.ones((1, V.shape[0], 3))
+    basis = []
+    for k in range(9):
+        e = np.zeros((1, 2, 27))
+        e[0, :, k] = 1.0
+        fk, _, lk, _ = orc.raster_fwd(faces, V[None], ext, intr, H, W, vcol=ones, shading="shaded", sh=e,
+                                      dtype=np.float64)
+        basis.append(lk[..., 0])
+        cov &= fk == face
+    g = rng.normal(size=uv32.shape) * cov[..., None]
+    _, _, _, gsh = orc.raster_bwd(faces, g, V[None], ext, intr, face, bary.astype(np.float64), vn, albedo="textured",
+                                  shading="shaded", tex=tex, sh=sh, texcoords=tc.astype(np.float32))
+    for c in range(2):
+        for d in range(3):
+            for k in range(9):
+                exp = float(np.sum(g[0, c, ..., d] * alb[0, c, ..., d] * basis[k][0, c]))
+                got = gsh[0, c, 9 * d + k]
+                assert abs(got - exp) <= 1e-5 * max(1.0, abs(exp)), (c, d, k, got, exp)
+    # the true bilinear albedo would not pass: it differs by gamma*(2 frac - 1) per axis
+    true_alb = ((U0 - 0.5) + TW * (V0 - 0.5))[..., None] * gamma
+    wrong = float(np.sum(g[0, 0, ..., 0] * true_alb[0, 0, ..., 0] * basis[0][0, 0]))
+    assert abs(wrong - gsh[0, 0, 0]) > 1e-2 * abs(gsh[0, 0, 0])
+
+
+def test_textured_shaded_with_constant_texture_equals_vertex_colour_shaded():
+    """A constant texture has zero image gradient and its (swapped) bilinear
+    albedo is the constant, so textured + shaded must give the position and
+    SH gradients of vertexColor + shaded with every colour equal to it (the
+    latter pinned by finite differences in test_oracle_raster.py)."""
+    ch = scenes.ellipsoid_character(2, scale=(0.6, 0.45, 0.35), seed=3)
+    H, W, TH, TW = 36, 40, 32, 32
+    ext, intr = two_cams(W, H, 45.0)
+    rng = np.random.default_rng(11)
+    sh = _random_sh(rng, 2)
+    tc = (0.1 + 0.8 * rng.uniform(size=(ch.F, 6)))
+    colour = np.array([0.7, 0.4, 0.55])
+    tex = np.broadcast_to(colour, (1, TH, TW, 3)).copy()
+    vcol = np.broadcast_to(colour, (1, ch.N, 3)).copy()
+    face, bary, render, vn = orc.raster_fwd(ch.faces, ch.verts[None], ext, intr, H, W, vcol=vcol, shading="shaded",
+                                            sh=sh, dtype=np.float64)
+    g = rng.normal(size=render.shape)
+    gv_t, _, _, gsh_t = orc.raster_bwd(ch.faces, g, ch.verts[None], ext, intr, face, bary, vn, albedo="textured",
+                                       shading="shaded", tex=tex, sh=sh, texcoords=tc)
+    gv_c, _, _, gsh_c = orc.raster_bwd(ch.faces, g, ch.verts[None], ext, intr, face, bary, vn, vcol=vcol,
+                                       albedo="vertexColor", shading="shaded", sh=sh)
+    assert np.abs(gv_c).max() > 1e-2
+    np.testing.assert_allclose(gv_t, gv_c, rtol=0, atol=1e-9 * np.abs(gv_c).max())
+    np.testing.assert_allclose(gsh_t, gsh_c, rtol=1e-10, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------
+# the composed step
+# ----------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def tiny_work():
+    wk = pipeline.build_works(1, verts_range=(1200, 1400), nodes_range=(24, 30), C=2, W=72, H=60, S=16, f=150.0,
+                              seed=13)[0]
+    C = wk.ext.shape[0]
+    # a target below every rendered colour: the L1 colour loss has one sign
+    # on every covered pixel, and a constant target has no image gradient
+    # (no model-to-data term)
+    wk.target = np.full((C, wk.H, wk.W, 3), -1.0, np.float32)
+    wk.dt = np.full((C, wk.S, wk.S), 255, np.uint8)
+    wk.crops = np.tile(np.array([0, 0, 5.0, 0, 0, 80.0, 80.0], np.float32), (C, 1))
+    return wk
+
+
+def _loss(wk, colors=None):
+    r = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64, colors=colors)
+    return float(r["loss"][0]), r["face"]
+
+
+def test_composed_step_gradients_match_central_differences(tiny_work):
+    """d(colour loss)/d(joint DQs, node translations, node rotations, vertex
+    colours) of the whole chain ED -> DQS -> raster -> L1 by FP64 central
+    differences of the step's own FP64 forward; a parameter whose +-h step
+    changes any triangle id is replaced by another.  A wrong adjoint hand-off
+    (e.g. the ED backward fed the gradient w.r.t. the skinned instead of the
+    rest positions) fails this."""
+    wk = tiny_work
+    ref = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
+    assert (ref["face"] >= 0).mean() > 0.1
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    checked = {}
+    for name, grad in (("dq", ref["g_dq"]), ("dT", ref["g_dT"]), ("dA", ref["g_dA"])):
+        base = getattr(wk, name)
+        scale = np.abs(grad).max()
+        assert scale > 0, name
+        order = rng.permutation(base.size)
+        # parameters with a non-negligible gradient first
+        order = np.concatenate([order[np.abs(grad.reshape(-1)[order]) > 1e-3 * scale],
+                                order[np.abs(grad.reshape(-1)[order]) <= 1e-3 * scale]])
+        n = 0
+        for i in order:
+            w = copy.copy(wk)
+            vals = []
+            faces = []
+            for sgn in (1.0, -1.0):
+                p = base.astype(np.float64).copy()
+                p.reshape(-1)[i] += sgn * h
+                setattr(w, name, p)
+                l, f = _loss(w)
+                vals.append(l)
+                faces.append(f)
+            if not (np.array_equal(faces[0], ref["face"]) and np.array_equal(faces[1], ref["face"])):
+                continue
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = float(grad.reshape(-1)[i])
+            assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-2 * scale), (name, int(i), fd, an)
+            n += 1
+            if n == 8:
+                break
+        checked[name] = n
+    # colours: the loss is linear in them at a fixed sign
+    col = wk.ch.colors.astype(np.float64)
+    gc = ref["g_col"]
+    for i in rng.choice(np.nonzero(np.abs(gc.reshape(-1)) > 0)[0], 8, replace=False):
+        cp = col.copy(); cp.reshape(-1)[i] += 1e-3
+        cm = col.copy(); cm.reshape(-1)[i] -= 1e-3
+        fd = (_loss(wk, cp)[0] - _loss(wk, cm)[0]) / 2e-3
+        assert abs(fd - gc.reshape(-1)[i]) <= 1e-8 * max(1.0, abs(fd)), (int(i), fd, gc.reshape(-1)[i])
+    assert checked == {"dq": 8, "dT": 8, "dA": 8}, checked
+
+
+def test_composed_step_fails_with_a_wrong_hand_off(tiny_work, monkeypatch):
+    """The check above is sensitive to the ED <-> DQS hand-off: feeding the ED
+    backward the gradient w.r.t. the skinned positions (dqs_bwd's input)
+    instead of the rest positions (its output) moves g_dT far off."""
+    wk = tiny_work
+    good = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
+    real = orc.dqs_bwd
+
+    def planted(g_pos, *a, **k):
+        g_dq, g_w, g_rest = real(g_pos, *a, **k)
+        return g_dq, g_w, g_pos
+    monkeypatch.setattr(orc, "dqs_bwd", planted)
+    bad = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
+    rel = np.linalg.norm(bad["g_dT"] - good["g_dT"]) / np.linalg.norm(good["g_dT"])
+    assert rel > 1e-2, rel
