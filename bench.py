@@ -89,7 +89,7 @@ def _cpu_step(args):
     wk, w_col, w_sil = args
     from oracle import pipeline_ref
     t0 = time.perf_counter()
-    r = pipeline_ref.step(wk, w_col, w_sil)
+    r = pipeline_ref.step(wk, w_col, w_sil, dtype=np.float32)
     return time.perf_counter() - t0, float(r["loss"][0] + r["loss"][1])
 
 
@@ -121,8 +121,9 @@ def cpu_baseline(works, w_col, w_sil, max_workers=16, passes=2):
     return dict(value=len(tasks) / wall, unit="char-frames/s", cores=workers, kind="port",
                 single_thread_value=len(tasks) / cpu_s, host_cpus=os.cpu_count(),
                 sample=f"{passes} full steps of the same {len(works)} characters (fwd+bwd, 4 views at 1024x1024), "
-                       f"one character per task (largest first), {workers} processes of 1 thread; "
-                       f"{cpu_s:.1f} s of CPU work, {wall:.1f} s wall")
+                       f"FP32 forward and backward, the native raster rule (the reference's ray test rejects "
+                       f"metre-scale triangles), one character per task (largest first), {workers} processes of "
+                       f"1 thread; {cpu_s:.1f} s of CPU work, {wall:.1f} s wall")
 
 
 def hbm_copy_gbs(torch, mib=2048, reps=5):
@@ -186,6 +187,127 @@ def atomic_rates(torch, n_addr=1 << 24, n_ops=1 << 27, reps=3):
 
 
 # ----------------------------------------------------------------------------
+# Secondary workloads: the plain forward, the single character, the temporal
+# sequence and the stress batch, each batched (captured graph) and plain
+# ----------------------------------------------------------------------------
+
+def time_batched(torch, works, steps, warmup, tile="auto", **kw):
+    """ms per captured step and the chosen tile width (kw: BatchedPipeline
+    options, e.g. the uv / normal buffers)."""
+    from deepcharacters_amd import pipeline
+    bp = pipeline.BatchedPipeline(works, **kw)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        if tile == "auto":
+            bp.autotune(reps=2)
+        bp.step()
+        s.synchronize()
+        bp.capture(s)
+        for _ in range(warmup):
+            bp.replay()
+        s.synchronize()
+        if not torch.isfinite(bp.loss).all():
+            raise RuntimeError("non-finite loss")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bp.replay()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+    tw = bp.tile_width
+    del bp
+    return ms, tw
+
+
+def time_plain(torch, works, steps):
+    from deepcharacters_amd import pipeline
+    pp = pipeline.PlainPipeline(works)
+    pp.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pp.step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    del pp
+    return ms
+
+
+def plain_forward_lines(torch, steps=50):
+    """LBS + rasterisation of the 10,242-vertex icosphere body into four 512^2
+    views (tri-id, barycentrics, vertex colours): frames/s of the binned and
+    the plain GPU renderer, and of the CPU restatement (all host cores)."""
+    from deepcharacters_amd import renderer, skinning, workloads
+    ch, T, w, disp, ext, intr, W, H = workloads.plain_forward_scene()
+    t = lambda a, dt=torch.float32: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+    args = (t(T), t(w), t(ch.joint_idx, torch.int32), t(ch.verts), t(ch.normals), t(disp))
+    col, e, k = t(ch.colors[None]), t(ext.reshape(1, -1)), t(intr.reshape(1, -1))
+    topo = renderer.Topology(ch.faces, ch.N)
+    out = {}
+    for path in ("binned", "plain"):
+        cfg = renderer.make_cfg(4, W, H, path=path)
+        ws = renderer.workspace(topo, cfg, 1, "cuda")
+
+        def frame():
+            pos = skinning.lb_skinning_gpu(*args)[0]
+            renderer.render_forward(topo, cfg, pos, col, None, None, e, k, ws=ws)
+        for _ in range(3):
+            frame()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            frame()
+        torch.cuda.synchronize()
+        out[f"gpu_{path}_frames_per_s"] = steps / (time.perf_counter() - t0)
+    from oracle import oracle as orc  # the CPU leg only
+    orc.lib()
+    reps, t0 = 0, time.perf_counter()
+    while reps < 3 or time.perf_counter() - t0 < 5.0:
+        pos, _ = orc.lbs_fwd(T, w, ch.joint_idx, ch.verts, ch.normals, disp, dtype=np.float32)
+        orc.raster_fwd(ch.faces, pos, ext.reshape(1, -1), intr.reshape(1, -1), H, W, vcol=ch.colors[None])
+        reps += 1
+    out["cpu_frames_per_s"] = reps / (time.perf_counter() - t0)
+    out["cpu_cores"] = os.cpu_count()
+    out["cpu_kind"] = "port (FP32 LBS + raster restatement, OpenMP over the views' rows)"
+    out["unit"] = "frames/s (one frame: LBS + 4 views at 512x512)"
+    return out
+
+
+def secondary_lines(torch, log):
+    from deepcharacters_amd import workloads
+    res = {}
+    t0 = time.time()
+    res["plain_forward_10k_verts_4x512sq"] = plain_forward_lines(torch)
+    log(f"plain forward: {res['plain_forward_10k_verts_4x512sq']} ({time.time() - t0:.0f} s)")
+    t0 = time.time()
+    wk, _, _ = workloads.single_character_work()
+    ms, tw = time_batched(torch, [wk], 20, 5)
+    pms = time_plain(torch, [wk], 5)
+    res["single_character_25k_verts_4x1024sq"] = dict(batched_ms=ms, plain_ms=pms, tile=tw,
+                                                     value=1000.0 / ms, plain_value=1000.0 / pms,
+                                                     unit="char-frames/s")
+    del wk
+    log(f"single character: {res['single_character_25k_verts_4x1024sq']} ({time.time() - t0:.0f} s)")
+    t0 = time.time()
+    ws = workloads.temporal_works()
+    ms, tw = time_batched(torch, ws, 5, 2, uv_buffer=True, normal_buffer=True)
+    pms = time_plain(torch, ws, 2)
+    res["temporal_64_frames_8x1285x940"] = dict(batched_ms=ms, plain_ms=pms, tile=tw, value=len(ws) * 1000.0 / ms,
+                                               plain_value=len(ws) * 1000.0 / pms, unit="frames/s (8 views each)",
+                                               note="batched step also writes the per-pixel UV and normal buffers")
+    del ws
+    log(f"temporal: {res['temporal_64_frames_8x1285x940']} ({time.time() - t0:.0f} s)")
+    t0 = time.time()
+    ws = workloads.stress_works()
+    ms, tw = time_batched(torch, ws, 5, 2)
+    pms = time_plain(torch, ws, 2)
+    res["stress_64_chars_mixed_res"] = dict(batched_ms=ms, plain_ms=pms, tile=tw, value=len(ws) * 1000.0 / ms,
+                                           plain_value=len(ws) * 1000.0 / pms, unit="char-frames/s (4 views each)")
+    del ws
+    log(f"stress: {res['stress_64_chars_mixed_res']} ({time.time() - t0:.0f} s)")
+    torch.cuda.empty_cache()
+    return res
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -197,6 +319,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the plain-forward, single-character, temporal and stress workloads")
     ap.add_argument("--tile", default="auto", choices=["auto", "16", "32", "64"],
                     help="raster tile width (tiles of w x 256/w pixels); auto = time each and keep the fastest")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -209,16 +333,15 @@ def main():
         sys.exit(2)
 
     import torch
-    from deepcharacters_amd import pipeline
+    from deepcharacters_amd import pipeline, workloads
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     t0 = time.time()
-    works = pipeline.build_works(args.chars, seed=2)
+    works = workloads.batched_works(n_chars=args.chars)
     log(f"built {len(works)} characters in {time.time() - t0:.1f} s "
         f"(N={sum(w.ch.N for w in works)}, F={sum(w.ch.F for w in works)})")
-    pipeline.make_targets(works, pipeline.gpu_pose_renderer(), np.random.default_rng(2))
     torch.cuda.synchronize()
     log("targets rendered")
 
@@ -329,6 +452,7 @@ def main():
         "atomics": atomics,
         "cpu_baseline": None,
     }
+    out["secondary"] = None if args.no_secondary else secondary_lines(torch, log)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(works, bp.w_color, bp.w_sil)

@@ -77,6 +77,7 @@ struct CharD {
   int N, F, M, J;
   int v0, f0, m0, j0;  // global offsets
   int view0, C;
+  int has_tc;          // texture coordinates given (the UV buffer is written)
 };
 
 struct ViewD {
@@ -112,10 +113,12 @@ struct Dev {
   const int* jv_ent;      // local vertex * 4 + slot, increasing
   const int4* jchunk;     // [nChunk] (first entry, end entry, character vertex base, joint)
   const int* jc_off;      // [sumJ+1] joint -> its chunks
+  const float* tc;        // [sumF,6] texture coordinates (0 for characters without)
   // scratch
   float* node_tab;        // [sumM,15]
   float* vdef;            // [sumN,3] skinned vertices
-  float* vnrm;            // [sumN,3]
+  float* vnrm;            // [sumN,3] unit vertex normals
+  float* vsum;            // [sumN,3] the unnormalised sums of the face normals (as CudaRendererGpu's)
   f4* proj;               // [totVN]
   float* ke;              // [V,12] per view: K * R (3x3 row-major), then K * t
   f2* nrm2;               // [totVN]
@@ -147,6 +150,7 @@ struct Dev {
   float* vq;              // [sumN,8] gradient w.r.t. the vertex's blended, normalised DQ
   float* jpart;           // [nChunk,8] per-chunk joint DQ gradient partials
   float* vws;             // [sumN,4] skinning weight x blend sign per slot
+  float* node_skin;       // [sumM,12] skinned node rotation (row-major) and translation (graph skinning)
 };
 
 struct IO {
@@ -169,6 +173,10 @@ struct IO {
   float* g_dT;
   float* g_dA;
   float* g_col;
+  float* uv;              // [totPix,2] (may be null)
+  float* nrm_out;         // [totPix,3] (may be null)
+  const float* skT;       // [sumM,3] skinned node translations, or null: dual-quaternion skinning
+  const float* skA;       // [sumM,3] skinned node Euler angles (null with skT)
 };
 
 struct ClearList {
@@ -339,6 +347,15 @@ __global__ __launch_bounds__(kB) void node_fwd_kernel(Dev d, IO io) {
   for (int k = 0; k < 9; ++k) row[k] = R.m[k];
   row[9] = g.x; row[10] = g.y; row[11] = g.z;
   row[12] = t.x; row[13] = t.y; row[14] = t.z;
+  if (io.skA) {
+    // the node's skinned transform (EmbeddedGraphGPUOp.cu:100-114 evaluates
+    // it per vertex and node; here once per node)
+    const m3 Rs = euler_mat(euler_sc(ld3(io.skA + 3 * (size_t)gi)));
+    const f3 ts = ld3(io.skT + 3 * (size_t)gi);
+    float* sk = d.node_skin + (size_t)gi * 12;
+    for (int k = 0; k < 9; ++k) sk[k] = Rs.m[k];
+    sk[9] = ts.x; sk[10] = ts.y; sk[11] = ts.z;
+  }
 }
 
 __device__ __forceinline__ f3 ed_inner(const Dev& d, const CharD& c, int gv) {
@@ -355,16 +372,38 @@ __device__ __forceinline__ f3 ed_inner(const Dev& d, const CharD& c, int gv) {
   return v;
 }
 
+// The embedded graph's outer sum with the skinned node transforms
+// (EmbeddedGraphGPUOp.cu:100-114): sum over the vertex's nodes s of
+// w_s (R_s v_nr + t_s), in node order.
+__device__ __forceinline__ f3 ed_outer(const Dev& d, const CharD& c, int gv, f3 vnr) {
+  f3 o = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const float* r = d.node_skin + ((size_t)c.m0 + d.enode[(size_t)gv * KE + k]) * 12;
+    m3 R;
+    for (int q = 0; q < 9; ++q) R.m[q] = r[q];
+    o += d.ew[(size_t)gv * KE + k] * (mul(R, vnr) + ld3(r + 9));
+  }
+  return o;
+}
+
 __global__ __launch_bounds__(kB) void vertex_fwd_kernel(Dev d, IO io) {
   __shared__ float s_tab[256 * 8];
   const CharD c = d.ch[blockIdx.y];
   if ((long long)blockIdx.x * kB >= c.N) return;  // no vertices here (uniform over the block)
-  for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
+  if (!io.skA)
+    for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
   __syncthreads();
   const int v = blockIdx.x * kB + threadIdx.x;
   if (v >= c.N) return;
   const int gv = c.v0 + v;
   const f3 vnr = ed_inner(d, c, gv);
+  if (io.skA) {
+    const f3 o = ed_outer(d, c, gv, vnr);
+    d.vdef[3 * (size_t)gv] = o.x; d.vdef[3 * (size_t)gv + 1] = o.y; d.vdef[3 * (size_t)gv + 2] = o.z;
+    if (io.vout) { io.vout[3 * (size_t)gv] = o.x; io.vout[3 * (size_t)gv + 1] = o.y; io.vout[3 * (size_t)gv + 2] = o.z; }
+    return;
+  }
   int idx[KS];
   float w[KS];
   for (int k = 0; k < KS; ++k) { idx[k] = d.jidx[(size_t)gv * KS + k]; w[k] = d.jw[(size_t)gv * KS + k]; }
@@ -389,6 +428,7 @@ __global__ __launch_bounds__(kB) void normals_kernel(Dev d) {
   const float l = sqrtf(dot(s, s));
   const f3 n = l > 0.f ? mk3(s.x / l, s.y / l, s.z / l) : mk3(0.f, 0.f, 0.f);
   d.vnrm[3 * (size_t)gv] = n.x; d.vnrm[3 * (size_t)gv + 1] = n.y; d.vnrm[3 * (size_t)gv + 2] = n.z;
+  d.vsum[3 * (size_t)gv] = s.x; d.vsum[3 * (size_t)gv + 1] = s.y; d.vsum[3 * (size_t)gv + 2] = s.z;
 }
 
 __device__ __forceinline__ f2 proj2(const float* K, f3 x) {
@@ -714,6 +754,7 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 
 __device__ __forceinline__ void bary_vjp_fast(const f4 P[3], const float* M, float px, float py, const float gb_in[3],
                                               f3 out[3]) {
+#pragma clang fp contract(fast)
   float e[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -840,8 +881,47 @@ __device__ __forceinline__ void emit_record(const Dev& d, const IO& io, const Ch
   }
 }
 
+// The optional 'uv' and 'normal' buffers of pixel o (CudaRendererGpu's uv and
+// normal albedo modes: the op renderer's shade_pixel, the same IEEE
+// operations in the same order, so they match it bit for bit); f < 0: off
+// the mesh, zeros.
+__device__ __forceinline__ void write_extras(const Dev& d, const IO& io, const CharD& c, int view, long long o, int f,
+                                             int i0, int i1, int i2, float a, float b, float px, float py) {
+  if (io.uv) {
+    float tu = 0.f, tv = 0.f;
+    if (f >= 0 && c.has_tc) {
+      const float cw = 1.f - a - b;
+      const float* t6 = d.tc + ((size_t)c.f0 + f) * 6;
+      tu = t6[0] * a + t6[2] * b + t6[4] * cw;
+      tv = (1.f - t6[1]) * a + (1.f - t6[3]) * b + (1.f - t6[5]) * cw;
+      tu = fminf(fmaxf(tu, 0.f), 1.f);
+      tv = fminf(fmaxf(tv, 0.f), 1.f);
+    }
+    io.uv[2 * o] = tu;
+    io.uv[2 * o + 1] = tv;
+  }
+  if (io.nrm_out) {
+    f3 col = mk3(0.f, 0.f, 0.f);
+    if (f >= 0) {
+      const float cw = 1.f - a - b;
+      const float* vs = d.vsum + 3 * (size_t)c.v0;
+      const f3 n0 = ld3(vs + 3 * i0), n1 = ld3(vs + 3 * i1), n2 = ld3(vs + 3 * i2);
+      f3 pn = mk3(n0.x * a + n1.x * b + n2.x * cw, n0.y * a + n1.y * b + n2.y * cw, n0.z * a + n1.z * b + n2.z * cw);
+      const float pl = sqrtf(dot(pn, pn));
+      pn = mk3(pn.x / pl, pn.y / pl, pn.z / pl);
+      const f3 rd = world_ray_dir(io.ext + 12 * (size_t)view, io.intr + 9 * (size_t)view, px, py);
+      if (dot(pn, rd) > 0.f) pn = -pn;
+      col = mk3((1.f + pn.x) / 2.f, (1.f + pn.y) / 2.f, (1.f + pn.z) / 2.f);
+    }
+    io.nrm_out[3 * o] = col.x;
+    io.nrm_out[3 * o + 1] = col.y;
+    io.nrm_out[3 * o + 2] = col.z;
+  }
+}
+
 // TWS: log2 of the tile width (tiles are (1 << TWS) x (1 << (8 - TWS)) pixels).
-template <int TWS>
+// EXTRA: the step also writes the uv / normal buffers.
+template <int TWS, bool EXTRA>
 __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) void fine_kernel(Dev d, IO io) {
   constexpr int tws = TWS, ths = 8 - TWS;
   constexpr int aw = (1 << tws) + 2 * kImgFs, an = aw * ((1 << ths) + 2 * kImgFs);  // apron row length, size
@@ -856,10 +936,10 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   __shared__ union U2 {
     float tgt[3 * an];
     struct {
-      unsigned int wcnt[kTilePix / 64][kFast];
-      unsigned short order[kTilePix];
-      unsigned int e[kFast];
-      int fj[kFast];
+      unsigned short wcnt[kTilePix / 64][kFast];  // pixels per (wavefront, list entry), then their prefix
+      unsigned short order[kTilePix];             // covered pixels sorted by triangle
+      unsigned char won[kFast];                   // list entries that won pixels, compacted
+      unsigned int e[kFast];                      // record slot of each list entry
     } srt;
     int face[kTilePix];
   } u2;
@@ -894,6 +974,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
       io.render[3 * o] = 0.f; io.render[3 * o + 1] = 0.f; io.render[3 * o + 2] = 0.f;
       lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
+      if (EXTRA) write_extras(d, io, c, w, o, -1, 0, 0, 0, 0.f, 0.f, 0.f, 0.f);
     }
     const float lt_sum = block_sum(lsum, s_red);
     if (threadIdx.x == 0) d.loss_tile[tile] = lt_sum;
@@ -1003,6 +1084,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
       io.render[3 * o] = 0.f; io.render[3 * o + 1] = 0.f; io.render[3 * o + 2] = 0.f;
       lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
+      if (EXTRA) write_extras(d, io, c, w, o, -1, 0, 0, 0, 0.f, 0.f, 0.f, 0.f);
     } else {
       const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
       const int i0 = fv[0], i1 = fv[1], i2 = fv[2];
@@ -1017,6 +1099,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       io.face[o] = f;
       io.bary[2 * o] = ab.x; io.bary[2 * o + 1] = ab.y;
       io.render[3 * o] = col.x; io.render[3 * o + 1] = col.y; io.render[3 * o + 2] = col.z;
+      if (EXTRA) write_extras(d, io, c, w, o, f, i0, i1, i2, ab.x, ab.y, px, py);
       const f3 df = col - tgt;
       lsum = io.w_col * (fabsf(df.x) + fabsf(df.y) + fabsf(df.z));
       auto sgn = [](float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); };
@@ -1024,6 +1107,11 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       // colour gradient g * w_k is formed when the records are summed
       s_g[9 * kGPad + pix] = g.x; s_g[10 * kGPad + pix] = g.y; s_g[11 * kGPad + pix] = g.z;
       s_g[12 * kGPad + pix] = ab.x; s_g[13 * kGPad + pix] = ab.y;
+      {
+      // gradients only from here on (they carry a tolerance; the buffers
+      // above are written with uncontracted IEEE arithmetic): multiply-adds
+      // may fuse
+#pragma clang fp contract(fast)
       const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
       const float* M = d.ke + (size_t)w * 12;
       const f4 P[3] = {pv[i0], pv[i1], pv[i2]};
@@ -1071,6 +1159,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
         s_g[(3 * k + 1) * kGPad + pix] = gx[k].y;
         s_g[(3 * k + 2) * kGPad + pix] = gx[k].z;
       }
+      }
     }
   }
   const float lt_sum = block_sum(lsum, s_red);  // its barriers also publish s_g and retire u2.tgt
@@ -1079,72 +1168,102 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   if (fast) {
     // ---- counting sort of the covered pixels by winning triangle, pixel order
     // kept within each triangle: ranks within the wavefront from ballots,
-    // then per-wavefront counts and a scan over the triangles
+    // then per-wavefront counts and one scan that gives both the triangles'
+    // offsets and a compact list of the triangles that won pixels (those
+    // whose first pixel -- rank 0 in the first wavefront holding any of it --
+    // is a pixel of this tile; most listed triangles win none)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < (kTilePix / 64) * kFast; i += kTilePix) (&u2.srt.wcnt[0][0])[i] = 0u;
-    if (threadIdx.x < n_items) { u2.srt.e[threadIdx.x] = my_e; u2.srt.fj[threadIdx.x] = my_f; }
+    static_assert(sizeof(u2.srt.wcnt) == kTilePix * 8, "one 8-byte clear per lane");
+    reinterpret_cast<uint2*>(&u2.srt.wcnt[0][0])[threadIdx.x] = make_uint2(0u, 0u);
+    if (threadIdx.x < n_items) u2.srt.e[threadIdx.x] = my_e;
     __syncthreads();
+    // one pass per distinct triangle in the wavefront; the leader's triangle
+    // is read into a scalar register (no LDS round trip per pass)
     int rank = 0;
     unsigned long long act = __ballot(bj >= 0);
     while (act) {
       const int leader = __ffsll((long long)act) - 1;
-      const int jv = __shfl(bj, leader, 64);
+      const int jv = __builtin_amdgcn_readlane(bj, leader);
       const unsigned long long m = __ballot(bj == jv);
-      if (bj == jv) rank = __popcll(m & ((1ull << lane) - 1ull));
-      if (lane == leader) u2.srt.wcnt[wv][jv] = (unsigned int)__popcll(m);
+      if (bj == jv) rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      if (lane == leader) u2.srt.wcnt[wv][jv] = (unsigned short)__popcll(m);
       act &= ~m;
     }
     __syncthreads();
-    unsigned int cnt = 0;
-    if (threadIdx.x < kFast) {
+    bool first = bj >= 0 && rank == 0;
+    for (int q = 0; q < wv; ++q) first = first && u2.srt.wcnt[q][max(bj, 0)] == 0;
+    unsigned int cnt = 0, pre[kTilePix / 64];
 #pragma unroll
-      for (int q = 0; q < kTilePix / 64; ++q) {
-        const unsigned int cq = u2.srt.wcnt[q][threadIdx.x];
-        u2.srt.wcnt[q][threadIdx.x] = cnt;  // prefix over wavefronts
-        cnt += cq;
-      }
+    for (int q = 0; q < kTilePix / 64; ++q) {
+      pre[q] = cnt;  // list entry threadIdx.x: its pixels in earlier wavefronts
+      cnt += u2.srt.wcnt[q][threadIdx.x];
     }
-    unsigned int total;
-    const unsigned int ex = block_scan256(cnt, s_wave, total);
-    if (threadIdx.x < kFast) s_start[threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_start[kFast] = total;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kTilePix / 64; ++q) u2.srt.wcnt[q][threadIdx.x] = (unsigned short)pre[q];
+    // counts (<= 256) in the high half, first-pixel flags in the low half
+    unsigned int tot2;
+    const unsigned int ex2 = block_scan256((cnt << 16) | (first ? 1u : 0u), s_wave, tot2);
+    s_start[threadIdx.x] = ex2 >> 16;
+    if (threadIdx.x == 0) s_start[kFast] = tot2 >> 16;
+    if (first) u2.srt.won[ex2 & 0xffffu] = (unsigned char)bj;
+    const int n_won = (int)(tot2 & 0xffffu);
     __syncthreads();
     if (bj >= 0) u2.srt.order[s_start[bj] + u2.srt.wcnt[wv][bj] + rank] = (unsigned short)pix;
+    if (threadIdx.x < n_items) {
+      const unsigned int e = u2.srt.e[threadIdx.x];
+      if (e < d.capacity) d.rec_vis[e] = cnt > 0 ? 1 : 0;
+    }
     __syncthreads();
-    // ---- one (triangle, vertex) pair per work item: the vertex's 3 position
-    // and 3 colour components, each summed over the triangle's pixels in
-    // pixel order
-    for (int i = threadIdx.x; i < 3 * (int)n_items; i += kTilePix) {
-      const int j = i / 3, k = i - 3 * j;
+    // ---- one (won triangle, vertex) pair per work item: the vertex's 3
+    // position and 3 colour components, each summed over the triangle's
+    // pixels in pixel order (so the sums repeat bit for bit, whatever order
+    // the tile list came in)
+    for (int i = threadIdx.x; i < 3 * n_won; i += kTilePix) {
+#pragma clang fp contract(fast)
+      const int wi = i / 3, k = i - 3 * wi;
+      const int j = u2.srt.won[wi];
       const unsigned int st = s_start[j], en = s_start[j + 1];
       // w_k = s0 w0 + s1 w1 + s2 (1 - w0 - w1) with one s = 1: exact, branch-free
       const float s0 = k == 0 ? 1.f : 0.f, s1 = k == 1 ? 1.f : 0.f, s2 = k == 2 ? 1.f : 0.f;
       const float* gk = s_g + (3 * k) * kGPad;
       float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      for (unsigned int r = st; r < en; ++r) {
-        const int p = u2.srt.order[r];
-        const float w0 = s_g[12 * kGPad + p], w1 = s_g[13 * kGPad + p];
-        const float wk = (w0 * s0 + w1 * s1) + (1.f - w0 - w1) * s2;
-        a[0] += gk[p];
-        a[1] += gk[kGPad + p];
-        a[2] += gk[2 * kGPad + p];
-        a[3] += s_g[9 * kGPad + p] * wk;
-        a[4] += s_g[10 * kGPad + p] * wk;
-        a[5] += s_g[11 * kGPad + p] * wk;
+      // kSumUnroll pixels' table reads are issued together (the LDS latency
+      // of one pixel's reads is paid once per group); the additions still run
+      // in pixel order
+      constexpr int kSumUnroll = 4;
+      for (unsigned int r0 = st; r0 < en; r0 += kSumUnroll) {
+        float v[kSumUnroll][6];
+#pragma unroll
+        for (int u = 0; u < kSumUnroll; ++u) {
+          const unsigned int r = min(r0 + (unsigned int)u, en - 1u);
+          const int p = u2.srt.order[r];
+          const float w0 = s_g[12 * kGPad + p], w1 = s_g[13 * kGPad + p];
+          const float wk = (w0 * s0 + w1 * s1) + (1.f - w0 - w1) * s2;
+          v[u][0] = gk[p];
+          v[u][1] = gk[kGPad + p];
+          v[u][2] = gk[2 * kGPad + p];
+          v[u][3] = s_g[9 * kGPad + p] * wk;
+          v[u][4] = s_g[10 * kGPad + p] * wk;
+          v[u][5] = s_g[11 * kGPad + p] * wk;
+        }
+#pragma unroll
+        for (int u = 0; u < kSumUnroll; ++u)
+          if (r0 + (unsigned int)u < en) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) a[q] += v[u][q];
+          }
       }
       const unsigned int e = u2.srt.e[j];
-      const bool won = en > st;
       if (e < d.capacity) {
-        if (k == 0) d.rec_vis[e] = won ? 1 : 0;
-        if (won) {
-          float2* rp = reinterpret_cast<float2*>(d.rec + (size_t)e * kGrad + 6 * k);
-          rp[0] = make_float2(a[0], a[1]);
-          rp[1] = make_float2(a[2], a[3]);
-          rp[2] = make_float2(a[4], a[5]);
-        }
-      } else if (won) {
+        float2* rp = reinterpret_cast<float2*>(d.rec + (size_t)e * kGrad + 6 * k);
+        rp[0] = make_float2(a[0], a[1]);
+        rp[1] = make_float2(a[2], a[3]);
+        rp[2] = make_float2(a[4], a[5]);
+      } else {
+        const int face = d.tri[d.bins[off + j]].face_flags & kFaceMask;
 #pragma unroll
-        for (int cmp = 0; cmp < 6; ++cmp) emit_record(d, io, c, e, u2.srt.fj[j], 6 * k + cmp, a[cmp], true);
+        for (int cmp = 0; cmp < 6; ++cmp) emit_record(d, io, c, e, face, 6 * k + cmp, a[cmp], true);
       }
     }
     return;
@@ -1549,10 +1668,12 @@ __global__ __launch_bounds__(kB) void face_sum_kernel(Dev d) {
 //     vertex with no silhouette gradient in any view returns at once.
 template <bool SIL>
 __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
+#pragma clang fp contract(fast)
   __shared__ float s_tab[256 * 8];
   const CharD c = d.ch[blockIdx.y];
   if ((long long)blockIdx.x * kB >= c.N) return;  // no vertices here (uniform over the block)
-  for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
+  if (!io.skA)
+    for (int i = threadIdx.x; i < c.J * 8; i += kB) s_tab[i] = io.dq[(size_t)c.j0 * 8 + i];
   __syncthreads();
   const int v = blockIdx.x * kB + threadIdx.x;
   if (v >= c.N) return;
@@ -1589,6 +1710,21 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
       g = g + mk3(E[0] * g0 + E[4] * g1 + E[8] * g2, E[1] * g0 + E[5] * g1 + E[9] * g2, E[2] * g0 + E[6] * g1 + E[10] * g2);
     }
     if (!any) return;
+  }
+  if (io.skA) {
+    // graph skinning: the outer sum's adjoint w.r.t. v_nr, sum_s w_s R_s^T g
+    f3 u = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const float* r = d.node_skin + ((size_t)c.m0 + d.enode[(size_t)gv * KE + k]) * 12;
+      m3 R;
+      for (int q = 0; q < 9; ++q) R.m[q] = r[q];
+      u += d.ew[(size_t)gv * KE + k] * mulT(R, g);
+    }
+    float* up = d.u + 3 * (size_t)gv;
+    if (!SIL) { up[0] = u.x; up[1] = u.y; up[2] = u.z; }
+    else { up[0] += u.x; up[1] += u.y; up[2] += u.z; }
+    return;
   }
   const f3 vnr = ed_inner(d, c, gv);
   int idx[KS];
@@ -1637,6 +1773,7 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
 // vertex counts are.  Pass 1: one wavefront per chunk, lane-strided then
 // reduced across the wavefront.
 __global__ __launch_bounds__(kB) void joint_partial_kernel(Dev d, int nChunk) {
+#pragma clang fp contract(fast)
   const int ck = (blockIdx.x * kB + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (ck >= nChunk) return;
@@ -1673,6 +1810,7 @@ __global__ __launch_bounds__(kB) void joint_reduce_kernel(Dev d, IO io, int totJ
 }
 
 __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
+#pragma clang fp contract(fast)
   const int node = (blockIdx.x * kB + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (node >= totM) return;
@@ -1785,7 +1923,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   p->B = n_chars;
   std::vector<CharD> ch(n_chars);
   std::vector<ViewD> vw;
-  std::vector<float> rest, jw, ew;
+  std::vector<float> rest, jw, ew, tcv;
   std::vector<int> faces, jidx, enode, node_vtx, n2v_off(1, 0), n2v_vtx, vf_off(1, 0), vf_list, jv_off(1, 0), jv_ent;
   std::vector<int> corner_slot;
   std::vector<int> jc_off(1, 0);
@@ -1806,6 +1944,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
     d.N = c.num_vertices; d.F = c.num_faces; d.M = c.num_nodes; d.J = c.num_joints;
     d.v0 = p->sumN; d.f0 = p->sumF; d.m0 = p->sumM; d.j0 = p->sumJ;
     d.view0 = (int)vw.size(); d.C = c.num_cameras;
+    d.has_tc = c.texcoords != nullptr;
+    for (int i = 0; i < 6 * d.F; ++i) tcv.push_back(c.texcoords ? c.texcoords[i] : 0.f);
     for (int i = 0; i < 3 * d.N; ++i) rest.push_back(c.rest_pos[i]);
     for (int i = 0; i < 3 * d.F; ++i) {
       if (c.faces[i] < 0 || c.faces[i] >= d.N) { delete p; return set_error(DCX_ERR_INVALID, "character %d: bad face index", b); }
@@ -1946,14 +2086,19 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   upload(&djvo, jv_off); upload(&djve, jv_ent); upload(&dcsl, corner_slot);
   int4* djck; int* djco;
   upload(&djck, jchunk); upload(&djco, jc_off);
+  float* dtc;
+  upload(&dtc, tcv);
+  d.tc = dtc;
   d.jchunk = djck; d.jc_off = djco;
   p->nChunk = (int)jchunk.size();
   d.ch = dch; d.vw = dvw; d.rest = drest; d.faces = dfaces; d.jidx = djidx; d.jw = djw;
   d.enode = denode; d.ew = dew; d.node_vtx = dnv; d.n2v_off = dn2o; d.n2v_vtx = dn2v; d.n2v_w = dn2w;
   d.vf_off = dvfo; d.vf_list = dvfl; d.jv_off = djvo; d.jv_ent = djve; d.corner_slot = dcsl;
   alloc(&d.node_tab, sizeof(float) * kNodeRow * (size_t)p->sumM);
+  alloc(&d.node_skin, sizeof(float) * 12 * (size_t)p->sumM);
   alloc(&d.vdef, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.vnrm, sizeof(float) * 3 * (size_t)p->sumN);
+  alloc(&d.vsum, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.proj, sizeof(f4) * (size_t)vn);
   alloc(&d.ke, sizeof(float) * 12 * (size_t)p->V);
   alloc(&d.nrm2, sizeof(f2) * (size_t)vn);
@@ -2022,14 +2167,17 @@ constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 // and ev[kNumStages] after the last one.
 static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipStream_t s,
                                hipEvent_t* ev = nullptr) {
-  DCX_REQUIRE(in->dq && in->delta_t && in->delta_a && in->colors && in->extrinsics && in->intrinsics && in->target &&
-                  in->dt && in->crops && in->face && in->bary && in->render && in->loss && in->g_dq &&
-                  in->g_delta_t && in->g_delta_a && in->g_colors,
+  DCX_REQUIRE((in->skinned_t == nullptr) == (in->skinned_a == nullptr),
+              "dcx_pipeline: skinned_t and skinned_a are both given or both NULL");
+  const bool graph_skin = in->skinned_a != nullptr;
+  DCX_REQUIRE((graph_skin || (in->dq && in->g_dq)) && in->delta_t && in->delta_a && in->colors && in->extrinsics &&
+                  in->intrinsics && in->target && in->dt && in->crops && in->face && in->bary && in->render &&
+                  in->loss && in->g_delta_t && in->g_delta_a && in->g_colors,
               "dcx_pipeline: null buffer in io");
   Dev d = p->d;
   IO io{in->dq, in->delta_t, in->delta_a, in->colors, in->extrinsics, in->intrinsics, in->target, in->dt, in->crops,
         in->w_color, in->w_sil, in->face, in->bary, in->render, in->vertices, in->loss, in->g_dq, in->g_delta_t,
-        in->g_delta_a, in->g_colors};
+        in->g_delta_a, in->g_colors, in->uv, in->normal, in->skinned_t, in->skinned_a};
   ClearList L{};
   auto add = [&](void* ptr, long long n) { L.p[L.count] = (unsigned int*)ptr; L.n[L.count] = n; L.count++; };
   add(d.g_v, 3ll * p->sumN);
@@ -2090,11 +2238,21 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   dcx_status st = DCX_OK;
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
+  // the normal buffer reads the side stream's vertex normals
+  const bool njoin_early = fork && io.nrm_out;
+  if (njoin_early) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
   {
     const dim3 grid((unsigned)(ceil_div(d.maxTiles, kXcd) * kXcd), V);  // x extent a multiple of kXcd
-    if (d.tws == 4) hipLaunchKernelGGL(fine_kernel<4>, grid, dim3(kTilePix), 0, s, d, io);
-    else if (d.tws == 6) hipLaunchKernelGGL(fine_kernel<6>, grid, dim3(kTilePix), 0, s, d, io);
-    else hipLaunchKernelGGL(fine_kernel<5>, grid, dim3(kTilePix), 0, s, d, io);
+    const dim3 blk(kTilePix);
+    if (io.uv || io.nrm_out) {
+      if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, true>), grid, blk, 0, s, d, io);
+      else if (d.tws == 6) hipLaunchKernelGGL((fine_kernel<6, true>), grid, blk, 0, s, d, io);
+      else hipLaunchKernelGGL((fine_kernel<5, true>), grid, blk, 0, s, d, io);
+    } else {
+      if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, false>), grid, blk, 0, s, d, io);
+      else if (d.tws == 6) hipLaunchKernelGGL((fine_kernel<6, false>), grid, blk, 0, s, d, io);
+      else hipLaunchKernelGGL((fine_kernel<5, false>), grid, blk, 0, s, d, io);
+    }
   }
   mark();
   if (fork) {
@@ -2104,7 +2262,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, p->side, d, io);
     DCX_HIP(hipEventRecord(p->ev_join, p->side));
   }
-  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
+  if (fork && !njoin_early) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
   hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
   mark();
   st = scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, s);
@@ -2132,10 +2290,14 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     if (dcx_status e = to_side()) return e;
     sj = p->side;
   }
-  hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, sj, d,
-                     p->nChunk);
+  // (graph skinning has no joint DQs: the two stages stay empty)
+  if (!graph_skin)
+    hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, sj, d,
+                       p->nChunk);
   mark();
-  hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, sj, d, io, p->sumJ);
+  if (!graph_skin)
+    hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, sj, d, io,
+                       p->sumJ);
   if (fork) DCX_HIP(hipEventRecord(p->ev_join, p->side));
   mark();
   hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);

@@ -16,10 +16,12 @@ Two executions of the same mathematics:
 
 Per character the step computes
   v_nr = sum_n w_n (R(dA_n)(v - g_n) + g_n + dT_n)          (ED, Euler angles)
-  v'   = DQS(v_nr; joint DQs, 4 joints per vertex)
+  v'   = DQS(v_nr; joint DQs, 4 joints per vertex), or, with the product's
+         skinning chain (skinned node transforms from Pose2EmbeddedGraph),
+         sum_s w_s (R(sA_s) v_nr + sT_s)                      (ED outer sum)
   L    = w_col * sum |render - target|_1 + w_sil * (sum res^2 + sum res1^2)
-and its gradients w.r.t. the joint DQs, the ED node parameters and the vertex
-colours.  Workloads are synthetic (scenes.py); targets are rendered from a
+and its gradients w.r.t. the joint DQs (DQ skinning only), the ED node
+parameters and the vertex colours.  Workloads are synthetic (scenes.py); targets are rendered from a
 perturbed pose and their DT images are encoded as uint8 with value 127 on the
 silhouette contour, the distance inside (< 127) and 127 + distance / 2 outside.
 """
@@ -35,12 +37,18 @@ from . import _lib, scenes
 from ._torch import ptr, stream_ptr
 
 
+def opt(t):
+    """Device pointer of an optional tensor (None for None)."""
+    return None if t is None else ptr(t)
+
+
 class CharDesc(ctypes.Structure):
     _fields_ = [("num_vertices", ctypes.c_int), ("num_faces", ctypes.c_int), ("num_nodes", ctypes.c_int),
                 ("num_joints", ctypes.c_int), ("num_cameras", ctypes.c_int), ("width", ctypes.c_int),
                 ("height", ctypes.c_int), ("dt_size", ctypes.c_int), ("rest_pos", ctypes.c_void_p),
                 ("faces", ctypes.c_void_p), ("joint_idx", ctypes.c_void_p), ("joint_w", ctypes.c_void_p),
-                ("node_vertex", ctypes.c_void_p), ("node_idx", ctypes.c_void_p), ("node_w", ctypes.c_void_p)]
+                ("node_vertex", ctypes.c_void_p), ("node_idx", ctypes.c_void_p), ("node_w", ctypes.c_void_p),
+                ("texcoords", ctypes.c_void_p)]
 
 
 class PipelineIO(ctypes.Structure):
@@ -50,7 +58,8 @@ class PipelineIO(ctypes.Structure):
                 ("w_color", ctypes.c_float), ("w_sil", ctypes.c_float), ("face", ctypes.c_void_p),
                 ("bary", ctypes.c_void_p), ("render", ctypes.c_void_p), ("vertices", ctypes.c_void_p),
                 ("loss", ctypes.c_void_p), ("g_dq", ctypes.c_void_p), ("g_delta_t", ctypes.c_void_p),
-                ("g_delta_a", ctypes.c_void_p), ("g_colors", ctypes.c_void_p)]
+                ("g_delta_a", ctypes.c_void_p), ("g_colors", ctypes.c_void_p), ("uv", ctypes.c_void_p),
+                ("normal", ctypes.c_void_p), ("skinned_t", ctypes.c_void_p), ("skinned_a", ctypes.c_void_p)]
 
 
 @dataclass
@@ -67,6 +76,15 @@ class CharacterWork:
     target: np.ndarray | None = None   # [C,H,W,3]
     dt: np.ndarray | None = None       # [C,S,S] uint8
     crops: np.ndarray | None = None    # [C,7]
+    # the product's skinning chain (Pose2EmbeddedGraphCpu -> EmbeddedGraphGpu,
+    # HPCMeshDL.py:932-943): skinned node translations / Euler angles [M,3]
+    # that replace the dual-quaternion skinning (dq is then unused)
+    skinned_t: np.ndarray | None = None
+    skinned_a: np.ndarray | None = None
+
+    @property
+    def graph_skinning(self) -> bool:
+        return self.skinned_t is not None
 
 
 def node_arrays(g: scenes.EmbeddedGraph, N: int, K: int = 4):
@@ -176,12 +194,24 @@ def check_works(works: list[CharacterWork]) -> None:
             raise ValueError(f"character {i}: unsupported sizes J={ch.J} C={C} S={wk.S} W={wk.W} H={wk.H}")
         if ch.faces.min() < 0 or ch.faces.max() >= ch.N or ch.joint_idx.min() < 0 or ch.joint_idx.max() >= ch.J:
             raise ValueError(f"character {i}: face or joint index out of range")
+        if (wk.skinned_t is None) != (wk.skinned_a is None):
+            raise ValueError(f"character {i}: skinned_t and skinned_a are both given or both None")
+        if wk.graph_skinning:
+            for name, a in (("skinned_t", wk.skinned_t), ("skinned_a", wk.skinned_a)):
+                if tuple(np.shape(a)) != (M, 3):
+                    raise ValueError(f"character {i}: {name} has shape {np.shape(a)}, expected {(M, 3)}")
+    if len({wk.graph_skinning for wk in works}) != 1:
+        raise ValueError("graph skinning (skinned_t / skinned_a) is given for some characters only")
 
 
 class BatchedPipeline:
     """Fused step over all characters (dcx_pipeline_*)."""
 
-    def __init__(self, works: list[CharacterWork], device="cuda", w_color=None, w_sil=1.0, bin_capacity=0):
+    def __init__(self, works: list[CharacterWork], device="cuda", w_color=None, w_sil=1.0, bin_capacity=0,
+                 uv_buffer=False, normal_buffer=False):
+        """uv_buffer / normal_buffer: also write CudaRendererGpu's per-pixel
+        'uv' ([pixels,2], from each character's texcoords [F,3,2]; zeros for
+        characters without) and 'normal' ([pixels,3]) buffers in the step."""
         check_works(works)
         self.works = works
         self.device = torch.device(device)
@@ -193,8 +223,12 @@ class BatchedPipeline:
             arrs = [np.ascontiguousarray(ch.verts, np.float32), np.ascontiguousarray(ch.faces, np.int32),
                     np.ascontiguousarray(ch.joint_idx, np.int32), np.ascontiguousarray(ch.joint_w, np.float32),
                     np.ascontiguousarray(ch.graph.node_vertex, np.int32), nidx, nw]
+            if uv_buffer and ch.texcoords is not None:
+                arrs.append(np.ascontiguousarray(np.asarray(ch.texcoords).reshape(ch.F, 6), np.float32))
             keep.append(arrs)
             p = [a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
+            if len(p) == 7:
+                p.append(None)
             descs[i] = CharDesc(ch.N, ch.F, ch.graph.num_nodes, ch.J, wk.ext.shape[0], wk.W, wk.H, wk.S, *p)
         lib = _lib.load()
         h = ctypes.c_void_p()
@@ -208,6 +242,9 @@ class BatchedPipeline:
         d = self.device
         t = lambda arrs, dt=torch.float32: torch.as_tensor(np.concatenate(arrs), dtype=dt, device=d).contiguous()
         self.dq = t([wk.dq for wk in works])
+        self.graph_skinning = works[0].graph_skinning
+        self.skinned_t = t([wk.skinned_t for wk in works]) if self.graph_skinning else None
+        self.skinned_a = t([wk.skinned_a for wk in works]) if self.graph_skinning else None
         self.dT = t([wk.dT for wk in works])
         self.dA = t([wk.dA for wk in works])
         self.colors = t([wk.ch.colors for wk in works])
@@ -229,16 +266,20 @@ class BatchedPipeline:
         self.render = torch.empty((npix, 3), device=d)
         self.vertices = torch.empty((self.sizes[2], 3), device=d)
         self.loss = torch.empty((len(works), 2), device=d)
-        self.g_dq = torch.empty_like(self.dq)
+        # graph skinning: no joint-DQ gradient (the step does not write one)
+        self.g_dq = None if self.graph_skinning else torch.empty_like(self.dq)
         self.g_dT = torch.empty_like(self.dT)
         self.g_dA = torch.empty_like(self.dA)
         self.g_colors = torch.empty_like(self.colors)
+        self.uv = torch.empty((npix, 2), device=d) if uv_buffer else None
+        self.normal = torch.empty((npix, 3), device=d) if normal_buffer else None
         self.w_color = float(1.0 / npix) if w_color is None else w_color
         self.w_sil = w_sil
         self.io = PipelineIO(ptr(self.dq), ptr(self.dT), ptr(self.dA), ptr(self.colors), ptr(self.ext), ptr(self.intr),
                              ptr(self.target), ptr(self.dt), ptr(self.crops), self.w_color, self.w_sil,
                              ptr(self.face), ptr(self.bary), ptr(self.render), ptr(self.vertices), ptr(self.loss),
-                             ptr(self.g_dq), ptr(self.g_dT), ptr(self.g_dA), ptr(self.g_colors))
+                             opt(self.g_dq), ptr(self.g_dT), ptr(self.g_dA), ptr(self.g_colors),
+                             opt(self.uv), opt(self.normal), opt(self.skinned_t), opt(self.skinned_a))
         self.captured_stream = None
 
     def step(self):

@@ -521,6 +521,7 @@ typedef struct {
   const int32_t* node_vertex; /* [M]  base-mesh vertex of each ED node */
   const int32_t* node_idx;    /* [N,4] ED nodes of each vertex */
   const float* node_w;        /* [N,4] */
+  const float* texcoords;     /* [F,6] (u, v per corner) or NULL: this character's UV buffer stays 0 */
 } dcx_char_desc;
 
 typedef struct {
@@ -549,6 +550,24 @@ typedef struct {
   float* g_delta_t;         /* [sum M,3] */
   float* g_delta_a;         /* [sum M,3] */
   float* g_colors;          /* [sum N,3] */
+  /* optional DEVICE outputs (NULL: not written), the per-pixel buffers of
+   * CudaRendererGpu's 'uv' and 'normal' albedo modes (CUDABasedRasterization.cu:
+   * 380-446; HPCMeshDL.py:1063-1146 renders them) at the step's vertices: */
+  float* uv;                /* [sum H*W,2] interpolated (u, 1 - v), clamped to [0,1]; 0 off the mesh */
+  float* normal;            /* [sum H*W,3] (1 + n) / 2 of the interpolated vertex normal, normalised and
+                               turned toward the camera; 0 off the mesh */
+  /* optional DEVICE inputs, the product's skinning chain (HPCMeshDL.py:932-943,
+   * 1015-1033: Pose2EmbeddedGraphCpu's skinned node transforms fed to
+   * EmbeddedGraphGpu): both NULL, or both given.  Given, the step deforms
+   * with the embedded graph's outer sum (EmbeddedGraphGPUOp.cu:100-114) in
+   * place of the dual-quaternion skinning, vertex = sum over the vertex's
+   * nodes s of w_s (R(skinned_a_s) v_nr + skinned_t_s); dq is then not read
+   * and g_dq not written (both may be NULL).  No gradient flows to the
+   * skinned transforms (Pose2EmbeddedGraphCpu's is zero,
+   * Pose2EmbeddedGraphCpu.py:55-60).  dcx_pose_to_graph writes them from the
+   * skinning joints' dual quaternions. */
+  const float* skinned_t;   /* [sum M,3] */
+  const float* skinned_a;   /* [sum M,3] Euler angles */
 } dcx_pipeline_io;
 
 /* bin_capacity <= 0 selects 4 tile entries per (view, face). */

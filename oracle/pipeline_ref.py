@@ -16,6 +16,10 @@ from . import oracle as orc
 def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None, fwd_dtype=np.float32, colors=None):
     """Returns dict(loss=[colour, silhouette], g_dq, g_dT, g_dA, g_col, face, render, pos).
 
+    A work with skinned_t / skinned_a runs the product's skinning chain: the
+    embedded graph with those skinned node transforms is the deformation (no
+    dual-quaternion skinning; g_dq is None).
+
     positions: optional skinned positions [N,3] at which the raster, losses
     and their adjoints are evaluated instead of the oracle's own skinning
     output (so that discrete decisions match a given producer exactly).
@@ -33,8 +37,16 @@ def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None, fwd_d
     # the forward runs in float32 like the product so that the discrete
     # decisions (tri ids, boundary flags, nearest points) see the same inputs
     fd = fwd_dtype
-    v_nr = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], zM, zM, np.zeros((1, N, 3)), dtype=fd)[0]
-    pos, _ = orc.dqs_fwd(wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=fd)
+    graph_skin = getattr(wk, "skinned_t", None) is not None
+    if graph_skin:
+        # the product's chain: the embedded graph with the skinned node
+        # transforms (its outer sum, EmbeddedGraphGPUOp.cu:100-114) is the
+        # whole deformation (HPCMeshDL.py:932-943)
+        sT, sA = wk.skinned_t[None], wk.skinned_a[None]
+        pos = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], sT, sA, np.zeros((1, N, 3)), dtype=fd)[0]
+    else:
+        v_nr = orc.ed_fwd(g, ch.verts, wk.dT[None], wk.dA[None], zM, zM, np.zeros((1, N, 3)), dtype=fd)[0]
+        pos, _ = orc.dqs_fwd(wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=fd)
     if positions is not None:
         pos = np.asarray(positions, fd).reshape(pos.shape)
     face, bary, render, vn = orc.raster_fwd(ch.faces, pos.astype(fd), ext, intr, wk.H, wk.W,
@@ -55,9 +67,14 @@ def step(wk, w_col: float, w_sil: float, dtype=np.float64, positions=None, fwd_d
     g_pos_r, g_col, _, _ = orc.raster_bwd(ch.faces, g_render, pos, ext, intr, face, bary, vn, vcol=col[None],
                                           target=wk.target[None], dtype=dtype)
     g_pos = g_pos + g_pos_r
-    g_dq, _, g_rest = orc.dqs_bwd(g_pos, wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=dtype)
-    gT, gA, _, _, _ = orc.ed_bwd(g, ch.verts, g_rest, wk.dA[None], zM, dtype=dtype)
-    return dict(loss=np.array([loss_col, loss_sil]), g_dq=g_dq[0], g_dT=gT[0], g_dA=gA[0], g_col=g_col[0],
+    if graph_skin:
+        g_dq = None
+        gT, gA, _, _, _ = orc.ed_bwd(g, ch.verts, g_pos, wk.dA[None], sA, dtype=dtype)
+    else:
+        g_dq, _, g_rest = orc.dqs_bwd(g_pos, wk.dq[None], ch.joint_w[None], ch.joint_idx, v_nr[0], dtype=dtype)
+        g_dq = g_dq[0]
+        gT, gA, _, _, _ = orc.ed_bwd(g, ch.verts, g_rest, wk.dA[None], zM, dtype=dtype)
+    return dict(loss=np.array([loss_col, loss_sil]), g_dq=g_dq, g_dT=gT[0], g_dA=gA[0], g_col=g_col[0],
                 face=face[0], render=render[0], pos=pos[0])
 
 

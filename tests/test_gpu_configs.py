@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from deepcharacters_amd import pipeline, renderer, scenes, skinning
+from deepcharacters_amd import pipeline, renderer, scenes, skinning, workloads
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -32,21 +32,12 @@ def _rel(a, b):
 
 
 def test_plain_forward_workload_bit_exact():
-    rng = np.random.default_rng(0)
-    ch = scenes.ellipsoid_character(5, scale=(1.0, 0.4, 0.3), n_joints=24, K=4, seed=0)
+    ch, T, w, disp, ext, intr, W, H = workloads.plain_forward_scene()
     assert (ch.N, ch.F, ch.J) == (10242, 20480, 24)
-    dq = scenes.random_joint_dqs(rng, ch.joint_centres, 1, 0.5, 0.02)[0]
-    qr, qd = dq[:, :4], dq[:, 4:]
-    R = scenes.quat_to_mat(qr)
-    t = 2 * scenes.quat_mul(qd, qr * np.array([-1, -1, -1, 1]))[:, :3]
-    T = np.concatenate([R.reshape(-1, 9), t], 1)[None].astype(np.float32)          # [1, J, 12]
-    w = ch.joint_w[None].astype(np.float32)
-    disp = np.zeros((1, ch.N, 3), np.float32)
     pos, _, _, _ = skinning.lb_skinning_gpu(_t(T), _t(w), _t(ch.joint_idx, torch.int32), _t(ch.verts),
                                             _t(ch.normals), _t(disp))
     rpos, _ = orc.lbs_fwd(T, w, ch.joint_idx, ch.verts, ch.normals, disp)
     assert _rel(pos.cpu().numpy(), rpos) < 1e-6
-    ext, intr = scenes.ring_cameras(4, 3.0, 1000.0, 512, 512)
     np.testing.assert_array_equal(intr[:, [2, 5]], 256.0)
     topo = renderer.Topology(ch.faces, ch.N)
     cfg = renderer.make_cfg(4, 512, 512)
@@ -63,19 +54,13 @@ def test_plain_forward_workload_bit_exact():
 
 def test_single_character_fwd_bwd_workload_vs_fp64_oracle():
     from oracle import pipeline_ref
-    rng = np.random.default_rng(1)
-    ch = scenes.capsule_body(25000, seed=1, ed_nodes=500)
+    wk, skel, dofs = workloads.single_character_work()
+    ch = wk.ch
     assert 23000 < ch.N < 27000 and ch.graph.num_nodes == 500
-    skel = scenes.body_skeleton()
     assert skel["num_dofs"] == 57
-    dofs = scenes.random_dofs(rng, skel, 1, 0.3, 0.02)
-    dq = orc.fk(skel, dofs)[0][0].astype(np.float32)                                 # [18, 8] from the 57 DOFs
-    _, dT, dA = pipeline.pose_params(rng, ch)
-    ext, intr = scenes.ring_cameras(4, 3.0, 2000.0, 1024, 1024, phase=0.4)
-    E = ext.reshape(4, 3, 4).copy()
-    E[:, :, 3] -= E[:, :, :3] @ np.array([0.0, 0.9, 0.0])
-    wk = pipeline.CharacterWork(ch, dq, dT, dA, E.reshape(4, 12), intr, 1024, 1024, 256)
-    pipeline.make_targets([wk], pipeline.gpu_pose_renderer(), np.random.default_rng(2))
+    # the joint DQs came from the GPU forward kinematics: they match the oracle's
+    dq_ref = orc.fk(skel, dofs)[0][0]
+    assert np.allclose(np.abs(np.sum(wk.dq[:, :4] * dq_ref[:, :4], 1)), 1.0, atol=1e-5)
     bp = pipeline.BatchedPipeline([wk])
     bp.step()
     torch.cuda.synchronize()

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from deepcharacters_amd import pipeline, renderer
+from deepcharacters_amd import pipeline, renderer, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -25,8 +25,7 @@ LOSS_BAR, GRAD_BAR = 1e-5, 1e-4
 
 @pytest.fixture(scope="module")
 def headline():
-    works = pipeline.build_works(16, seed=2)
-    pipeline.make_targets(works, pipeline.gpu_pose_renderer(), np.random.default_rng(2))
+    works = workloads.batched_works()
     bp = pipeline.BatchedPipeline(works)
     bp.step()
     torch.cuda.synchronize()

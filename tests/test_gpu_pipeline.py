@@ -1,9 +1,11 @@
 """Batched fused pipeline vs the plain op-by-op pipeline and the op-level renderer."""
+import copy
+
 import numpy as np
 import pytest
 import torch
 
-from deepcharacters_amd import pipeline, renderer
+from deepcharacters_amd import pipeline, renderer, skinning
 
 pytestmark = pytest.mark.gpu
 
@@ -48,7 +50,8 @@ def _split(bp, works):
     j0 = m0 = v0 = 0
     for b, wk in enumerate(works):
         J, M, N = wk.ch.J, wk.ch.graph.num_nodes, wk.ch.N
-        yield b, wk, dict(loss=bp.loss[b], g_dq=bp.g_dq[j0:j0 + J], g_dT=bp.g_dT[m0:m0 + M],
+        yield b, wk, dict(loss=bp.loss[b], g_dq=None if bp.g_dq is None else bp.g_dq[j0:j0 + J],
+                          g_dT=bp.g_dT[m0:m0 + M],
                           g_dA=bp.g_dA[m0:m0 + M], g_col=bp.g_colors[v0:v0 + N], pos=bp.vertices[v0:v0 + N])
         j0 += J; m0 += M; v0 += N
 
@@ -264,3 +267,112 @@ def test_set_tile_waits_for_queued_replays(works):
             assert torch.equal(res[k], want[k]), k
         for k in range(3, 8):
             assert _rel(res[k], want[k]) < 1e-5, k
+
+
+@pytest.mark.parametrize("tile_w", [32, 16])
+def test_uv_and_normal_buffers_match_op_renderer(works, tile_w):
+    """The step's optional per-pixel buffers -- CudaRendererGpu's 'uv' and
+    'normal' albedo modes -- bit-exact against the op renderer at the step's
+    vertices (the first character without texture coordinates gets a zero UV
+    buffer); the step's other outputs are those of the step without them."""
+    from deepcharacters_amd import scenes
+    tcs = [None] + [scenes.uv_atlas(wk.ch.faces) for wk in works[1:]]
+    saved = [wk.ch.texcoords for wk in works]
+    try:
+        for wk, tc in zip(works, tcs):
+            wk.ch.texcoords = tc
+        bp = pipeline.BatchedPipeline(works, uv_buffer=True, normal_buffer=True)
+        ref = pipeline.BatchedPipeline(works)
+    finally:
+        for wk, tc in zip(works, saved):
+            wk.ch.texcoords = tc
+    bp.set_tile(tile_w)
+    ref.set_tile(tile_w)
+    bp.step()
+    ref.step()
+    torch.cuda.synchronize()
+    for a, b in ((bp.face, ref.face), (bp.bary, ref.bary), (bp.render, ref.render)):
+        assert torch.equal(a, b)
+    for a, b in ((bp.loss, ref.loss), (bp.g_colors, ref.g_colors), (bp.g_dq, ref.g_dq), (bp.g_dT, ref.g_dT)):
+        assert _rel(a, b) < 1e-5
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float32, device="cuda")
+    v0 = p0 = 0
+    for wk, tc in zip(works, tcs):
+        N, C = wk.ch.N, wk.ext.shape[0]
+        n = C * wk.W * wk.H
+        pos = bp.vertices[v0:v0 + N][None].contiguous()
+        topo = renderer.Topology(wk.ch.faces, N, None if tc is None else tc.reshape(-1, 6))
+        zero = torch.zeros_like(pos)
+        cfg_n = renderer.make_cfg(C, wk.W, wk.H, albedo="normal")
+        _, face, nrm, _ = renderer.render_forward(topo, cfg_n, pos, zero, None, None, t(wk.ext.reshape(1, -1)),
+                                                  t(wk.intr.reshape(1, -1)))
+        assert torch.equal(bp.face[p0:p0 + n], face.reshape(-1))
+        assert torch.equal(bp.normal[p0:p0 + n], nrm.reshape(-1, 3))
+        if tc is None:
+            assert not torch.any(bp.uv[p0:p0 + n])
+        else:
+            cfg_u = renderer.make_cfg(C, wk.W, wk.H, albedo="uv")
+            _, _, uv, _ = renderer.render_forward(topo, cfg_u, pos, zero, None, None, t(wk.ext.reshape(1, -1)),
+                                                  t(wk.intr.reshape(1, -1)))
+            assert torch.equal(bp.uv[p0:p0 + n], uv.reshape(-1, 3)[:, :2])
+            assert (bp.uv[p0:p0 + n] > 0).any()
+        v0 += N
+        p0 += n
+
+
+@pytest.fixture(scope="module")
+def graph_works(works):
+    """The same characters under the product's skinning chain: each one's
+    joint DQs -> Pose2EmbeddedGraph on the GPU -> skinned node transforms
+    that the step's embedded graph applies in place of the DQ skinning."""
+    out = []
+    for wk in works:
+        ch = wk.ch
+        dev = lambda a, dt=torch.float32: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+        t, a = skinning.pose_to_embedded_graph(dev(wk.dq[None]), dev(ch.joint_idx, torch.int32), dev(ch.joint_w),
+                                               dev(ch.graph.node_vertex, torch.int32))
+        w = copy.copy(wk)
+        w.skinned_t = t[0].cpu().numpy()
+        w.skinned_a = a[0].cpu().numpy()
+        out.append(w)
+    return out
+
+
+def test_graph_skinning_step_matches_cpu_oracle(graph_works):
+    """The fused step with skinned node transforms (EmbeddedGraphGpu's outer
+    sum, no DQ skinning) against the oracle: vertices against the oracle's
+    embedded graph, then losses and gradients evaluated by the oracle at the
+    GPU's vertices; no joint-DQ gradient is written."""
+    from oracle import pipeline_ref
+    bp = pipeline.BatchedPipeline(graph_works)
+    assert bp.graph_skinning and bp.g_dq is None
+    bp.step()
+    torch.cuda.synchronize()
+    assert float(bp.loss[:, 0].sum()) > 0 and float(bp.loss[:, 1].sum()) > 0
+    for b, wk, got in _split(bp, graph_works):
+        own = pipeline_ref.step(wk, bp.w_color, bp.w_sil)
+        assert own["g_dq"] is None
+        assert _rel(got["pos"], torch.as_tensor(own["pos"])) < 1e-6
+        ref = pipeline_ref.step(wk, bp.w_color, bp.w_sil, positions=got["pos"].cpu().numpy())
+        t = lambda a: torch.as_tensor(a)
+        assert _rel(got["loss"], t(ref["loss"])) < 1e-5
+        assert _rel(got["g_col"], t(ref["g_col"])) < 1e-4
+        for key in ("g_dT", "g_dA"):
+            assert _rel(got[key], t(ref[key])) < 1e-4, key
+
+
+def test_graph_skinning_replay_matches_eager(graph_works):
+    bp = pipeline.BatchedPipeline(graph_works)
+    bp.step()
+    torch.cuda.synchronize()
+    ref = [x.clone() for x in (bp.face, bp.render, bp.loss, bp.g_dT, bp.g_dA, bp.g_colors)]
+    s = torch.cuda.Stream()
+    bp.capture(s)
+    for rep in range(2):
+        bp.replay()
+        s.synchronize()
+        torch.cuda.synchronize()
+        got = (bp.face, bp.render, bp.loss, bp.g_dT, bp.g_dA, bp.g_colors)
+        assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1]), rep
+        for k in range(2, 6):
+            assert _rel(got[k], ref[k]) < 1e-4, (rep, k)

@@ -10,49 +10,14 @@ import numpy as np
 import pytest
 import torch
 
-from deepcharacters_amd import pipeline, renderer, scenes
+from deepcharacters_amd import pipeline, renderer, workloads
 
 pytestmark = pytest.mark.gpu
 
 
-def _fast_graph(ch, M, rng):
-    """An embedded graph on a random vertex subset (4 nearest nodes, Gaussian
-    weights): node placement does not matter for a scale test, and farthest-
-    point sampling of 64 large meshes would dominate the test's time."""
-    from scipy.spatial import cKDTree
-    nodes = np.sort(rng.choice(ch.N, M, replace=False)).astype(np.int32)
-    d, idx = cKDTree(ch.verts[nodes]).query(ch.verts, k=4)
-    w = np.exp(-d ** 2 / (2 * 0.05 ** 2))
-    bad = w.sum(1) <= 1e-12
-    w[bad] = 0.0
-    w[bad, 0] = 1.0
-    w /= w.sum(1, keepdims=True)
-    return scenes.EmbeddedGraph(nodes, np.arange(0, 4 * ch.N + 1, 4, dtype=np.int32),
-                                idx.reshape(-1).astype(np.int32), w.reshape(-1))
-
-
-def build_fullsize_works():
-    rng = np.random.default_rng(44)
-    nv = np.exp(rng.uniform(np.log(5000), np.log(100000), 64)).astype(int)
-    res = rng.choice([512, 1024, 2048], 64)
-    ws = []
-    for b in range(64):
-        ch = scenes.capsule_body(int(nv[b]), seed=900 + b, ed_nodes=0)
-        ch.graph = _fast_graph(ch, int(rng.uniform(500, 1500)), rng)
-        dq, dT, dA = pipeline.pose_params(rng, ch, max_angle=1.5 if b % 5 == 0 else 0.3)
-        W = H = int(res[b])
-        f = 2.0 * W  # the body spans about a third of the image at 3 m
-        ext, intr = scenes.ring_cameras(4, 3.0, f, W, H, phase=rng.uniform(0, 2 * np.pi))
-        E = ext.reshape(4, 3, 4).copy()
-        E[:, :, 3] -= E[:, :, :3] @ np.array([0.0, 0.9, 0.0])
-        ws.append(pipeline.CharacterWork(ch, dq, dT, dA, E.reshape(4, 12), intr, W, H, 256))
-    pipeline.make_targets(ws, pipeline.gpu_pose_renderer(), np.random.default_rng(5))
-    return ws
-
-
 @pytest.fixture(scope="module")
 def fullsize():
-    ws = build_fullsize_works()
+    ws = workloads.stress_works()
     bp = pipeline.BatchedPipeline(ws)
     tuned = bp.autotune(reps=1)
     bp.step()

@@ -356,3 +356,69 @@ def test_composed_step_fails_with_a_wrong_hand_off(tiny_work, monkeypatch):
     bad = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
     rel = np.linalg.norm(bad["g_dT"] - good["g_dT"]) / np.linalg.norm(good["g_dT"])
     assert rel > 1e-2, rel
+
+
+@pytest.fixture(scope="module")
+def tiny_graph_work(tiny_work):
+    """tiny_work under the product's skinning chain: per-node skinned
+    rotations (Euler angles up to ~0.15 rad) and translations."""
+    w = copy.copy(tiny_work)
+    rng = np.random.default_rng(5)
+    M = w.ch.graph.num_nodes
+    w.skinned_a = rng.uniform(-0.15, 0.15, (M, 3))
+    w.skinned_t = rng.normal(0, 0.01, (M, 3))
+    return w
+
+
+def test_graph_skinning_step_gradients_match_central_differences(tiny_graph_work):
+    """The same FP64 check for the product's chain (embedded graph with skinned
+    node transforms, no DQ skinning): d(colour loss)/d(node translations,
+    node rotations) through the outer sum's adjoint."""
+    wk = tiny_graph_work
+    ref = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
+    assert ref["g_dq"] is None
+    assert (ref["face"] >= 0).mean() > 0.1
+    rng = np.random.default_rng(2)
+    h = 1e-6
+    for name, grad in (("dT", ref["g_dT"]), ("dA", ref["g_dA"])):
+        base = getattr(wk, name)
+        scale = np.abs(grad).max()
+        assert scale > 0, name
+        order = rng.permutation(base.size)
+        order = np.concatenate([order[np.abs(grad.reshape(-1)[order]) > 1e-3 * scale],
+                                order[np.abs(grad.reshape(-1)[order]) <= 1e-3 * scale]])
+        n = 0
+        for i in order:
+            w = copy.copy(wk)
+            vals, faces = [], []
+            for sgn in (1.0, -1.0):
+                p = base.astype(np.float64).copy()
+                p.reshape(-1)[i] += sgn * h
+                setattr(w, name, p)
+                l, f = _loss(w)
+                vals.append(l)
+                faces.append(f)
+            if not (np.array_equal(faces[0], ref["face"]) and np.array_equal(faces[1], ref["face"])):
+                continue
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = float(grad.reshape(-1)[i])
+            assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-2 * scale), (name, int(i), fd, an)
+            n += 1
+            if n == 8:
+                break
+        assert n == 8, (name, n)
+
+
+def test_graph_skinning_step_fails_without_the_skinned_rotations(tiny_graph_work, monkeypatch):
+    """The check above sees the outer sum's adjoint: an ED backward that
+    ignores the skinned rotations (R_s^T g dropped) moves g_dT far off."""
+    wk = tiny_graph_work
+    good = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
+    real = orc.ed_bwd
+
+    def planted(graph, base, g_verts, dA, sA, *a, **k):
+        return real(graph, base, g_verts, dA, np.zeros_like(sA), *a, **k)
+    monkeypatch.setattr(orc, "ed_bwd", planted)
+    bad = pipeline_ref.step(wk, 1.0, 0.0, fwd_dtype=np.float64)
+    rel = np.linalg.norm(bad["g_dT"] - good["g_dT"]) / np.linalg.norm(good["g_dT"])
+    assert rel > 1e-2, rel

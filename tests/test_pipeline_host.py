@@ -59,6 +59,27 @@ def test_check_works_rejects_mismatched_shapes(small_works):
     pipeline.check_works(small_works)
 
 
+def test_check_works_graph_skinning_inputs(small_works):
+    """Skinned node transforms: [M,3] each, both or neither, and for every
+    character or none (the step takes one skinning mode for the batch)."""
+    import copy
+    ws = [copy.copy(w) for w in small_works]
+    M0 = ws[0].ch.graph.num_nodes
+    ws[0].skinned_t = np.zeros((M0, 3))
+    with pytest.raises(ValueError, match="both"):
+        pipeline.check_works(ws)
+    ws[0].skinned_a = np.zeros((M0 + 1, 3))
+    with pytest.raises(ValueError, match="skinned_a"):
+        pipeline.check_works(ws)
+    ws[0].skinned_a = np.zeros((M0, 3))
+    with pytest.raises(ValueError, match="some characters"):
+        pipeline.check_works(ws)
+    for w in ws[1:]:
+        M = w.ch.graph.num_nodes
+        w.skinned_t, w.skinned_a = np.zeros((M, 3)), np.zeros((M, 3))
+    pipeline.check_works(ws)
+
+
 def test_cpu_step_runs_and_is_finite(small_works):
     from oracle import pipeline_ref
     for wk in small_works:
