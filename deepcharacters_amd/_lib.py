@@ -27,6 +27,7 @@ SIGNATURES: dict[str, str] = {
     "dcx_topo_create": "piipp",
     "dcx_topo_destroy": "p",
     "dcx_render_fwd": "ppipppiippppppppzS",
+    "dcx_render_fwd_noalloc": "ppipppiipppppppS",
     "dcx_render_bwd": "ppippppiippppppppppppzS",
     "dcx_edgraph_create": "piippppipppp",
     "dcx_edgraph_destroy": "p",

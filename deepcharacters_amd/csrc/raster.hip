@@ -5,6 +5,7 @@
 // triangle setup + tile binning, then one workgroup per 32x8 screen tile
 // resolving depth in LDS with 64-bit ds_min on packed (depth, face) keys and
 // writing every output pixel exactly once, coalesced).
+#include <mutex>
 #include <vector>
 
 #include "raster_common.hpp"
@@ -153,6 +154,13 @@ __device__ __forceinline__ void resolve_pixel(const RenderArgs& a, int b, int vi
 __global__ __launch_bounds__(kB) void clear_keys_kernel(unsigned long long* keys, long long n) {
   const long long i = (long long)blockIdx.x * kB + threadIdx.x;
   if (i < n) keys[i] = kEmptyKey;
+}
+
+// Zero fill of a counter array (a kernel rather than a memset, so that a
+// caller may capture the forward into a graph).
+__global__ __launch_bounds__(kB) void clear_u32_kernel(unsigned int* a, long long n) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i < n) a[i] = 0u;
 }
 
 // One lane per (camera, face): walks the triangle's pixel range and folds
@@ -453,8 +461,8 @@ dcx_status render_fwd(const dcx_topo* t, const dcx_raster_cfg* c, int B, const f
   g.bins = (unsigned int*)(w + L.bins);
   g.capacity = (unsigned int)((ws_bytes - L.bins) / sizeof(unsigned int));
   const long long views = g.views;
-  DCX_HIP(hipMemsetAsync(g.tile_cnt, 0, sizeof(unsigned int) * L.tiles, s));
-  DCX_HIP(hipMemsetAsync(g.tile_fill, 0, sizeof(unsigned int) * L.tiles, s));
+  hipLaunchKernelGGL(clear_u32_kernel, dim3(ceil_div(L.tiles, kB)), dim3(kB), 0, s, g.tile_cnt, L.tiles);
+  hipLaunchKernelGGL(clear_u32_kernel, dim3(ceil_div(L.tiles, kB)), dim3(kB), 0, s, g.tile_fill, L.tiles);
   hipLaunchKernelGGL(project_kernel, dim3(ceil_div(views * N, kB)), dim3(kB), 0, s, vpos, ext, intr, N, C,
                      (int)views, a.proj);
   hipLaunchKernelGGL(face_normal_kernel, dim3(ceil_div((long long)B * F, kB)), dim3(kB), 0, s, vpos, t->faces,
@@ -499,6 +507,64 @@ extern "C" size_t dcx_render_workspace_size(const dcx_topo* t, const dcx_raster_
   size_t fixed = render_ws_fixed(t, c, B, nullptr);
   if (c->path == DCX_PATH_BINNED) fixed += sizeof(unsigned int) * default_bin_capacity(t, c, B);
   return fixed;
+}
+
+// ---------------------------------------------------------------------------
+// dcx_render_fwd_noalloc: the same forward with its scratch in a static
+// device array instead of a caller workspace.  Each device the library runs
+// on holds its own copy of the array for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in).  Every call on a
+// device shares that device's array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[DCX_RENDER_NOALLOC_SCRATCH_BYTES];
+
+namespace dcx {
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Address of the current device's scratch array (looked up once per device).
+dcx_status scratch_for_current_device(void** out) {
+  int dev = -1;
+  DCX_HIP(hipGetDevice(&dev));
+  DCX_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "dcx_render_fwd_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    DCX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    DCX_REQUIRE(ptr != nullptr, "dcx_render_fwd_noalloc: no scratch address on device %d", dev);
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return DCX_OK;
+}
+}  // namespace
+}  // namespace dcx
+
+extern "C" dcx_status dcx_render_fwd_noalloc(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,
+                                             const float* vertex_pos, const float* vertex_color,
+                                             const float* texture, int tex_h, int tex_w, const float* sh_coeff,
+                                             const float* extrinsics, const float* intrinsics,
+                                             float* barycentric, int32_t* face, float* render,
+                                             float* vertex_normal, void* stream) {
+  dcx_status st = validate_cfg(topo, cfg, B);
+  if (st) return st;
+  // the binned path needs at least kMinNoAllocBins tile-list entries beyond
+  // the fixed part (fewer would only send more tiles down the overflow path)
+  constexpr size_t kMinNoAllocBins = 65536;
+  const size_t fixed = render_ws_fixed(topo, cfg, B, nullptr) +
+                       (cfg->path == DCX_PATH_BINNED ? sizeof(unsigned int) * kMinNoAllocBins : 0);
+  DCX_REQUIRE(fixed <= (size_t)DCX_RENDER_NOALLOC_SCRATCH_BYTES,
+              "dcx_render_fwd_noalloc: this batch needs %zu bytes of scratch, more than the %zu of the static "
+              "array (use dcx_render_fwd with a workspace)",
+              fixed, (size_t)DCX_RENDER_NOALLOC_SCRATCH_BYTES);
+  void* ptr = nullptr;
+  if ((st = scratch_for_current_device(&ptr))) return st;
+  return render_fwd(topo, cfg, B, vertex_pos, vertex_color, texture, tex_h, tex_w, sh_coeff, extrinsics,
+                    intrinsics, barycentric, face, render, vertex_normal, ptr, DCX_RENDER_NOALLOC_SCRATCH_BYTES,
+                    as_stream(stream));
 }
 
 extern "C" dcx_status dcx_render_fwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,

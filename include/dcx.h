@@ -202,6 +202,28 @@ dcx_status dcx_render_fwd(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B
                           float* render, float* vertex_normal, void* workspace,
                           size_t workspace_bytes, void* stream);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dcx_render_fwd with its scratch (projections, normals,
+ * triangle records, tile counters and lists) in a static device array of
+ * DCX_RENDER_NOALLOC_SCRATCH_BYTES per device instead of a caller workspace.
+ * It calls no hipMalloc, hipFree or allocating library routine; its counters
+ * are cleared by kernels, not memsets.  A batch whose scratch would exceed the
+ * array (dcx_render_workspace_size's fixed part plus 65536 tile-list entries)
+ * is rejected with DCX_ERR_INVALID.  Every call on a device shares that
+ * device's array and leaves it holding its own intermediates: calls that
+ * overlap on one device (two streams or two threads) race.  The first call on
+ * a device looks up the array's address there (the device must be current).
+ * Arguments and outputs as dcx_render_fwd.
+ */
+#define DCX_RENDER_NOALLOC_SCRATCH_BYTES (64u << 20)
+dcx_status dcx_render_fwd_noalloc(const dcx_topo* topo, const dcx_raster_cfg* cfg, int B,
+                                  const float* vertex_pos, const float* vertex_color, const float* texture,
+                                  int tex_h, int tex_w, const float* sh_coeff, const float* extrinsics,
+                                  const float* intrinsics, float* barycentric, int32_t* face,
+                                  float* render, float* vertex_normal, void* stream);
+
 /* compute_normal_map attribute values other than 'none' */
 #define DCX_NORMAL_MAP_NORMAL 1   /* 'normal': interpolated unit vertex normals, renormalised */
 #define DCX_NORMAL_MAP_POSITION 2 /* 'position': interpolated vertex positions */
