@@ -108,6 +108,65 @@ __global__ __launch_bounds__(kB) void vertex_normal_kernel(const float* __restri
     }
 }
 
+// Reference rule (uv2barycentric / rayTriangleIntersect, RendererUtil.h:26-128,
+// with getRayCuda2, CameraUtil.h:251-258): the ray from the camera centre
+// through a pixel centre, from the camera's inverse matrices, intersected
+// with the triangle after every coordinate is divided by 1000.
+struct RefCam {
+  f3 o;         // camera centre: last column of invE, dehomogenised
+  f3 os;        // o / 1000
+  float P[12];  // first three rows of invP
+};
+__device__ __forceinline__ RefCam load_ref_cam(const float* rc) {
+  RefCam c;
+  const float ow = rc[15];
+  c.o = mk3(rc[3] / ow, rc[7] / ow, rc[11] / ow);
+  c.os = mk3(c.o.x / 1000.f, c.o.y / 1000.f, c.o.z / 1000.f);
+  for (int k = 0; k < 12; ++k) c.P[k] = rc[16 + k];
+  return c;
+}
+struct RefTri {
+  f3 v0, v1, v2;  // vertices / 1000
+  f3 n;           // unnormalised normal (v1 - v0) x (v2 - v0)
+  float d0;       // v0 . n
+};
+__device__ __forceinline__ RefTri ref_tri(const float* vp, int i0, int i1, int i2) {
+  RefTri t;
+  t.v0 = mk3(vp[3 * i0] / 1000.f, vp[3 * i0 + 1] / 1000.f, vp[3 * i0 + 2] / 1000.f);
+  t.v1 = mk3(vp[3 * i1] / 1000.f, vp[3 * i1 + 1] / 1000.f, vp[3 * i1 + 2] / 1000.f);
+  t.v2 = mk3(vp[3 * i2] / 1000.f, vp[3 * i2 + 1] / 1000.f, vp[3 * i2 + 2] / 1000.f);
+  t.n = cross(t.v1 - t.v0, t.v2 - t.v0);
+  t.d0 = dot(t.v0, t.n);
+  return t;
+}
+// Ray through (u, v) against the triangle: false on a miss (parallel within
+// 1e-7, behind the origin, or outside an edge); otherwise the distance t
+// along the unit ray and the weights (a, b) of vertices 0 and 1.
+__device__ __forceinline__ bool ref_hit(const RefCam& c, const RefTri& t, float u, float v, float& tt, float& a,
+                                        float& b) {
+  const float h0 = u * 1000.f, h1 = v * 1000.f, h2 = 1000.f, h3 = 1.f;
+  const f3 q = mk3(c.P[0] * h0 + c.P[1] * h1 + c.P[2] * h2 + c.P[3] * h3,
+                   c.P[4] * h0 + c.P[5] * h1 + c.P[6] * h2 + c.P[7] * h3,
+                   c.P[8] * h0 + c.P[9] * h1 + c.P[10] * h2 + c.P[11] * h3);
+  const f3 dd = q - c.o;
+  const float l = sqrtf(dot(dd, dd));
+  const f3 d = mk3(dd.x / l, dd.y / l, dd.z / l);
+  const float nd = dot(d, t.n);
+  if (fabsf(nd) < 0.0000001f) return false;
+  tt = (t.d0 - dot(c.os, t.n)) / nd;
+  if (tt < 0.f) return false;
+  const f3 P = mk3(c.os.x + tt * d.x, c.os.y + tt * d.y, c.os.z + tt * d.z);
+  if (dot(t.n, cross(t.v1 - t.v0, P - t.v0)) < 0.f) return false;
+  a = dot(t.n, cross(t.v2 - t.v1, P - t.v1));
+  if (a < 0.f) return false;
+  b = dot(t.n, cross(t.v0 - t.v2, P - t.v2));
+  if (b < 0.f) return false;
+  const float den = dot(t.n, t.n);
+  a /= den;
+  b /= den;
+  return true;
+}
+
 __device__ __forceinline__ ShadeParams make_shade(const RenderArgs& a, int b, int view) {
   ShadeParams s;
   s.albedo = a.albedo;
@@ -139,7 +198,18 @@ __device__ __forceinline__ void resolve_pixel(const RenderArgs& a, int b, int vi
   const int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
   const f4* pv = a.proj + (size_t)(view - a.view0) * a.N;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
-  const f2 ab = native_bary(pv[i0], pv[i1], pv[i2], px, py);
+  f2 ab;
+  if (a.rule == DCX_RULE_REFERENCE) {
+    // the winner's ray hit, evaluated again (the same operations as in the
+    // depth pass, so the same values)
+    const RefCam rc = load_ref_cam(a.refcam + (size_t)(view - a.view0) * 32);
+    const RefTri rt = ref_tri(a.vpos + (size_t)b * a.N * 3, i0, i1, i2);
+    float t, wa = 0.f, wb = 0.f;
+    (void)ref_hit(rc, rt, px, py, t, wa, wb);
+    ab = mk2(wa, wb);
+  } else {
+    ab = native_bary(pv[i0], pv[i1], pv[i2], px, py);
+  }
   const ShadeParams s = make_shade(a, b, view);
   const f3 col = shade_pixel(s, f, i0, i1, i2, ab.x, ab.y, px, py);
   a.face_out[o] = f;
@@ -150,6 +220,108 @@ __device__ __forceinline__ void resolve_pixel(const RenderArgs& a, int b, int vi
 // ---------------------------------------------------------------------------
 // Plain path (one batch element per launch sequence)
 // ---------------------------------------------------------------------------
+
+// Camera inverses of the reference rule: inverse([E; 0 0 0 1]) and
+// inverse([K 0; 0 0 0 1] [E; 0 0 0 1]) per camera (initializeCamerasDevice,
+// CUDABasedRasterization.cu:23-67, takes them in float; here the adjugate
+// over the determinant in FP64, rounded to float once).  refcam[32 c ..]:
+// invE row-major, then invP.
+__device__ void inverse4(const double m[16], double out[16]) {
+  double r[16];
+  r[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+         m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  r[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+         m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  r[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+         m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  r[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+          m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  r[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+         m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  r[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+         m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  r[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+         m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  r[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+          m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  r[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+         m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  r[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+         m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  r[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+          m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  r[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+          m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  r[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+         m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  r[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+         m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  r[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+          m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  r[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+          m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  const double det = m[0] * r[0] + m[1] * r[4] + m[2] * r[8] + m[3] * r[12];
+  for (int i = 0; i < 16; ++i) out[i] = r[i] / det;
+}
+
+__global__ __launch_bounds__(kB) void ref_camera_kernel(const float* ext, const float* intr, int C, float* refcam) {
+  const int c = blockIdx.x * kB + threadIdx.x;
+  if (c >= C) return;
+  const float* E = ext + (size_t)c * 12;
+  const float* K = intr + (size_t)c * 9;
+  double e4[16] = {}, k4[16] = {}, p4[16], ie[16], ip[16];
+  for (int r = 0; r < 3; ++r) {
+    for (int q = 0; q < 4; ++q) e4[4 * r + q] = (double)E[4 * r + q];
+    for (int q = 0; q < 3; ++q) k4[4 * r + q] = (double)K[3 * r + q];
+  }
+  e4[15] = 1.0;
+  k4[15] = 1.0;
+  for (int r = 0; r < 4; ++r)
+    for (int q = 0; q < 4; ++q) {
+      double acc = 0.0;
+      for (int k = 0; k < 4; ++k) acc += k4[4 * r + k] * e4[4 * k + q];
+      p4[4 * r + q] = acc;
+    }
+  inverse4(e4, ie);
+  inverse4(p4, ip);
+  float* o = refcam + (size_t)c * 32;
+  for (int i = 0; i < 16; ++i) { o[i] = (float)ie[i]; o[16 + i] = (float)ip[i]; }
+}
+
+// Reference rule, one lane per (camera, face): the face's projected bounding
+// box widened by half a pixel (projectFacesDevice, CUDABasedRasterization.cu:
+// 184-208), a ray per pixel centre against the triangle (renderDepthBufferDevice,
+// :215-255), and the depth key (int(t * 1e4) << 32) | face folded in with a
+// 64-bit atomicMin: the nearest quantised depth wins and equal depths go to
+// the lowest face id (the reference lets them race).
+__global__ __launch_bounds__(kB) void raster_ref_kernel(RenderArgs a, int b, unsigned long long* keys) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= (long long)a.C * a.F) return;
+  const int c = (int)(i / a.F), f = (int)(i - (long long)c * a.F);
+  const int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
+  const f4* pv = a.proj + (size_t)c * a.N;
+  const f4 p0 = pv[i0], p1 = pv[i1], p2 = pv[i2];
+  const float bx0 = fmaxf(fminf(p0.x, fminf(p1.x, p2.x)) - 0.5f, 0.f);
+  const float by0 = fmaxf(fminf(p0.y, fminf(p1.y, p2.y)) - 0.5f, 0.f);
+  const float bx1 = fminf(fmaxf(p0.x, fmaxf(p1.x, p2.x)) + 0.5f, (float)(a.W - 1));
+  const float by1 = fminf(fmaxf(p0.y, fmaxf(p1.y, p2.y)) + 0.5f, (float)(a.H - 1));
+  // truncated to int as the reference's integer box; the clamps only keep
+  // boxes of faces far off screen inside int range (they visit no pixel)
+  const int x0 = (int)fminf(bx0, (float)a.W), y0 = (int)fminf(by0, (float)a.H);
+  const int x1 = (int)fmaxf(bx1, -1.f), y1 = (int)fmaxf(by1, -1.f);
+  const RefCam rc = load_ref_cam(a.refcam + (size_t)c * 32);
+  const RefTri rt = ref_tri(a.vpos + (size_t)b * a.N * 3, i0, i1, i2);
+  unsigned long long* kv = keys + (size_t)c * a.H * a.W;
+  for (int x = x0; x <= x1; ++x)
+    for (int y = y0; y <= y1; ++y) {
+      float t, wa, wb;
+      if (!ref_hit(rc, rt, (float)x + 0.5f, (float)y + 0.5f, t, wa, wb)) continue;
+      if (!(wa >= -0.001f && wb >= -0.001f && wa <= 1.001f && wb <= 1.001f)) continue;
+      const unsigned long long k =
+          ((unsigned long long)(unsigned int)(int)(t * 10000.f) << 32) | (unsigned long long)(unsigned int)f;
+      atomicMin(kv + (size_t)y * a.W + x, k);
+    }
+}
 
 __global__ __launch_bounds__(kB) void clear_keys_kernel(unsigned long long* keys, long long n) {
   const long long i = (long long)blockIdx.x * kB + threadIdx.x;
@@ -354,9 +526,15 @@ dcx_status exclusive_scan(const unsigned int* in, unsigned int* out, unsigned in
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// The reference rule runs in the per-(camera, face) launch structure whatever
+// the path (it has no tiled form), so it takes the plain path's workspace.
+static bool plain_layout(const dcx_raster_cfg* c) {
+  return c->path == DCX_PATH_PLAIN || c->rule == DCX_RULE_REFERENCE;
+}
+
 size_t render_ws_fixed(const dcx_topo* t, const dcx_raster_cfg* c, int B, WsLayout* L) {
   const int N = t->N, F = t->F, C = c->num_cameras;
-  const bool plain = c->path == DCX_PATH_PLAIN;
+  const bool plain = plain_layout(c);
   const long long views = plain ? C : (long long)B * C;
   WsLayout l{};
   size_t o = 0;
@@ -365,6 +543,7 @@ size_t render_ws_fixed(const dcx_topo* t, const dcx_raster_cfg* c, int B, WsLayo
   l.vnrm = o; o = align256(o + sizeof(float) * 3 * (size_t)(plain ? 1 : B) * N);
   if (plain) {
     l.keys = o; o = align256(o + sizeof(unsigned long long) * (size_t)C * c->height * c->width);
+    if (c->rule == DCX_RULE_REFERENCE) { l.refcam = o; o = align256(o + sizeof(float) * 32 * (size_t)C); }
   } else {
     const int tx = (c->width + kTileW - 1) / kTileW, ty = (c->height + kTileH - 1) / kTileH;
     const long long T = views * tx * ty;
@@ -395,7 +574,7 @@ dcx_status validate_cfg(const dcx_topo* t, const dcx_raster_cfg* c, int B) {
   DCX_REQUIRE(c->shading_mode == DCX_SHADING_SHADELESS || c->shading_mode == DCX_SHADING_SHADED,
               "shading mode %d", c->shading_mode);
   DCX_REQUIRE(c->path == DCX_PATH_BINNED || c->path == DCX_PATH_PLAIN, "path %d", c->path);
-  DCX_REQUIRE(c->rule == DCX_RULE_NATIVE, "only the native raster rule runs on the GPU (rule %d)", c->rule);
+  DCX_REQUIRE(c->rule == DCX_RULE_NATIVE || c->rule == DCX_RULE_REFERENCE, "raster rule %d", c->rule);
   return DCX_OK;
 }
 
@@ -425,9 +604,12 @@ dcx_status render_fwd(const dcx_topo* t, const dcx_raster_cfg* c, int B, const f
   a.proj = (f4*)(w + L.proj);
   a.vnrm = (float*)(w + L.vnrm);
   a.face_out = face; a.bary_out = bary; a.render_out = render;
+  a.rule = c->rule;
+  float* refcam = c->rule == DCX_RULE_REFERENCE ? (float*)(w + L.refcam) : nullptr;
+  a.refcam = refcam;
   float* fnrm = (float*)(w + L.fnrm);
   const int N = t->N, F = t->F, C = c->num_cameras;
-  if (c->path == DCX_PATH_PLAIN) {
+  if (plain_layout(c)) {
     // Reference launch structure (CudaRenderer.cpp:325-344 -> CUDABasedRasterization.cu:553-576).
     unsigned long long* keys = (unsigned long long*)(w + L.keys);
     for (int b = 0; b < B; ++b) {
@@ -445,7 +627,13 @@ dcx_status render_fwd(const dcx_topo* t, const dcx_raster_cfg* c, int B, const f
       hipLaunchKernelGGL(vertex_normal_kernel, dim3(ceil_div(N, kB)), dim3(kB), 0, s, fnrm, t->vf_off,
                          t->vf_list, N, F, 1, C, a.vnrm,
                          vnormal ? vnormal + (size_t)b * C * N * 3 : nullptr);
-      hipLaunchKernelGGL(raster_plain_kernel, dim3(ceil_div((long long)C * F, kB)), dim3(kB), 0, s, ab, b, keys);
+      if (c->rule == DCX_RULE_REFERENCE) {
+        hipLaunchKernelGGL(ref_camera_kernel, dim3(ceil_div(C, kB)), dim3(kB), 0, s, ext + (size_t)b * C * 12,
+                           intr + (size_t)b * C * 9, C, refcam);
+        hipLaunchKernelGGL(raster_ref_kernel, dim3(ceil_div((long long)C * F, kB)), dim3(kB), 0, s, ab, b, keys);
+      } else {
+        hipLaunchKernelGGL(raster_plain_kernel, dim3(ceil_div((long long)C * F, kB)), dim3(kB), 0, s, ab, b, keys);
+      }
       hipLaunchKernelGGL(resolve_plain_kernel, dim3(ceil_div(HW, kB)), dim3(kB), 0, s, ab, b, keys);
     }
     return check_launch("dcx_render_fwd(plain)");
@@ -505,7 +693,7 @@ extern "C" dcx_status dcx_topo_destroy(dcx_topo* t) {
 extern "C" size_t dcx_render_workspace_size(const dcx_topo* t, const dcx_raster_cfg* c, int B) {
   if (validate_cfg(t, c, B) != DCX_OK) return 0;
   size_t fixed = render_ws_fixed(t, c, B, nullptr);
-  if (c->path == DCX_PATH_BINNED) fixed += sizeof(unsigned int) * default_bin_capacity(t, c, B);
+  if (!plain_layout(c)) fixed += sizeof(unsigned int) * default_bin_capacity(t, c, B);
   return fixed;
 }
 
@@ -555,7 +743,7 @@ extern "C" dcx_status dcx_render_fwd_noalloc(const dcx_topo* topo, const dcx_ras
   // the fixed part (fewer would only send more tiles down the overflow path)
   constexpr size_t kMinNoAllocBins = 65536;
   const size_t fixed = render_ws_fixed(topo, cfg, B, nullptr) +
-                       (cfg->path == DCX_PATH_BINNED ? sizeof(unsigned int) * kMinNoAllocBins : 0);
+                       (plain_layout(cfg) ? 0 : sizeof(unsigned int) * kMinNoAllocBins);
   DCX_REQUIRE(fixed <= (size_t)DCX_RENDER_NOALLOC_SCRATCH_BYTES,
               "dcx_render_fwd_noalloc: this batch needs %zu bytes of scratch, more than the %zu of the static "
               "array (use dcx_render_fwd with a workspace)",

@@ -36,6 +36,8 @@ struct RenderArgs {
   int* face_out;
   float* bary_out;
   float* render_out;
+  int rule;                   // DCX_RULE_*
+  const float* refcam;        // reference rule: [views held,32] camera inverses (invE, invP)
 };
 
 struct BinArgs {
@@ -49,7 +51,7 @@ struct BinArgs {
 };
 
 struct WsLayout {
-  size_t proj, fnrm, vnrm, keys, tri, tile_cnt, tile_off, tile_fill, aux, bins;
+  size_t proj, fnrm, vnrm, keys, refcam, tri, tile_cnt, tile_off, tile_fill, aux, bins;
   long long tiles;
 };
 

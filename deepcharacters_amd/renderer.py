@@ -26,6 +26,7 @@ ALBEDO_MODES = {"vertexColor": 0, "textured": 1, "normal": 2, "lighting": 3,
 SHADING_MODES = {"shadeless": 0, "shaded": 1}
 NORMAL_MAP_MODES = ("none", "normal", "position", "face")
 RULE_NATIVE, RULE_REFERENCE = 0, 1
+RULES = {"native": RULE_NATIVE, "reference": RULE_REFERENCE}
 PATHS = {"binned": 0, "plain": 1}
 
 
@@ -69,15 +70,22 @@ class Topology:
 
 
 def make_cfg(C, W, H, albedo="vertexColor", shading="shadeless", image_filter_size=2,
-             texture_filter_size=2, path="binned") -> RasterCfg:
+             texture_filter_size=2, path="binned", rule="native") -> RasterCfg:
+    """rule: "native" (watertight fixed-point edge functions, exact depth key)
+    or "reference" (the reference's per-pixel ray test on coordinates divided
+    by 1000 and its int(t * 1e4) depth, CUDABasedRasterization.cu:184-255;
+    lowest face id on equal depths; it always runs the per-(camera, face)
+    kernel, whatever the path, and covers nothing at metre scale)."""
     if albedo not in ALBEDO_MODES:
         raise ValueError(f"invalid albedo mode {albedo!r}")
     if shading not in SHADING_MODES:
         raise ValueError(f"invalid shading mode {shading!r}")
     if path not in PATHS:
         raise ValueError(f"invalid path {path!r}")
+    if rule not in RULES:
+        raise ValueError(f"invalid raster rule {rule!r}")
     return RasterCfg(C, W, H, ALBEDO_MODES[albedo], SHADING_MODES[shading], image_filter_size,
-                     texture_filter_size, RULE_NATIVE, PATHS[path])
+                     texture_filter_size, RULES[rule], PATHS[path])
 
 
 def workspace(topo: Topology, cfg: RasterCfg, B: int, device, extra_bins: int = 0) -> torch.Tensor:

@@ -159,7 +159,11 @@ typedef struct dcx_topo dcx_topo;
 #define DCX_SHADING_SHADED 1
 /* coverage / visibility rule */
 #define DCX_RULE_NATIVE 0    /* watertight fixed-point edge functions, exact depth key */
-#define DCX_RULE_REFERENCE 1 /* the reference's ray test + int(t*1e4) depth (oracle only) */
+#define DCX_RULE_REFERENCE 1 /* the reference's per-pixel ray test on coordinates / 1000 and its
+                                int(t*1e4) depth key (CUDABasedRasterization.cu:184-255,
+                                RendererUtil.h:26-128), lowest face id on equal keys; runs the
+                                per-(camera, face) kernel on either path; mm-scale scenes only
+                                (at metre scale the ray test finds every triangle parallel) */
 /* execution path */
 #define DCX_PATH_BINNED 0 /* all batch elements + views per launch, tile-binned LDS depth */
 #define DCX_PATH_PLAIN 1  /* reference launch structure: host loop, lane per (camera, face) */

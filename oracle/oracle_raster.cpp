@@ -408,7 +408,10 @@ void raster_fwd(const FwdCfg& cf, const int32_t* faces, const T* tc, const T* vp
         T by0 = std::fmax(std::fmin(p0.v, std::fmin(p1.v, p2.v)) - T(0.5), T(0));
         T bx1 = std::fmin(std::fmax(p0.u, std::fmax(p1.u, p2.u)) + T(0.5), T(cf.W - 1));
         T by1 = std::fmin(std::fmax(p0.v, std::fmax(p1.v, p2.v)) + T(0.5), T(cf.H - 1));
-        int ix0 = (int)bx0, iy0 = (int)by0, ix1 = (int)bx1, iy1 = (int)by1;
+        // truncated to int as the reference's integer box; the clamps only
+        // keep boxes of faces far off screen inside int range (no pixels)
+        int ix0 = (int)std::fmin(bx0, T(cf.W)), iy0 = (int)std::fmin(by0, T(cf.H));
+        int ix1 = (int)std::fmax(bx1, T(-1)), iy1 = (int)std::fmax(by1, T(-1));
         V3<T> w0 = v3(vp[3 * fv[0]], vp[3 * fv[0] + 1], vp[3 * fv[0] + 2]);
         V3<T> w1 = v3(vp[3 * fv[1]], vp[3 * fv[1] + 1], vp[3 * fv[1] + 2]);
         V3<T> w2 = v3(vp[3 * fv[2]], vp[3 * fv[2] + 1], vp[3 * fv[2] + 2]);

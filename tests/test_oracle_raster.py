@@ -204,6 +204,33 @@ def test_rules_agree_on_mm_scaled_scenes():
     assert np.abs(br - bn)[m].max() < 5e-3
 
 
+def test_reference_rule_weights_describe_the_ray_hit():
+    """The reference rule's (a, b) are the weights of vertices 0 and 1 of the
+    3D hit point: a * v0 + b * v1 + (1 - a - b) * v2, projected with the
+    camera, lands on the pixel centre (FP64 evaluation of the rule; its
+    camera inverses are rounded to float, so within 1e-4 px), for a large
+    millimetre-scale triangle; a planted swap of a and b misses by > 1e-2 px."""
+    ext, intr = scenes.ring_cameras(1, 3.0, 300.0, 48, 40, phase=0.2)
+    E = ext.reshape(3, 4) * np.array([1, 1, 1, 1000.0])
+    R, t = E[:, :3], E[:, 3]
+    K = intr.reshape(3, 3)
+    # a triangle 2.5 m in front of the camera, in its view, in world mm
+    cam = np.array([[-300.0, -250.0, 2500.0], [350.0, -200.0, 2600.0], [0.0, 300.0, 2400.0]])
+    V = ((cam - t) @ R)[None]
+    face, bary, _, _ = orc.raster_fwd(np.array([[0, 1, 2]]), V, E.reshape(1, -1), intr.reshape(1, -1), 40, 48,
+                                      rule="reference", dtype=np.float64)
+    ys, xs = np.nonzero(face[0, 0] == 0)
+    assert len(xs) > 200
+    a, b = bary[0, 0, ys, xs, 0], bary[0, 0, ys, xs, 1]
+    P = a[:, None] * V[0, 0] + b[:, None] * V[0, 1] + (1 - a - b)[:, None] * V[0, 2]
+    h = (K @ (R @ P.T + t[:, None])).T
+    assert np.abs(h[:, 0] / h[:, 2] - (xs + 0.5)).max() < 1e-4
+    assert np.abs(h[:, 1] / h[:, 2] - (ys + 0.5)).max() < 1e-4
+    P2 = b[:, None] * V[0, 0] + a[:, None] * V[0, 1] + (1 - a - b)[:, None] * V[0, 2]
+    h2 = (K @ (R @ P2.T + t[:, None])).T
+    assert np.abs(h2[:, 0] / h2[:, 2] - (xs + 0.5)).max() > 1e-2
+
+
 def test_reference_rule_rejects_metre_scale_triangles():
     """Documented reference behaviour: with vertices in metres the 1/1000 scaling
     makes |N . d| < 1e-7 for small triangles, so the ray test never hits."""
