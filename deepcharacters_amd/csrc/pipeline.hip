@@ -142,7 +142,8 @@ struct Dev {
   float* corner_sum;      // [sumF*3,6] raster gradient per face corner (position 3, colour 3), summed over
                           // tiles and views, stored at the corner's vf_list slot
   float* g_v;             // [sumN,3] position gradient from records beyond capacity
-  float* g_pts;           // [totVN,2]
+  float* g_pts;           // [totVN,2] model->data gradient w.r.t. the projected points
+  long long* g_fx;        // [totVN,2] data->model residuals summed per point, fixed point (units of 2^-40)
   float* u;               // [sumN,3] gradient w.r.t. the ED-deformed position
   float* loss_tile;       // [totTiles] colour loss per tile
   float* loss_pt;         // [V*gxN] silhouette model->data loss per block
@@ -1401,6 +1402,9 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
     }
     d.g_pts[2 * pi] = gx;
     d.g_pts[2 * pi + 1] = gy;
+    // the data->model residual sums of this point start at zero (the pixel
+    // pass adds to them after this kernel)
+    *reinterpret_cast<longlong2*>(d.g_fx + 2 * pi) = make_longlong2(0, 0);
   }
   const float t = block_sum(lsum, s_red);
   if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
@@ -1420,6 +1424,7 @@ __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
 }
 
 constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
+constexpr float kResidualFx = 1099511627776.f;  // 2^40: fixed-point unit of the data->model residual sums
 constexpr int kDtApron = kDtTile + 4;  // the tile plus the 2-pixel border of the 5x5 boundary test
 constexpr int kMaxRegionPts = 2048;
 constexpr int kMaxRegionRows = 64;
@@ -1612,8 +1617,13 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       if (sub == 0 && best >= 0) {
         const float r0 = (bx - ix) / (30.f * sc), r1 = (by - iy) / (30.f * sc);
         lsum = io.w_sil * (r0 * r0 + r1 * r1);
-        atomicAdd(d.g_pts + 2 * (vw.vn0 + best), 2.f * io.w_sil * r0 / (30.f * sc));
-        atomicAdd(d.g_pts + 2 * (vw.vn0 + best) + 1, 2.f * io.w_sil * r1 / (30.f * sc));
+        // the residual (|r| <= 1) goes to the nearest point's sums in fixed
+        // point: integer additions give the same bits in any order, so the
+        // joint and node gradients repeat exactly; the gradient factor
+        // 2 w_sil / (30 sc) is applied once, when the sums are read
+        unsigned long long* q = reinterpret_cast<unsigned long long*>(d.g_fx + 2 * (vw.vn0 + best));
+        atomicAdd(q, (unsigned long long)(long long)rintf(r0 * kResidualFx));
+        atomicAdd(q + 1, (unsigned long long)(long long)rintf(r1 * kResidualFx));
       }
     }
   }
@@ -1695,7 +1705,16 @@ __global__ __launch_bounds__(kB) void vertex_bwd_kernel(Dev d, IO io) {
     for (int k = 0; k < c.C; ++k) {
       const int w = c.view0 + k;
       const ViewD vw = d.vw[w];
-      const f2 gp = ((const f2*)d.g_pts)[vw.vn0 + v];
+      f2 gp = ((const f2*)d.g_pts)[vw.vn0 + v];
+      const longlong2 q = *reinterpret_cast<const longlong2*>(d.g_fx + 2 * (vw.vn0 + v));
+      if (q.x != 0 || q.y != 0) {
+        // data->model part: the residual sums times 2 w_sil / (30 sc)
+        // (MultiViewSilhouetteLossGPUOpGrad.cu:11-72)
+        const float sc = io.crops[(size_t)w * 7 + 2];
+        const float unit = 2.f * io.w_sil / (30.f * sc);
+        gp.x += unit * (float)((double)q.x * (1.0 / (double)kResidualFx));
+        gp.y += unit * (float)((double)q.y * (1.0 / (double)kResidualFx));
+      }
       if (gp.x == 0.f && gp.y == 0.f) continue;
       any = true;
       const float* E = io.ext + (size_t)w * 12;
@@ -2120,6 +2139,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.corner_sum, sizeof(float) * kGrad * (size_t)p->sumF);
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
+  alloc(&d.g_fx, sizeof(long long) * 2 * (size_t)vn);
   alloc(&d.u, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.loss_tile, sizeof(float) * (size_t)tiles);
   alloc(&d.loss_pt, sizeof(float) * (size_t)p->V * d.gxN);

@@ -162,21 +162,29 @@ def test_record_capacity_overflow_matches(works):
 
 
 def test_batched_step_is_deterministic(works):
-    """The raster path reduces its gradients without float atomics (per
-    (triangle, tile) records gathered in a fixed order) and the losses from
-    per-block partials: losses, buffers and colour gradients repeat bit for
-    bit.  (Joint-DQ and node gradients also carry the silhouette's
-    nearest-point scatter, a float atomicAdd, so only their values, not their
-    bits, are order-independent.)"""
+    """No float atomics anywhere in the step: the raster path reduces its
+    gradients through per-(triangle, tile) records gathered in a fixed order,
+    the silhouette's nearest-point scatter adds fixed-point residuals with
+    integer atomics, losses come from per-block partials and the joint / node
+    reductions run in a fixed order.  Every output repeats bit for bit, also
+    across a captured graph's replays."""
     bp = pipeline.BatchedPipeline(works)
     bp.step()
     torch.cuda.synchronize()
-    first = [x.clone() for x in (bp.loss, bp.face, bp.bary, bp.render, bp.g_colors, bp.g_dq)]
-    bp.step()
+    names = ("loss", "face", "bary", "render", "g_colors", "g_dq", "g_dT", "g_dA")
+    first = [getattr(bp, n).clone() for n in names]
+    for _ in range(2):
+        bp.step()
+        torch.cuda.synchronize()
+        for n, a in zip(names, first):
+            assert torch.equal(a, getattr(bp, n)), n
+    s = torch.cuda.Stream()
+    bp.capture(s)
+    bp.replay()
+    s.synchronize()
     torch.cuda.synchronize()
-    for a, b in zip(first[:5], (bp.loss, bp.face, bp.bary, bp.render, bp.g_colors)):
-        assert torch.equal(a, b)
-    assert _rel(bp.g_dq, first[5]) < 1e-5
+    for n, a in zip(names, first):
+        assert torch.equal(a, getattr(bp, n)), n
 
 
 def _outputs(bp):
