@@ -72,6 +72,12 @@ constexpr float img_grad_inv_norm() {
 constexpr float kImgGradInvNorm = img_grad_inv_norm();
 constexpr int kGrad = 18;     // per (triangle, tile) record: 3 vertices x (pos 3 + colour 3)
 constexpr int kXcd = 8;       // L2 domains the workgroups are dealt round-robin to
+constexpr int kFast = 256;    // tiles with at most this many triangles stage their list in LDS
+// Tiles with longer lists (or past the record store) take the fine kernel's
+// slow path, several times longer than a typical tile: the tile scan lists up
+// to kHeavyCap of them and the first kHeavyCap workgroups of the launch take
+// them, so they start first instead of ending the launch late.
+constexpr int kHeavyCap = 4096;
 
 struct CharD {
   int N, F, M, J;
@@ -139,6 +145,8 @@ struct Dev {
   unsigned int* cell_fill;
   f4* cell_pts;           // [totVN] (x, y, point id bits, 0) grouped by cell
   unsigned int* scan_aux;
+  unsigned int* heavy;     // [kHeavyCap] global indices of the slow-path tiles (any order)
+  unsigned int* heavy_n;   // number of slow-path tiles (may exceed kHeavyCap: then none is listed)
   float* corner_sum;      // [sumF*3,6] raster gradient per face corner (position 3, colour 3), summed over
                           // tiles and views, stored at the corner's vf_list slot
   float* g_v;             // [sumN,3] position gradient from records beyond capacity
@@ -247,8 +255,11 @@ __global__ __launch_bounds__(kB) void scan_sum_kernel(const unsigned int* in, un
   scan_sum_body(in, sums, n, blockIdx.x);
 }
 
-__global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, unsigned int* out,
-                                                        const unsigned int* sums, long long n) {
+// heavy != nullptr (the tile scan): also lists the tiles whose count exceeds
+// kFast or whose list ends past `capacity` record slots.
+__device__ __forceinline__ void scan_apply_body(const unsigned int* in, unsigned int* out, const unsigned int* sums,
+                                                long long n, unsigned int* heavy, unsigned int* heavy_n,
+                                                unsigned int capacity) {
   __shared__ unsigned int s_wave[kB / 64];
   unsigned int pre = 0;
   for (unsigned int i = threadIdx.x; i < blockIdx.x; i += kB) pre += sums[i];
@@ -264,10 +275,28 @@ __global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, 
   unsigned int ex = offset + block_scan256(t, s_wave, chunk);
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
-    if (base + k < n) out[base + k] = ex;
+    if (base + k < n) {
+      out[base + k] = ex;
+      if (heavy && (v[k] > (unsigned int)kFast || ex + v[k] > capacity)) {
+        const unsigned int h = atomicAdd(heavy_n, 1u);
+        if (h < (unsigned int)kHeavyCap) heavy[h] = (unsigned int)(base + k);
+      }
+    }
     ex += v[k];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = offset + chunk;
+}
+
+__global__ __launch_bounds__(kB) void scan_apply_kernel(const unsigned int* in, unsigned int* out,
+                                                        const unsigned int* sums, long long n) {
+  scan_apply_body(in, out, sums, n, nullptr, nullptr, 0u);
+}
+
+__global__ __launch_bounds__(kB) void tile_scan_apply_kernel(const unsigned int* in, unsigned int* out,
+                                                             const unsigned int* sums, long long n,
+                                                             unsigned int* heavy, unsigned int* heavy_n,
+                                                             unsigned int capacity) {
+  scan_apply_body(in, out, sums, n, heavy, heavy_n, capacity);
 }
 
 // In-place exclusive scan of a[0..n) by one workgroup, kOnePer entries per
@@ -798,9 +827,20 @@ __device__ __forceinline__ float block_sum(float v, float* s_red) {
   return t;
 }
 
+// View holding global tile index t (the views' tiles are consecutive).
+__device__ __forceinline__ int view_of_tile(const Dev& d, long long t) {
+  int lo = 0, hi = d.V - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.vw[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Workgroup (x, view) -> tile of the view.  Workgroups are dealt to the L2
-// domains (XCDs) round-robin and the grid's x extent n is a multiple of kXcd,
-// so (x, view) runs on domain x % kXcd.  Each view is cut into kXcd strips of
+// domains (XCDs) round-robin; (x, view) is workgroup kHeavyCap + view * n + x
+// of the fine kernel's grid, with kHeavyCap and n multiples of kXcd, so it
+// runs on domain x % kXcd.  Each view is cut into kXcd strips of
 // n / kXcd consecutive tiles and domain k takes strip (k + view) % kXcd: a
 // contiguous run, so triangle records and target pixels shared by
 // neighbouring tiles are fetched into one L2, and a different strip in every
@@ -811,7 +851,6 @@ __device__ __forceinline__ int xcd_tile(unsigned int x, unsigned int n, unsigned
   return (int)((((x & (kXcd - 1)) + view) & (kXcd - 1)) * (n / kXcd) + x / kXcd);
 }
 
-constexpr int kFast = 256;          // tiles with at most this many triangles stage their list in LDS
 
 // Fast-path walk record of one listed triangle, clipped to the tile: the
 // three biased edge values (inside <=> all >= 0) at the centre of the clipped
@@ -947,10 +986,30 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   __shared__ unsigned int s_start[kFast + 1];
   __shared__ unsigned int s_wave[kTilePix / 64];
   __shared__ float s_red[kTilePix / 64];
-  const int w = blockIdx.y;
-  const int lt = xcd_tile(blockIdx.x, gridDim.x, (unsigned int)w);
+  // workgroup -> (view, tile): the first kHeavyCap take the listed slow-path
+  // tiles (when all of them are listed), the rest walk every view's tiles in
+  // the domain-strip order of xcd_tile and skip the listed ones
+  const unsigned int n_heavy = *d.heavy_n;
+  const bool listed = n_heavy <= (unsigned int)kHeavyCap;
+  int w, lt;
+  if (blockIdx.x < (unsigned int)kHeavyCap) {
+    if (!listed || blockIdx.x >= n_heavy) return;
+    const long long t = d.heavy[blockIdx.x];
+    w = view_of_tile(d, t);
+    lt = (int)(t - d.vw[w].tile0);
+  } else {
+    const unsigned int nx = (unsigned int)((d.maxTiles + kXcd - 1) / kXcd * kXcd);
+    const unsigned int j = blockIdx.x - (unsigned int)kHeavyCap;
+    w = (int)(j / nx);
+    lt = xcd_tile(j - (unsigned int)w * nx, nx, (unsigned int)w);
+    if (lt >= d.vw[w].tiles) return;
+    if (listed) {
+      const long long t = d.vw[w].tile0 + lt;
+      const unsigned int o0 = d.tile_off[t], o1 = d.tile_off[t + 1];
+      if (o1 - o0 > (unsigned int)kFast || o1 > d.capacity) return;  // taken by a listed workgroup
+    }
+  }
   const ViewD vw = d.vw[w];
-  if (lt >= vw.tiles) return;
   const CharD c = d.ch[vw.ch];
   const long long tile = vw.tile0 + lt;
   const int tx = lt % vw.tiles_x, ty = lt / vw.tiles_x;
@@ -2128,6 +2187,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.tile_off, 4 * (size_t)(tiles + 1));
   alloc(&d.tile_fill, 4 * (size_t)tiles);
   alloc(&d.scan_aux, 4 * (size_t)aux);
+  alloc(&d.heavy, 4 * (size_t)kHeavyCap);
+  alloc(&d.heavy_n, 4);
   alloc(&d.bins, 4 * (size_t)cap);
   alloc(&d.rec, sizeof(float) * kGrad * (size_t)cap);
   alloc(&d.rec_vis, (size_t)cap);
@@ -2206,6 +2267,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   add(d.tile_fill, d.totTiles);
   add(d.cell_cnt, d.totCells);
   add(d.cell_fill, d.totCells);
+  add(d.heavy_n, 1);
   int stage = 0;
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
   const int B = p->B, V = p->V;
@@ -2250,7 +2312,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     if (nb < 1 || nb >= (1ll << 31) - 1) return set_error(DCX_ERR_INVALID, "scan: bad length %lld", nt);
     hipLaunchKernelGGL(scan_sum_with_one_kernel, dim3((unsigned)nb + 1), dim3(kB), 0, s, d.tile_cnt, d.scan_aux, nt,
                        d.tri_blk, (long long)V * ceil_div(d.maxF, kB), d.tri_base + d.totVF);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kB), 0, s, d.tile_cnt, d.tile_off, d.scan_aux, nt);
+    hipLaunchKernelGGL(tile_scan_apply_kernel, dim3((unsigned)nb), dim3(kB), 0, s, d.tile_cnt, d.tile_off, d.scan_aux,
+                       nt, d.heavy, d.heavy_n, d.capacity);
     if (dcx_status e = check_launch("scan")) return e;
   }
   mark();
@@ -2262,7 +2325,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   const bool njoin_early = fork && io.nrm_out;
   if (njoin_early) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
   {
-    const dim3 grid((unsigned)(ceil_div(d.maxTiles, kXcd) * kXcd), V);  // x extent a multiple of kXcd
+    // the slow-path tiles' workgroups first, then per view a multiple of kXcd
+    const dim3 grid((unsigned)(kHeavyCap + (long long)ceil_div(d.maxTiles, kXcd) * kXcd * V));
     const dim3 blk(kTilePix);
     if (io.uv || io.nrm_out) {
       if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, true>), grid, blk, 0, s, d, io);
