@@ -1198,12 +1198,13 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
             if (yy != 0) tv += gi * ((float)yy / den);
           }
         const float gu = -tu * kImgGradInvNorm, gvv = -tv * kImgGradInvNorm;
-        const float* vb = d.vdef + 3 * (size_t)c.v0;
-        const f3 X0 = ld3(vb + 3 * i0), X1 = ld3(vb + 3 * i1), X2 = ld3(vb + 3 * i2);
-        const f3 fp = X0 * wgt[0] + X1 * wgt[1] + X2 * wgt[2];
-        const float hx = M[0] * fp.x + M[1] * fp.y + M[2] * fp.z + M[9];
-        const float hy = M[3] * fp.x + M[4] * fp.y + M[5] * fp.z + M[10];
-        const float hz = M[6] * fp.x + M[7] * fp.y + M[8] * fp.z + M[11];
+        // the fragment's homogeneous image point M (sum_k w_k X_k) + m: the
+        // projection is linear in homogeneous coordinates, so it is the same
+        // blend of the vertices' (u z, v z, z) (a winning triangle has every
+        // vertex in front of the camera, so these are its true projections)
+        const float hx = P[0].x * P[0].z * wgt[0] + P[1].x * P[1].z * wgt[1] + P[2].x * P[2].z * wgt[2];
+        const float hy = P[0].y * P[0].z * wgt[0] + P[1].y * P[1].z * wgt[1] + P[2].y * P[2].z * wgt[2];
+        const float hz = P[0].z * wgt[0] + P[1].z * wgt[1] + P[2].z * wgt[2];
         if (fabsf(hz) > 0.0001f) {
           const float rz = rcp(hz);
           const float gh0 = gu * rz, gh1 = gvv * rz, gh2 = -(gu * hx + gvv * hy) * rz * rz;
