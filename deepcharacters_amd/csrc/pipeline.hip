@@ -865,7 +865,8 @@ struct RowRec {
   float iz[3];
   int box;
   unsigned int tag;
-  int bias;  // bit i set: edge i is not top-left (bias -1)
+  int bias;  // bit i < 3 set: edge i is not top-left (bias -1); bits 8..31: the record's first row in
+             // the tile's row list
 };
 static_assert(sizeof(RowRec) == sizeof(TriRec), "the walk record takes the staged triangle's place");
 
@@ -980,10 +981,10 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       unsigned short order[kTilePix];             // covered pixels sorted by triangle
       unsigned char won[kFast];                   // list entries that won pixels, compacted
       unsigned int e[kFast];                      // record slot of each list entry
+      unsigned int start[kFast + 1];              // first sorted pixel of each list entry
     } srt;
     int face[kTilePix];
   } u2;
-  __shared__ unsigned int s_start[kFast + 1];
   __shared__ unsigned int s_wave[kTilePix / 64];
   __shared__ float s_red[kTilePix / 64];
   // workgroup -> (view, tile): the first kHeavyCap take the listed slow-path
@@ -1097,7 +1098,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     }
     unsigned int n_rows;
     const unsigned int row0 = block_scan256(nrows, s_wave, n_rows);
-    s_start[threadIdx.x] = row0;
+    if (nrows) u1.r.rec[threadIdx.x].bias |= (int)(row0 << 8);
     for (unsigned int base = 0; base < n_rows; base += kTilePix) {
       // entry k's rows [row0, row0 + nrows) that fall in this round
       const unsigned int lo = max(row0, base), hi = min(row0 + nrows, base + (unsigned int)kTilePix);
@@ -1105,7 +1106,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       __syncthreads();
       if (base + threadIdx.x < n_rows) {
         const int k = u1.r.item[threadIdx.x];
-        raster_row<tws>(u1.r.rec[k], (int)(base + threadIdx.x - s_start[k]), s_key);
+        raster_row<tws>(u1.r.rec[k], (int)(base + threadIdx.x - ((unsigned int)u1.r.rec[k].bias >> 8)), s_key);
       }
       __syncthreads();
     }
@@ -1265,12 +1266,12 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     // counts (<= 256) in the high half, first-pixel flags in the low half
     unsigned int tot2;
     const unsigned int ex2 = block_scan256((cnt << 16) | (first ? 1u : 0u), s_wave, tot2);
-    s_start[threadIdx.x] = ex2 >> 16;
-    if (threadIdx.x == 0) s_start[kFast] = tot2 >> 16;
+    u2.srt.start[threadIdx.x] = ex2 >> 16;
+    if (threadIdx.x == 0) u2.srt.start[kFast] = tot2 >> 16;
     if (first) u2.srt.won[ex2 & 0xffffu] = (unsigned char)bj;
     const int n_won = (int)(tot2 & 0xffffu);
     __syncthreads();
-    if (bj >= 0) u2.srt.order[s_start[bj] + u2.srt.wcnt[wv][bj] + rank] = (unsigned short)pix;
+    if (bj >= 0) u2.srt.order[u2.srt.start[bj] + u2.srt.wcnt[wv][bj] + rank] = (unsigned short)pix;
     if (threadIdx.x < n_items) {
       const unsigned int e = u2.srt.e[threadIdx.x];
       if (e < d.capacity) d.rec_vis[e] = cnt > 0 ? 1 : 0;
@@ -1284,7 +1285,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
 #pragma clang fp contract(fast)
       const int wi = i / 3, k = i - 3 * wi;
       const int j = u2.srt.won[wi];
-      const unsigned int st = s_start[j], en = s_start[j + 1];
+      const unsigned int st = u2.srt.start[j], en = u2.srt.start[j + 1];
       // w_k = s0 w0 + s1 w1 + s2 (1 - w0 - w1) with one s = 1: exact, branch-free
       const float s0 = k == 0 ? 1.f : 0.f, s1 = k == 1 ? 1.f : 0.f, s2 = k == 2 ? 1.f : 0.f;
       const float* gk = s_g + (3 * k) * kGPad;
