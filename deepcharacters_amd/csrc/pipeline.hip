@@ -1487,7 +1487,7 @@ __global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
 constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
 constexpr float kResidualFx = 1099511627776.f;  // 2^40: fixed-point unit of the data->model residual sums
 constexpr int kDtApron = kDtTile + 4;  // the tile plus the 2-pixel border of the 5x5 boundary test
-constexpr int kMaxRegionPts = 2048;
+constexpr int kMaxRegionPts = 1280;  // regions with more points search the global grid
 constexpr int kMaxRegionRows = 64;
 constexpr int kMaxRegionCells = 1024;  // entries of the per-cell LDS offset table (rows x (cols + 1))
 
@@ -1532,6 +1532,13 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
   const int ui = tu0 + (int)(threadIdx.x % kDtTile);
   const int vi = tv0 + (int)(threadIdx.x / kDtTile);
   const unsigned char* img = io.dt + vw.sil0;
+  // a boundary pixel has the contour value 127: a tile without one (most of
+  // them) is done before anything is staged
+  const bool contour = tile_ok && ui < S && vi < S && img[(size_t)vi * S + ui] == 127;
+  if (!__syncthreads_or(contour)) {
+    if (threadIdx.x == 0) d.loss_px[(size_t)w * gridDim.x + blockIdx.x] = 0.f;
+    return;
+  }
   if (tile_ok)
     for (int i = threadIdx.x; i < kDtApron * kDtApron; i += kB) {
       const int ay = i / kDtApron, ax = i - ay * kDtApron;
