@@ -417,8 +417,17 @@ __device__ __forceinline__ f3 ed_outer(const Dev& d, const CharD& c, int gv, f3 
   return o;
 }
 
-__global__ __launch_bounds__(kB) void vertex_fwd_kernel(Dev d, IO io) {
+// Also clears the step's accumulators and counters (grid-stride over the
+// whole grid, before any early return): the first of them is read by the
+// binning that follows, so the clear needs no launch of its own.
+__global__ __launch_bounds__(kB) void vertex_fwd_kernel(Dev d, IO io, ClearList L) {
   __shared__ float s_tab[256 * 8];
+  {
+    const long long stride = (long long)gridDim.x * gridDim.y * kB;
+    const long long i0 = ((long long)blockIdx.y * gridDim.x + blockIdx.x) * kB + threadIdx.x;
+    for (int k = 0; k < L.count; ++k)
+      for (long long i = i0; i < L.n[k]; i += stride) L.p[k][i] = 0u;
+  }
   const CharD c = d.ch[blockIdx.y];
   if ((long long)blockIdx.x * kB >= c.N) return;  // no vertices here (uniform over the block)
   if (!io.skA)
@@ -469,16 +478,9 @@ __device__ __forceinline__ f2 proj2(const float* K, f3 x) {
   return mk2(h0 / h2, h1 / h2);
 }
 
-// Also clears the step's accumulators and counters (grid-stride over the
-// whole grid, before any early return): the first of them is read by the
-// binning that follows, so the clear needs no launch of its own.
-__global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io, ClearList L) {
-  {
-    const long long stride = (long long)gridDim.x * gridDim.y * kB;
-    const long long i0 = ((long long)blockIdx.y * gridDim.x + blockIdx.x) * kB + threadIdx.x;
-    for (int k = 0; k < L.count; ++k)
-      for (long long i = i0; i < L.n[k]; i += stride) L.p[k][i] = 0u;
-  }
+// Projected vertices and K [R | t] per view, for the fine kernel and the
+// silhouette stages (the binning projects its corners itself).
+__global__ __launch_bounds__(kB) void project_kernel(Dev d, IO io) {
   const int w = blockIdx.y;
   const ViewD vw = d.vw[w];
   const CharD c = d.ch[vw.ch];
@@ -558,7 +560,7 @@ __device__ __forceinline__ int hash_find(unsigned int* keys, unsigned int tile, 
   return -1;
 }
 
-__global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
+__global__ __launch_bounds__(kB) void tri_count_kernel(Dev d, IO io) {
   __shared__ unsigned int s_keys[kHash];
   __shared__ unsigned int s_cnt[kHash];
   __shared__ unsigned int s_wave[kB / 64];
@@ -578,10 +580,16 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d) {
   unsigned int* cnt = d.tile_cnt + vw.tile0;
   if (f < c.F) {
     const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
-    const f4* pv = d.proj + vw.vn0;
+    // the corners' projections, the same operations as project_kernel's (which
+    // runs beside this kernel for the later stages)
+    const float* E = io.ext + (size_t)w * 12;
+    const float* K = io.intr + (size_t)w * 9;
+    const float* vb = d.vdef + 3 * (size_t)c.v0;
+    const f4 p0 = project_vertex(E, K, vb + 3 * fv[0]), p1 = project_vertex(E, K, vb + 3 * fv[1]);
+    const f4 p2 = project_vertex(E, K, vb + 3 * fv[2]);
     TriRec t;
     const long long ri = vw.vf0 + f;
-    if (!setup_tri(pv[fv[0]], pv[fv[1]], pv[fv[2]], f, vw.W, vw.H, t)) {
+    if (!setup_tri(p0, p1, p2, f, vw.W, vw.H, t)) {
       t.bx0_by0 = 0x7fff7fff;
       t.bx1_by1 = 0;
       d.tri[ri] = t;
@@ -2281,13 +2289,14 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
   const int B = p->B, V = p->V;
   // Eager and captured steps run on two streams: the side stream takes the
-  // vertex normals and their image-space directions (read only by the
-  // model->data residual) beside the forward and binning stages, the
-  // per-face record sums and the raster part of the vertex backward beside
-  // the silhouette stages, the loss totals, and the joint-DQ reduction beside
-  // the node gradients.  (A join costs ~10-20 us in a replayed graph, so the
-  // buffer clears run inside the projection kernel on s.)  Each fork is an event recorded on s and waited on by the
-  // side stream right away; each join is an event of its own.  A profiled
+  // vertex normals, their image-space directions and the vertex projections
+  // (read by the fine kernel and the silhouette stages; the binning projects
+  // its corners itself) beside the binning, the per-face record sums and the
+  // raster part of the vertex backward beside the silhouette stages, the loss
+  // totals, and the joint-DQ reduction beside the node gradients.  (A join
+  // costs ~10-20 us in a replayed graph, so the buffer clears run inside the
+  // vertex kernel on s.)  Each fork is an event recorded on s and waited on by
+  // the side stream right away; each join is an event of its own.  A profiled
   // step keeps every stage on one stream so that its events bracket it.
   const bool fork = ev == nullptr;
   auto to_side = [&]() -> dcx_status {
@@ -2297,10 +2306,10 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   };
   hipStream_t sn = fork ? p->side : s;
   mark();
-  mark();  // the clears run inside project_kernel
+  mark();  // the clears run inside vertex_fwd_kernel
   hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
   mark();
-  hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io);
+  hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io, L);
   mark();
   if (fork) {
     if (dcx_status e = to_side()) return e;
@@ -2308,11 +2317,11 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, sn, d);
   mark();
   hipLaunchKernelGGL(view_normal_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, sn, d, io);
+  mark();
+  hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, sn, d, io);
   if (fork) DCX_HIP(hipEventRecord(p->ev_njoin, p->side));
   mark();
-  hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io, L);
-  mark();
-  hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
+  hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d, io);
   mark();
   {
     // tile counts -> tile offsets, and in the same first launch the
@@ -2330,9 +2339,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   dcx_status st = DCX_OK;
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
   mark();
-  // the normal buffer reads the side stream's vertex normals
-  const bool njoin_early = fork && io.nrm_out;
-  if (njoin_early) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
+  // the fine kernel reads the side stream's projections, K [R | t] and vertex normals
+  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
   {
     // the slow-path tiles' workgroups first, then per view a multiple of kXcd
     const dim3 grid((unsigned)(kHeavyCap + (long long)ceil_div(d.maxTiles, kXcd) * kXcd * V));
@@ -2355,7 +2363,6 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, p->side, d, io);
     DCX_HIP(hipEventRecord(p->ev_join, p->side));
   }
-  if (fork && !njoin_early) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
   hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
   mark();
   st = scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, s);
