@@ -1051,13 +1051,16 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   }
 
   // target colours of the tile plus a kImgFs apron (clamped reads; the apron is
-  // only used by pixels whose whole filter window lies inside the image)
-  for (int i = threadIdx.x; i < an; i += kTilePix) {
-    const int ay = i / aw, ax = i - ay * aw;
-    const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
-    const f3 t = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
-    u2.tgt[i] = t.x; u2.tgt[an + i] = t.y; u2.tgt[2 * an + i] = t.z;
-  }
+  // only used by pixels whose whole filter window lies inside the image).
+  // The fast path stages them while its triangle records are in flight.
+  auto stage_target = [&]() {
+    for (int i = threadIdx.x; i < an; i += kTilePix) {
+      const int ay = i / aw, ax = i - ay * aw;
+      const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
+      const f3 t = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
+      u2.tgt[i] = t.x; u2.tgt[an + i] = t.y; u2.tgt[2 * an + i] = t.z;
+    }
+  };
 
   int f = -1;   // winning face of this pixel
   int bj = -1;  // fast path: index of the winning triangle in the tile list
@@ -1073,9 +1076,13 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     s_key[threadIdx.x] = kEmptyKey;
     unsigned int nrows = 0, ri = 0;
     bool wide = false;
+    TriRec t;
     if (threadIdx.x < n_items) {
       ri = d.bins[off + threadIdx.x];
-      const TriRec t = d.tri[ri];
+      t = d.tri[ri];
+    }
+    stage_target();
+    if (threadIdx.x < n_items) {
       my_f = t.face_flags & kFaceMask;
       const TileRange r = tile_range(t, tws, ths);
       my_e = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
@@ -1128,6 +1135,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       bj = (int)(tag & ((1u << kTagBits) - 1u));
     }
   } else {
+    stage_target();
     s_key[threadIdx.x] = kEmptyKey;
     __syncthreads();
     for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
