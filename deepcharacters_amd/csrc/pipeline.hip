@@ -90,7 +90,9 @@ struct ViewD {
   int ch, W, H, S;
   int tiles_x, tiles;
   long long vn0, vf0, pix0, sil0, tile0, cell0;
-  int gx, gy;  // silhouette grid
+  long long mask0;  // first word of the view's background mask
+  int mpitch;       // words per mask row: 2 * ceil(W / 64), so a row holds every tile shape's columns
+  int gx, gy;       // silhouette grid
 };
 
 struct Dev {
@@ -129,6 +131,9 @@ struct Dev {
   float* ke;              // [V,12] per view: K * R (3x3 row-major), then K * t
   f2* nrm2;               // [totVN]
   TriRec* tri;            // [totVF]
+  uint2* trange;          // [totVF] tile range of the current shape: (tx0 | ty0 << 16, ntx | nty << 16),
+                          // ntx = nty = 0 for a culled triangle (the binning reads these 8 bytes, not
+                          // the 48-byte record)
   unsigned int* tri_blk;  // [V * ceil(maxF / kB)] tiles touched per binning workgroup, then their
                           // exclusive scan (the workgroup's first record slot)
   unsigned int* tri_base; // [totVF+1] first record slot per (view, face)
@@ -140,6 +145,10 @@ struct Dev {
   unsigned char* rec_vis; // [capacity] 1 if the triangle won a pixel of the tile
   unsigned int capacity;
   unsigned char* isb;     // [totVN]
+  unsigned int* fgmask;   // [totMask] per view H rows of mpitch words: bit x of row y set <=> pixel (x, y)
+                          // has a face (the boundary test's 3x3 reads stay in a mask 1/32 of the face
+                          // buffer's size)
+  long long totMask;
   unsigned int* cell_cnt; // [totCells]
   unsigned int* cell_off; // [totCells+1]
   unsigned int* cell_fill;
@@ -593,9 +602,12 @@ __global__ __launch_bounds__(kB) void tri_count_kernel(Dev d, IO io) {
       t.bx0_by0 = 0x7fff7fff;
       t.bx1_by1 = 0;
       d.tri[ri] = t;
+      d.trange[ri] = make_uint2(0u, 0u);
     } else {
       d.tri[ri] = t;
       const TileRange r = tile_range(t, d.tws, d.ths);
+      d.trange[ri] = make_uint2((unsigned int)r.tx0 | ((unsigned int)r.ty0 << 16),
+                                (unsigned int)r.ntx | ((unsigned int)r.nty << 16));
       my_cnt = (unsigned int)(r.ntx * r.nty);
       const bool big = r.ntx * r.nty > kBigTri;
       for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
@@ -633,11 +645,9 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
   TileRange r{0, 0, 0, 0};
   bool big = false;
   if (f < c.F) {
-    const TriRec& t = d.tri[ri];
-    if (t.bx0_by0 != 0x7fff7fff) {
-      r = tile_range(t, d.tws, d.ths);
-      big = r.ntx * r.nty > kBigTri;
-    }
+    const uint2 tr = d.trange[ri];
+    r = TileRange{(int)(tr.x & 0xffffu), (int)(tr.x >> 16), (int)(tr.y & 0xffffu), (int)(tr.y >> 16)};
+    big = r.ntx * r.nty > kBigTri;
   }
   // first record slot of each face: the workgroup's offset plus the scan of
   // its faces' tile counts (the same counts tri_count summed)
@@ -968,6 +978,24 @@ __device__ __forceinline__ void write_extras(const Dev& d, const IO& io, const C
   }
 }
 
+// Face bits of the tile's rows (the mask starts the step cleared, so tiles
+// that no triangle touches store nothing): the first lane of each tile row in
+// the wavefront stores that row's bits from the wavefront's ballot (lanes are
+// pixels in row-major tile order), a 16-, 32- or 64-bit word at the tile's
+// column of the row, never shared with another tile.  Rows below the image
+// are not stored; bits right of it are 0 and never read.
+template <int TWS>
+__device__ __forceinline__ void store_fg(const Dev& d, const ViewD& vw, int tx, int y, int px_i, bool fg) {
+  const unsigned long long m = __ballot(fg);
+  if (px_i == 0 && y < vw.H) {
+    unsigned int* row = d.fgmask + vw.mask0 + (long long)y * vw.mpitch;
+    const int sh = (int)(threadIdx.x & 63u);
+    if (TWS == 6) reinterpret_cast<unsigned long long*>(row)[tx] = m;
+    else if (TWS == 5) row[tx] = (unsigned int)(m >> sh);
+    else reinterpret_cast<unsigned short*>(row)[tx] = (unsigned short)(m >> sh);
+  }
+}
+
 // TWS: log2 of the tile width (tiles are (1 << TWS) x (1 << (8 - TWS)) pixels).
 // EXTRA: the step also writes the uv / normal buffers.
 template <int TWS, bool EXTRA>
@@ -1147,6 +1175,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     f = (inimg && key != kEmptyKey) ? (int)(key & 0xffffffffull) : -1;
   }
   __syncthreads();  // the keys and staged triangles are overwritten by the gradient table below
+  store_fg<tws>(d, vw, tx, y, px_i, f >= 0);
 
   // ---- per pixel: outputs, colour loss, gradient contributions -> LDS
   float* s_g = u1.g;
@@ -1385,18 +1414,23 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
 // ---------------------------------------------------------------------------
 
 // Boundary flag of a projected vertex: inside the image with a 1-pixel
-// margin and a background pixel in its 3x3 neighbourhood of the face buffer.
-__device__ __forceinline__ bool boundary_p(const IO& io, const ViewD& vw, f4 p) {
-  bool fl = false;
+// margin and a background pixel (face < 0) in its 3x3 neighbourhood, read
+// from the fine kernel's face mask: per row the two words holding columns
+// u - 1 and u + 1 (the same word unless the three straddle a word boundary),
+// shifted so that those three bits come lowest; a clear bit is background.
+__device__ __forceinline__ bool boundary_p(const Dev& d, const ViewD& vw, f4 p) {
   // (int)p - 1 >= 0 && (int)p + 1 < W, tested in float (NaN / huge safe)
-  if (p.x >= 1.f && p.x < (float)(vw.W - 1) && p.y >= 1.f && p.y < (float)(vw.H - 1)) {
-    const int u = (int)p.x, r0 = (int)p.y;
-    const int* fb = io.face + vw.pix0;
-    for (int cc = u - 1; cc <= u + 1; ++cc)
-      for (int r = r0 - 1; r <= r0 + 1; ++r)
-        if (fb[(size_t)r * vw.W + cc] < 0) fl = true;
+  if (!(p.x >= 1.f && p.x < (float)(vw.W - 1) && p.y >= 1.f && p.y < (float)(vw.H - 1))) return false;
+  const int u = (int)p.x, r0 = (int)p.y;
+  const int wa = (u - 1) >> 5, wb = (u + 1) >> 5, sh = (u - 1) & 31;
+  const unsigned int* m = d.fgmask + vw.mask0 + (long long)(r0 - 1) * vw.mpitch;
+  unsigned int all = 7u;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const unsigned int* row = m + (long long)r * vw.mpitch;
+    all &= (unsigned int)((((unsigned long long)row[wb] << 32) | row[wa]) >> sh);
   }
-  return fl;
+  return (all & 7u) != 7u;
 }
 
 __device__ __forceinline__ float remap_p(int v) { return v >= 127 ? (float)((v - 127) * 2) : (float)v; }
@@ -1440,7 +1474,7 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
     const long long pi = vw.vn0 + v;
     const float* cr = io.crops + (size_t)w * 7;
     const f4 p = d.proj[pi];
-    const bool isb = boundary_p(io, vw, p);
+    const bool isb = boundary_p(d, vw, p);
     d.isb[pi] = isb ? 1 : 0;
     if (isb) {
       const int cl = cell_of_p(grid_geom_p(cr), vw.gx, vw.gy, p.x, p.y);
@@ -2032,7 +2066,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   std::vector<int> jc_off(1, 0);
   std::vector<int4> jchunk;
   std::vector<float> n2v_w;
-  long long vn = 0, vf = 0, pix = 0, sil = 0, tiles = 0, cells = 0;
+  long long vn = 0, vf = 0, pix = 0, sil = 0, tiles = 0, cells = 0, mask_words = 0;
   int maxN = 0, maxF = 0, maxM = 0, maxS2 = 0, maxJ = 0, maxTiles = 0;
   for (int b = 0; b < n_chars; ++b) {
     const dcx_char_desc& c = chars[b];
@@ -2117,6 +2151,9 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
       ViewD v{};
       v.ch = b; v.W = c.width; v.H = c.height; v.S = c.dt_size;
       v.vn0 = vn; v.vf0 = vf; v.pix0 = pix; v.sil0 = sil; v.cell0 = cells;
+      v.mpitch = 2 * (int)ceil_div(c.width, 64);
+      v.mask0 = mask_words;
+      mask_words += (long long)v.mpitch * c.height;
       v.gx = c.dt_size * kRing / 30 + 2 * kGridMargin + 3; v.gy = v.gx;
       vn += d.N; vf += d.F; pix += (long long)c.width * c.height; sil += (long long)c.dt_size * c.dt_size;
       cells += (long long)v.gx * v.gy;
@@ -2206,6 +2243,7 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.ke, sizeof(float) * 12 * (size_t)p->V);
   alloc(&d.nrm2, sizeof(f2) * (size_t)vn);
   alloc(&d.tri, sizeof(TriRec) * (size_t)vf);
+  alloc(&d.trange, sizeof(uint2) * (size_t)vf);
   alloc(&d.tri_blk, 4 * (size_t)p->V * (size_t)ceil_div(maxF, kB));
   alloc(&d.tri_base, 4 * (size_t)(vf + 1));
   alloc(&d.tile_cnt, 4 * (size_t)tiles);
@@ -2218,6 +2256,8 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.rec, sizeof(float) * kGrad * (size_t)cap);
   alloc(&d.rec_vis, (size_t)cap);
   alloc(&d.isb, (size_t)vn);
+  alloc(&d.fgmask, 4 * (size_t)mask_words);
+  d.totMask = mask_words;
   alloc(&d.cell_cnt, 4 * (size_t)cells);
   alloc(&d.cell_off, 4 * (size_t)(cells + 1));
   alloc(&d.cell_fill, 4 * (size_t)cells);
@@ -2293,6 +2333,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   add(d.cell_cnt, d.totCells);
   add(d.cell_fill, d.totCells);
   add(d.heavy_n, 1);
+  add(d.fgmask, d.totMask);
   int stage = 0;
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
   const int B = p->B, V = p->V;
