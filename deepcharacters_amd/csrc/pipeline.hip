@@ -2337,123 +2337,178 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   int stage = 0;
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[stage], s); stage++; };
   const int B = p->B, V = p->V;
-  // Eager and captured steps run on two streams: the side stream takes the
-  // vertex normals, their image-space directions and the vertex projections
-  // (read by the fine kernel and the silhouette stages; the binning projects
-  // its corners itself) beside the binning, the per-face record sums and the
-  // raster part of the vertex backward beside the silhouette stages, the loss
-  // totals, and the joint-DQ reduction beside the node gradients.  (A join
-  // costs ~10-20 us in a replayed graph, so the buffer clears run inside the
-  // vertex kernel on s.)  Each fork is an event recorded on s and waited on by
-  // the side stream right away; each join is an event of its own.  A profiled
-  // step keeps every stage on one stream so that its events bracket it.
-  const bool fork = ev == nullptr;
-  auto to_side = [&]() -> dcx_status {
-    DCX_HIP(hipEventRecord(p->ev_fork, s));
-    DCX_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-    return DCX_OK;
+  // the launches of the stages, each on a given stream
+  auto k_node_fwd = [&](hipStream_t q) {
+    hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, q, d, io);
   };
-  hipStream_t sn = fork ? p->side : s;
-  mark();
-  mark();  // the clears run inside vertex_fwd_kernel
-  hipLaunchKernelGGL(node_fwd_kernel, dim3(ceil_div(d.maxM, kB), B), dim3(kB), 0, s, d, io);
-  mark();
-  hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, s, d, io, L);
-  mark();
-  if (fork) {
-    if (dcx_status e = to_side()) return e;
-  }
-  hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, sn, d);
-  mark();
-  hipLaunchKernelGGL(view_normal_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, sn, d, io);
-  mark();
-  hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, sn, d, io);
-  if (fork) DCX_HIP(hipEventRecord(p->ev_njoin, p->side));
-  mark();
-  hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d, io);
-  mark();
-  {
-    // tile counts -> tile offsets, and in the same first launch the
-    // per-workgroup record-slot counts -> record-slot offsets
+  auto k_vertex_fwd = [&](hipStream_t q) {
+    hipLaunchKernelGGL(vertex_fwd_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, q, d, io, L);
+  };
+  auto k_normals = [&](hipStream_t q) {
+    hipLaunchKernelGGL(normals_kernel, dim3(ceil_div(d.maxN, kB), B), dim3(kB), 0, q, d);
+  };
+  auto k_view_normal = [&](hipStream_t q) {
+    hipLaunchKernelGGL(view_normal_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, q, d, io);
+  };
+  auto k_project = [&](hipStream_t q) {
+    hipLaunchKernelGGL(project_kernel, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, q, d, io);
+  };
+  auto k_tri_count = [&](hipStream_t q) {
+    hipLaunchKernelGGL(tri_count_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, q, d, io);
+  };
+  // tile counts -> tile offsets, and in the same first launch the
+  // per-workgroup record-slot counts -> record-slot offsets
+  auto k_tile_scan = [&](hipStream_t q) -> dcx_status {
     const long long nt = d.totTiles, nb = (nt + kScanChunk - 1) / kScanChunk;
     if (nb < 1 || nb >= (1ll << 31) - 1) return set_error(DCX_ERR_INVALID, "scan: bad length %lld", nt);
-    hipLaunchKernelGGL(scan_sum_with_one_kernel, dim3((unsigned)nb + 1), dim3(kB), 0, s, d.tile_cnt, d.scan_aux, nt,
+    hipLaunchKernelGGL(scan_sum_with_one_kernel, dim3((unsigned)nb + 1), dim3(kB), 0, q, d.tile_cnt, d.scan_aux, nt,
                        d.tri_blk, (long long)V * ceil_div(d.maxF, kB), d.tri_base + d.totVF);
-    hipLaunchKernelGGL(tile_scan_apply_kernel, dim3((unsigned)nb), dim3(kB), 0, s, d.tile_cnt, d.tile_off, d.scan_aux,
+    hipLaunchKernelGGL(tile_scan_apply_kernel, dim3((unsigned)nb), dim3(kB), 0, q, d.tile_cnt, d.tile_off, d.scan_aux,
                        nt, d.heavy, d.heavy_n, d.capacity);
-    if (dcx_status e = check_launch("scan")) return e;
-  }
-  mark();
-  mark();  // the record-slot scan ran inside the tile scan's first launch
-  dcx_status st = DCX_OK;
-  hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, s, d);
-  mark();
-  // the fine kernel reads the side stream's projections, K [R | t] and vertex normals
-  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
-  {
+    return check_launch("scan");
+  };
+  auto k_bin_scatter = [&](hipStream_t q) {
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(ceil_div(d.maxF, kB), V), dim3(kB), 0, q, d);
+  };
+  auto k_fine = [&](hipStream_t q) {
     // the slow-path tiles' workgroups first, then per view a multiple of kXcd
     const dim3 grid((unsigned)(kHeavyCap + (long long)ceil_div(d.maxTiles, kXcd) * kXcd * V));
     const dim3 blk(kTilePix);
     if (io.uv || io.nrm_out) {
-      if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, true>), grid, blk, 0, s, d, io);
-      else if (d.tws == 6) hipLaunchKernelGGL((fine_kernel<6, true>), grid, blk, 0, s, d, io);
-      else hipLaunchKernelGGL((fine_kernel<5, true>), grid, blk, 0, s, d, io);
+      if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, true>), grid, blk, 0, q, d, io);
+      else if (d.tws == 6) hipLaunchKernelGGL((fine_kernel<6, true>), grid, blk, 0, q, d, io);
+      else hipLaunchKernelGGL((fine_kernel<5, true>), grid, blk, 0, q, d, io);
     } else {
-      if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, false>), grid, blk, 0, s, d, io);
-      else if (d.tws == 6) hipLaunchKernelGGL((fine_kernel<6, false>), grid, blk, 0, s, d, io);
-      else hipLaunchKernelGGL((fine_kernel<5, false>), grid, blk, 0, s, d, io);
+      if (d.tws == 4) hipLaunchKernelGGL((fine_kernel<4, false>), grid, blk, 0, q, d, io);
+      else if (d.tws == 6) hipLaunchKernelGGL((fine_kernel<6, false>), grid, blk, 0, q, d, io);
+      else hipLaunchKernelGGL((fine_kernel<5, false>), grid, blk, 0, q, d, io);
     }
-  }
-  mark();
-  if (fork) {
-    // the raster part of the vertex backward follows the per-face sums there
-    if (dcx_status e = to_side()) return e;
-    hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, p->side, d);
-    hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, p->side, d, io);
-    DCX_HIP(hipEventRecord(p->ev_join, p->side));
-  }
-  hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, s, d, io);
-  mark();
-  st = scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, s);
-  if (st) return st;
-  mark();
-  hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, s, d, io);
-  mark();
-  hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, s, d, io);
-  mark();
-  if (fork) {
-    // every loss partial is written: the totals go to the side stream
-    if (dcx_status e = to_side()) return e;
-    hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, p->side, d, io);
-    DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // face_sum and the raster vertex part done
-  } else {
-    hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, s, d);
+  };
+  auto k_face_sum = [&](hipStream_t q) {
+    hipLaunchKernelGGL(face_sum_kernel, dim3(ceil_div(d.maxF, kB), B), dim3(kB), 0, q, d);
+  };
+  auto k_vertex_bwd_raster = [&](hipStream_t q) {
+    hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, q, d, io);
+  };
+  auto k_sil_vertex = [&](hipStream_t q) {
+    hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, q, d, io);
+  };
+  auto k_sil_grid_scan = [&](hipStream_t q) { return scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, q); };
+  auto k_sil_grid_scatter = [&](hipStream_t q) {
+    hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, q, d, io);
+  };
+  auto k_sil_pixel = [&](hipStream_t q) {
+    hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, q, d, io);
+  };
+  auto k_vertex_bwd_sil = [&](hipStream_t q) {
+    hipLaunchKernelGGL(vertex_bwd_kernel<true>, dim3(d.gxN, B), dim3(kB), 0, q, d, io);
+  };
+  // (graph skinning has no joint DQs: the two joint stages stay empty)
+  auto k_joint_partial = [&](hipStream_t q) {
+    if (!graph_skin)
+      hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, q, d,
+                         p->nChunk);
+  };
+  auto k_joint_reduce = [&](hipStream_t q) {
+    if (!graph_skin)
+      hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, q, d, io,
+                         p->sumJ);
+  };
+  auto k_node_bwd = [&](hipStream_t q) {
+    hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, q, d, io, p->sumM);
+  };
+  auto k_finalize = [&](hipStream_t q) { hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, q, d, io); };
+
+  if (ev) {
+    // A profiled step: every stage on s, in stage order, bracketed by events.
     mark();
-    hipLaunchKernelGGL(vertex_bwd_kernel<false>, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
+    mark();  // the clears run inside vertex_fwd_kernel
+    k_node_fwd(s); mark();
+    k_vertex_fwd(s); mark();
+    k_normals(s); mark();
+    k_view_normal(s); mark();
+    k_project(s); mark();
+    k_tri_count(s); mark();
+    if (dcx_status e = k_tile_scan(s)) return e;
+    mark();
+    mark();  // the record-slot scan ran inside the tile scan's first launch
+    k_bin_scatter(s); mark();
+    k_fine(s); mark();
+    k_sil_vertex(s); mark();
+    if (dcx_status e = k_sil_grid_scan(s)) return e;
+    mark();
+    k_sil_grid_scatter(s); mark();
+    k_sil_pixel(s); mark();
+    k_face_sum(s); mark();
+    k_vertex_bwd_raster(s); mark();
+    k_vertex_bwd_sil(s); mark();
+    k_joint_partial(s); mark();
+    k_joint_reduce(s); mark();
+    k_node_bwd(s); mark();
+    k_finalize(s); mark();
+    return check_launch("dcx_pipeline_step");
   }
-  mark();
-  hipLaunchKernelGGL(vertex_bwd_kernel<true>, dim3(d.gxN, B), dim3(kB), 0, s, d, io);
-  mark();
-  hipStream_t sj = s;
-  if (fork) {
-    if (dcx_status e = to_side()) return e;
-    sj = p->side;
-  }
-  // (graph skinning has no joint DQs: the two stages stay empty)
-  if (!graph_skin)
-    hipLaunchKernelGGL(joint_partial_kernel, dim3(ceil_div((long long)p->nChunk * 64, kB)), dim3(kB), 0, sj, d,
-                       p->nChunk);
-  mark();
-  if (!graph_skin)
-    hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, sj, d, io,
-                       p->sumJ);
-  if (fork) DCX_HIP(hipEventRecord(p->ev_join, p->side));
-  mark();
-  hipLaunchKernelGGL(node_bwd_kernel, dim3(ceil_div((long long)p->sumM * 64, kB)), dim3(kB), 0, s, d, io, p->sumM);
-  mark();
-  if (!fork) hipLaunchKernelGGL(finalize_kernel, dim3(B), dim3(kB), 0, s, d, io);
-  if (fork) DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // joint reduction (and all earlier side work) done
-  mark();
+
+  // Eager and captured steps run on two streams.  The side stream takes the
+  // vertex normals, their image-space directions and the vertex projections
+  // (read by the fine kernel and the silhouette stages; the binning projects
+  // its corners itself) beside the binning, the per-face record sums and the
+  // raster part of the vertex backward beside the silhouette stages, and the
+  // loss totals and the node gradients beside the joint-DQ reduction.  Each
+  // fork records an event on s and, before the side stream waits for it,
+  // launches the next critical-path stage on s, so that in a captured graph
+  // that stage is the fork's first dependent: the graph keeps a node's first
+  // dependent on the node's own hardware queue, and every hand-over between
+  // queues costs ~10-20 us, so the critical path stays on one queue.  Each
+  // join is an event of its own.  The buffer clears run inside the vertex
+  // kernel on s (a join would cost more than they do).
+  hipStream_t sd = p->side;
+  auto fork_rec = [&]() -> dcx_status {
+    DCX_HIP(hipEventRecord(p->ev_fork, s));
+    return DCX_OK;
+  };
+  auto fork_wait = [&]() -> dcx_status {
+    DCX_HIP(hipStreamWaitEvent(sd, p->ev_fork, 0));
+    return DCX_OK;
+  };
+  k_node_fwd(s);
+  k_vertex_fwd(s);
+  if (dcx_status e = fork_rec()) return e;
+  k_tri_count(s);
+  if (dcx_status e = fork_wait()) return e;
+  k_normals(sd);
+  k_view_normal(sd);
+  k_project(sd);
+  DCX_HIP(hipEventRecord(p->ev_njoin, sd));
+  if (dcx_status e = k_tile_scan(s)) return e;
+  k_bin_scatter(s);
+  // the fine kernel reads the side stream's projections, K [R | t] and vertex normals
+  DCX_HIP(hipStreamWaitEvent(s, p->ev_njoin, 0));
+  k_fine(s);
+  if (dcx_status e = fork_rec()) return e;
+  k_sil_vertex(s);
+  if (dcx_status e = fork_wait()) return e;
+  // the raster part of the vertex backward follows the per-face sums there
+  k_face_sum(sd);
+  k_vertex_bwd_raster(sd);
+  DCX_HIP(hipEventRecord(p->ev_join, sd));
+  if (dcx_status e = k_sil_grid_scan(s)) return e;
+  k_sil_grid_scatter(s);
+  k_sil_pixel(s);
+  // every loss partial is written: the totals go to the side stream
+  if (dcx_status e = fork_rec()) return e;
+  DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // face_sum and the raster vertex part done
+  k_vertex_bwd_sil(s);
+  if (dcx_status e = fork_wait()) return e;
+  k_finalize(sd);
+  // the node gradients read the vertex pass's ED-position gradient
+  if (dcx_status e = fork_rec()) return e;
+  k_joint_partial(s);
+  if (dcx_status e = fork_wait()) return e;
+  k_node_bwd(sd);
+  DCX_HIP(hipEventRecord(p->ev_join, sd));
+  k_joint_reduce(s);
+  DCX_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // the node gradients and the loss totals (all side work) done
   return check_launch("dcx_pipeline_step");
 }
 
