@@ -149,10 +149,11 @@ struct Dev {
                           // has a face (the boundary test's 3x3 reads stay in a mask 1/32 of the face
                           // buffer's size)
   long long totMask;
-  unsigned int* cell_cnt; // [totCells]
-  unsigned int* cell_off; // [totCells+1]
-  unsigned int* cell_fill;
-  f4* cell_pts;           // [totVN] (x, y, point id bits, 0) grouped by cell
+  unsigned int* cell_cnt; // [totCells] boundary points that fell in each cell (may exceed kCellCap)
+  f4* cell_pts;           // [totCells * kCellCap] a cell's first kCellCap points (x, y, point id bits, 0),
+                          // in arrival order
+  f4* ov_pts;             // [totVN] per view (from vn0) the points that found their cell full
+  unsigned int* ov_cnt;   // [V]
   unsigned int* scan_aux;
   unsigned int* heavy;     // [kHeavyCap] global indices of the slow-path tiles (any order)
   unsigned int* heavy_n;   // number of slow-path tiles (may exceed kHeavyCap: then none is listed)
@@ -1441,6 +1442,9 @@ __device__ __forceinline__ float remap_p(int v) { return v >= 127 ? (float)((v -
 // pixel lies inside the grid.
 constexpr int kRing = 8;
 constexpr int kGridMargin = kRing + 1;
+// Point slots per grid cell; a point that finds its cell full goes to its
+// view's overflow list, which every search scans in full.
+constexpr int kCellCap = 32;
 struct GridGeomP { float ox, oy, cell; };
 __device__ __forceinline__ GridGeomP grid_geom_p(const float* cr) {
   const float sc = cr[2];
@@ -1477,8 +1481,17 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
     const bool isb = boundary_p(d, vw, p);
     d.isb[pi] = isb ? 1 : 0;
     if (isb) {
+      // the point goes straight into a slot of its cell (or the overflow
+      // list): the nearest-point search takes the minimum of (distance,
+      // index), which does not depend on the order of a cell's points
       const int cl = cell_of_p(grid_geom_p(cr), vw.gx, vw.gy, p.x, p.y);
-      if (cl >= 0) atomicAdd(d.cell_cnt + vw.cell0 + cl, 1u);
+      if (cl >= 0) {
+        const long long cell = vw.cell0 + cl;
+        const unsigned int k = atomicAdd(d.cell_cnt + cell, 1u);
+        const f4 pt = mk4(p.x, p.y, __int_as_float(v), 0.f);
+        if (k < (unsigned int)kCellCap) d.cell_pts[cell * kCellCap + k] = pt;
+        else d.ov_pts[vw.vn0 + atomicAdd(d.ov_cnt + w, 1u)] = pt;  // at most one entry per vertex
+      }
     }
     const float pu = p.x, pv = p.y;
     const bool in_view = pu - 2 > cr[3] && pu + 2 < cr[5] && pv - 2 > cr[4] && pv + 2 < cr[6];
@@ -1521,24 +1534,10 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
   if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
-  const int w = blockIdx.y;
-  const ViewD vw = d.vw[w];
-  const CharD c = d.ch[vw.ch];
-  const int v = blockIdx.x * kB + threadIdx.x;
-  if (v >= c.N || !d.isb[vw.vn0 + v]) return;
-  const f4 p = d.proj[vw.vn0 + v];
-  const int cl = cell_of_p(grid_geom_p(io.crops + (size_t)w * 7), vw.gx, vw.gy, p.x, p.y);
-  if (cl < 0) return;
-  const long long cell = vw.cell0 + cl;
-  d.cell_pts[d.cell_off[cell] + atomicAdd(d.cell_fill + cell, 1u)] = mk4(p.x, p.y, __int_as_float(v), 0.f);
-}
-
 constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
 constexpr float kResidualFx = 1099511627776.f;  // 2^40: fixed-point unit of the data->model residual sums
 constexpr int kDtApron = kDtTile + 4;  // the tile plus the 2-pixel border of the 5x5 boundary test
 constexpr int kMaxRegionPts = 1280;  // regions with more points search the global grid
-constexpr int kMaxRegionRows = 64;
 constexpr int kMaxRegionCells = 1024;  // entries of the per-cell LDS offset table (rows x (cols + 1))
 
 __device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, float& bd, int& best, float& bx,
@@ -1566,8 +1565,6 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
   __shared__ f2 s_pxy[kMaxRegionPts];
   __shared__ int s_pid[kMaxRegionPts];
   __shared__ unsigned int s_coff[kMaxRegionCells];
-  __shared__ unsigned int s_rstart[kMaxRegionRows + 1];
-  __shared__ unsigned int s_ra[kMaxRegionRows];
   __shared__ unsigned char s_dt[kDtApron * kDtApron];
   __shared__ unsigned char s_bp[kDtTile * kDtTile];  // the tile's boundary pixels, compacted
   __shared__ unsigned int s_wave[kB / 64];
@@ -1621,38 +1618,49 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
     const int cy0 = max(0, (int)floorf((ylo - gg.oy) / gg.cell) - 1);
     const int cy1 = min(vw.gy - 1, (int)floorf((yhi - gg.oy) / gg.cell) + 1);
     const int nrows = cy1 - cy0 + 1, ncols = cx1 - cx0 + 1;
-    const bool shape_ok = nrows > 0 && ncols > 0 && nrows <= kMaxRegionRows && nrows * (ncols + 1) <= kMaxRegionCells;
-    unsigned int len = 0;
-    if (shape_ok && (int)threadIdx.x < nrows) {
-      const long long rc = vw.cell0 + (long long)(cy0 + (int)threadIdx.x) * vw.gx;
-      const unsigned int a = d.cell_off[rc + cx0];
-      len = d.cell_off[rc + cx1 + 1] - a;
-      s_ra[threadIdx.x] = a;
-    }
+    const int ncell = nrows * (ncols + 1);  // region cells row-major, plus an end entry per row
+    const bool shape_ok = nrows > 0 && ncols > 0 && ncell <= kMaxRegionCells;
+    // the region's cell counts (capped), scanned in row-major order: entry
+    // (r, k) of s_coff is the LDS offset of cell (cy0 + r, cx0 + k), entry
+    // (r, ncols) the end of row r; thread t scans entries 4t .. 4t + 3
+    unsigned int cnt4[4] = {0u, 0u, 0u, 0u}, part = 0u;
+    if (shape_ok)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = 4 * (int)threadIdx.x + j;
+        const int r = i / (ncols + 1), k = i - r * (ncols + 1);
+        if (i < ncell && k < ncols)
+          cnt4[j] = min(d.cell_cnt[vw.cell0 + (long long)(cy0 + r) * vw.gx + cx0 + k], (unsigned int)kCellCap);
+        part += cnt4[j];
+      }
     unsigned int total;
-    const unsigned int ex = block_scan256(len, s_wave, total);
-    if (threadIdx.x < kMaxRegionRows) s_rstart[threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_rstart[kMaxRegionRows] = total;
+    unsigned int ex = block_scan256(part, s_wave, total);
+    if (shape_ok)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = 4 * (int)threadIdx.x + j;
+        if (i < ncell) s_coff[i] = ex;
+        ex += cnt4[j];
+      }
     __syncthreads();
     const bool in_lds = shape_ok && total <= (unsigned int)kMaxRegionPts;
-    if (in_lds) {
-      // LDS offset of cell (cy0 + r, cx0 + k): row start + the cell's offset within its row
-      for (int i = threadIdx.x; i < nrows * (ncols + 1); i += kB) {
-        const int r = i / (ncols + 1), k = i - r * (ncols + 1);
-        const long long rc = vw.cell0 + (long long)(cy0 + r) * vw.gx;
-        s_coff[i] = s_rstart[r] + (d.cell_off[rc + cx0 + k] - s_ra[r]);
-      }
+    const unsigned int n_ov = d.ov_cnt[w];  // points of this view whose cell was full (searched by every pixel)
+    if (in_lds)
       for (unsigned int i = threadIdx.x; i < total; i += kB) {
-        int r = 0, rh = nrows - 1;  // the last row starting at or before i
-        while (r < rh) {
-          const int mid = (r + rh + 1) >> 1;
-          if (s_rstart[mid] <= i) r = mid; else rh = mid - 1;
+        // the entry holding point i: the last one whose offset is <= i (an
+        // entry holding points is followed by a larger offset; row ends and
+        // empty cells share the offset of the next entry)
+        int e = 0, eh = ncell - 1;
+        while (e < eh) {
+          const int mid = (e + eh + 1) >> 1;
+          if (s_coff[mid] <= i) e = mid; else eh = mid - 1;
         }
-        const f4 it = d.cell_pts[s_ra[r] + (i - s_rstart[r])];
+        const int r = e / (ncols + 1), k = e - r * (ncols + 1);
+        const long long cell = vw.cell0 + (long long)(cy0 + r) * vw.gx + cx0 + k;
+        const f4 it = d.cell_pts[cell * kCellCap + (i - s_coff[e])];
         s_pxy[i] = mk2(it.x, it.y);
         s_pid[i] = __float_as_int(it.z);
       }
-    }
     __syncthreads();
     // compact the boundary pixels, then give each a group of L lanes (a power
     // of two, L * count <= 256, at most a wavefront) that stride the points
@@ -1705,6 +1713,11 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
           gbd = fminf(gbd, bd);
           for (int o = L >> 1; o > 0; o >>= 1) gbd = fminf(gbd, __shfl_xor(gbd, o, 64));
         }
+        for (unsigned int q = (unsigned int)sub; q < n_ov; q += (unsigned int)L) {
+          const f4 it = d.ov_pts[vw.vn0 + q];
+          const float dx = it.x - ix, dyy = it.y - iy;
+          nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+        }
         for (int o = L >> 1; o > 0; o >>= 1) {
           const float obd = __shfl_xor(bd, o, 64), obx = __shfl_xor(bx, o, 64), oby = __shfl_xor(by, o, 64);
           const int ob = __shfl_xor(best, o, 64);
@@ -1720,9 +1733,9 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
               const int xx = cx + ox;
               if (xx < 0 || xx >= vw.gx) continue;
               const long long cell = vw.cell0 + yy * vw.gx + xx;
-              const unsigned int q1 = d.cell_off[cell + 1];
-              for (unsigned int q = d.cell_off[cell]; q < q1; ++q) {
-                const f4 it = d.cell_pts[q];
+              const unsigned int q1 = min(d.cell_cnt[cell], (unsigned int)kCellCap);
+              for (unsigned int q = 0; q < q1; ++q) {
+                const f4 it = d.cell_pts[cell * kCellCap + q];
                 const float dx = it.x - ix, dyy = it.y - iy;
                 nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
               }
@@ -1730,6 +1743,11 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
           }
           const float reach = (float)k * gg.cell * (1.f - 1e-4f);
           if (bd < reach || reach >= R) break;
+        }
+        for (unsigned int q = 0; q < n_ov; ++q) {
+          const f4 it = d.ov_pts[vw.vn0 + q];
+          const float dx = it.x - ix, dyy = it.y - iy;
+          nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
         }
       }
       if (sub == 0 && best >= 0) {
@@ -2259,9 +2277,9 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.fgmask, 4 * (size_t)mask_words);
   d.totMask = mask_words;
   alloc(&d.cell_cnt, 4 * (size_t)cells);
-  alloc(&d.cell_off, 4 * (size_t)(cells + 1));
-  alloc(&d.cell_fill, 4 * (size_t)cells);
-  alloc(&d.cell_pts, sizeof(f4) * (size_t)vn);
+  alloc(&d.cell_pts, sizeof(f4) * kCellCap * (size_t)cells);
+  alloc(&d.ov_pts, sizeof(f4) * (size_t)vn);
+  alloc(&d.ov_cnt, 4 * (size_t)p->V);
   alloc(&d.corner_sum, sizeof(float) * kGrad * (size_t)p->sumF);
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
@@ -2305,7 +2323,7 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "view_normals", "project",
     "tri_count",      "tile_scan",     "tri_scan",         "bin_scatter", "fine",       "sil_vertex",
-    "sil_grid_scan",  "sil_grid_scatter", "sil_pixel",     "face_sum",  "vertex_bwd_raster",
+    "sil_pixel",      "face_sum",      "vertex_bwd_raster",
     "vertex_bwd_sil", "joint_partial", "joint_reduce",     "node_bwd",  "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
@@ -2331,7 +2349,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   add(d.tile_cnt, d.totTiles);
   add(d.tile_fill, d.totTiles);
   add(d.cell_cnt, d.totCells);
-  add(d.cell_fill, d.totCells);
+  add(d.ov_cnt, p->V);
   add(d.heavy_n, 1);
   add(d.fgmask, d.totMask);
   int stage = 0;
@@ -2393,10 +2411,6 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   auto k_sil_vertex = [&](hipStream_t q) {
     hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, q, d, io);
   };
-  auto k_sil_grid_scan = [&](hipStream_t q) { return scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, q); };
-  auto k_sil_grid_scatter = [&](hipStream_t q) {
-    hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, q, d, io);
-  };
   auto k_sil_pixel = [&](hipStream_t q) {
     hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, q, d, io);
   };
@@ -2435,9 +2449,6 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     k_bin_scatter(s); mark();
     k_fine(s); mark();
     k_sil_vertex(s); mark();
-    if (dcx_status e = k_sil_grid_scan(s)) return e;
-    mark();
-    k_sil_grid_scatter(s); mark();
     k_sil_pixel(s); mark();
     k_face_sum(s); mark();
     k_vertex_bwd_raster(s); mark();
@@ -2492,8 +2503,6 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   k_face_sum(sd);
   k_vertex_bwd_raster(sd);
   DCX_HIP(hipEventRecord(p->ev_join, sd));
-  if (dcx_status e = k_sil_grid_scan(s)) return e;
-  k_sil_grid_scatter(s);
   k_sil_pixel(s);
   // every loss partial is written: the totals go to the side stream
   if (dcx_status e = fork_rec()) return e;
