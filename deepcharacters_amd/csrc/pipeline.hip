@@ -1934,10 +1934,21 @@ __global__ __launch_bounds__(kB) void joint_partial_kernel(Dev d, int nChunk) {
   if (ck >= nChunk) return;
   const int4 r = d.jchunk[ck];
   float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int e = r.x + lane; e < r.y; e += 64) {
-    const int ent = d.jv_ent[e];
-    const size_t gv = (size_t)r.z + (ent >> 2);
-    const float ws = d.vws[KS * gv + (ent & 3)];
+  // a chunk has at most kJChunk = 8 x 64 entries: every lane's (at most 8)
+  // entry ids are loaded first, so the entries' data loads do not wait on
+  // one another; the sum still runs over the lane's entries in order
+  static_assert(kJChunk <= 8 * 64, "a lane takes at most 8 entries of a chunk");
+  int ent[8];
+#pragma unroll
+  for (int it = 0; it < 8; ++it) {
+    const int e = r.x + lane + 64 * it;
+    ent[it] = e < r.y ? d.jv_ent[e] : -1;
+  }
+#pragma unroll
+  for (int it = 0; it < 8; ++it) {
+    if (ent[it] < 0) break;
+    const size_t gv = (size_t)r.z + (ent[it] >> 2);
+    const float ws = d.vws[KS * gv + (ent[it] & 3)];
     const f4* q8 = (const f4*)(d.vq + 8 * gv);
     const f4 qa = q8[0], qb = q8[1];
     acc[0] += ws * qa.x; acc[1] += ws * qa.y; acc[2] += ws * qa.z; acc[3] += ws * qa.w;
@@ -1953,15 +1964,29 @@ __global__ __launch_bounds__(kB) void joint_partial_kernel(Dev d, int nChunk) {
   }
 }
 
-// Pass 2: one lane per (joint, component), the joint's chunk partials summed
-// in chunk order (a joint without entries gets 0).
+// Pass 2: one wavefront per joint: lane l sums the joint's chunk partials
+// l, l + 64, ... in order, then the lanes are reduced across the wavefront in
+// a fixed order (a joint without entries gets 0).
 __global__ __launch_bounds__(kB) void joint_reduce_kernel(Dev d, IO io, int totJ) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= 8 * totJ) return;
-  const int j = i >> 3, q = i & 7;
-  float sum = 0.f;
-  for (int k = d.jc_off[j]; k < d.jc_off[j + 1]; ++k) sum += d.jpart[8 * (size_t)k + q];
-  io.g_dq[i] = sum;
+#pragma clang fp contract(fast)
+  const int j = (blockIdx.x * kB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (j >= totJ) return;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int k = d.jc_off[j] + lane; k < d.jc_off[j + 1]; k += 64) {
+    const f4* pp = reinterpret_cast<const f4*>(d.jpart + 8 * (size_t)k);
+    const f4 a = pp[0], b = pp[1];
+    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = wave_sum(acc[q]);
+  if (lane < 8) {
+    float v = acc[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) v = lane == q ? acc[q] : v;
+    io.g_dq[8 * (size_t)j + lane] = v;
+  }
 }
 
 __global__ __launch_bounds__(kB) void node_bwd_kernel(Dev d, IO io, int totM) {
@@ -2001,8 +2026,16 @@ __global__ __launch_bounds__(kB) void finalize_kernel(Dev d, IO io) {
   const int b = blockIdx.x;
   const CharD c = d.ch[b];
   const ViewD v0 = d.vw[c.view0], v1 = d.vw[c.view0 + c.C - 1];
-  float s = 0.f;
-  for (long long t = v0.tile0 + threadIdx.x; t < v1.tile0 + v1.tiles; t += kB) s += d.loss_tile[t];
+  // the character's tile partials: 8 independent running sums per lane (8
+  // loads in flight), added in a fixed order
+  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const long long tend = v1.tile0 + v1.tiles;
+  long long t = v0.tile0 + threadIdx.x;
+  for (; t + 7 * kB < tend; t += 8 * kB)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += d.loss_tile[t + u * kB];
+  for (; t < tend; t += kB) a[0] += d.loss_tile[t];
+  float s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   const float lc = block_sum(s, s_red);
   s = 0.f;
   for (long long i = (long long)c.view0 * d.gxN + threadIdx.x; i < (long long)(c.view0 + c.C) * d.gxN; i += kB)
@@ -2425,7 +2458,7 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   };
   auto k_joint_reduce = [&](hipStream_t q) {
     if (!graph_skin)
-      hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 8, kB)), dim3(kB), 0, q, d, io,
+      hipLaunchKernelGGL(joint_reduce_kernel, dim3(ceil_div((long long)p->sumJ * 64, kB)), dim3(kB), 0, q, d, io,
                          p->sumJ);
   };
   auto k_node_bwd = [&](hipStream_t q) {
