@@ -323,6 +323,9 @@ def main():
                     help="skip the plain-forward, single-character, temporal and stress workloads")
     ap.add_argument("--tile", default="auto", choices=["auto", "16", "32", "64"],
                     help="raster tile width (tiles of w x 256/w pixels); auto = time each and keep the fastest")
+    ap.add_argument("--grid", default="auto",
+                    help="silhouette grid layout: slots per cell, 0 = counting sort; auto = the autotuner's "
+                         "choice (counting sort when the tile is given)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -365,6 +368,9 @@ def main():
                 + f" -> {tuned['tile_w']}x{256 // tuned['tile_w']}")
         else:
             bp.set_tile(int(args.tile))
+        if args.grid != "auto":
+            bp.set_grid(int(args.grid))
+        log(f"silhouette grid: {bp.grid_cap} slots per cell" if bp.grid_cap else "silhouette grid: counting sort")
         bp.step()
         stream.synchronize()
         bp.capture(stream)
@@ -436,6 +442,7 @@ def main():
                    "faces": int(sum(w.ch.F for w in works)),
                    "ed_nodes": int(sum(w.ch.graph.num_nodes for w in works)),
                    "tile": f"{bp.tile_width}x{256 // bp.tile_width}",
+                   "silhouette_grid": (f"{bp.grid_cap} slots per cell" if bp.grid_cap else "counting sort"),
                    "autotune_ms_per_tile": ({f"{w}x{256 // w}": ms for w, ms in tuned["ms"].items()}
                                             if tuned else None)},
         "plain": plain,

@@ -44,7 +44,7 @@ SIGNATURES: dict[str, str] = {
     "dcx_pipeline_step": "ppS",
     "dcx_pipeline_capture": "ppS",
     "dcx_pipeline_replay": "pS",
-    "dcx_pipeline_profile": "pppiS", "dcx_pipeline_set_tile": "pi", "dcx_projected_mesh_boundary": "piiiippppppzS", "dcx_pipeline_autotune": "ppiSppp",
+    "dcx_pipeline_profile": "pppiS", "dcx_pipeline_set_tile": "pi", "dcx_pipeline_set_grid": "pi", "dcx_projected_mesh_boundary": "piiiippppppzS", "dcx_pipeline_autotune": "ppiSppp",
     "dcx_skeleton_create": "iipppppipppp",
     "dcx_skeleton_destroy": "p",
     "dcx_fk": "pippppS", "dcx_fk_transforms": "pippppS", "dcx_pose_to_graph": "ppppiiiiippS", "dcx_arap_fwd": "ppppppiiipppppS",
@@ -66,7 +66,7 @@ _RESTYPES = {"dcx_render_workspace_size": ctypes.c_size_t, "dcx_ed_workspace_siz
              "dcx_pipeline_stage_name": ctypes.c_char_p, "dcx_normal_map_workspace_size": ctypes.c_size_t,
              "dcx_edt_workspace_size": ctypes.c_size_t, "dcx_dtloss_workspace_size": ctypes.c_size_t,
              "dcx_render_bwd_workspace_size": ctypes.c_size_t,
-             "dcx_pipeline_tile_width": ctypes.c_int,
+             "dcx_pipeline_tile_width": ctypes.c_int, "dcx_pipeline_grid_cap": ctypes.c_int,
              "dcx_projected_mesh_boundary_workspace_size": ctypes.c_size_t}
 _EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
                    "dcx_ed_workspace_size": [ctypes.c_void_p, ctypes.c_int],
@@ -75,7 +75,7 @@ _EXTRA_ARGTYPES = {"dcx_render_workspace_size": [ctypes.c_void_p, ctypes.c_void_
                    "dcx_normal_map_workspace_size": [ctypes.c_void_p, ctypes.c_int],
                    "dcx_edt_workspace_size": [ctypes.c_int] * 3, "dcx_dtloss_workspace_size": [ctypes.c_int] * 2,
                    "dcx_render_bwd_workspace_size": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
-                   "dcx_pipeline_tile_width": [ctypes.c_void_p],
+                   "dcx_pipeline_tile_width": [ctypes.c_void_p], "dcx_pipeline_grid_cap": [ctypes.c_void_p],
                    "dcx_projected_mesh_boundary_workspace_size": [ctypes.c_int] * 5}
 
 _CODES = {

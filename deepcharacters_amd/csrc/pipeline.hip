@@ -149,10 +149,19 @@ struct Dev {
                           // has a face (the boundary test's 3x3 reads stay in a mask 1/32 of the face
                           // buffer's size)
   long long totMask;
-  unsigned int* cell_cnt; // [totCells] boundary points that fell in each cell (may exceed kCellCap)
-  f4* cell_pts;           // [totCells * kCellCap] a cell's first kCellCap points (x, y, point id bits, 0),
-                          // in arrival order
-  f4* ov_pts;             // [totVN] per view (from vn0) the points that found their cell full
+  // nearest-point grid, one of two layouts with the same search results:
+  // cell_cap == 0, counting sort: cell_pts [totVN] grouped by cell at
+  // cell_off; cell_cap > 0, fixed slots: a cell's first cell_cap points in
+  // cell_slots [totCells * cell_cap] in arrival order, later ones in its
+  // view's overflow list (ov_pts from vn0, ov_cnt).  Points are (x, y, point
+  // id bits, 0).
+  int cell_cap;
+  unsigned int* cell_cnt; // [totCells] boundary points that fell in each cell
+  unsigned int* cell_off; // [totCells+1]
+  unsigned int* cell_fill;
+  f4* cell_pts;
+  f4* cell_slots;
+  f4* ov_pts;
   unsigned int* ov_cnt;   // [V]
   unsigned int* scan_aux;
   unsigned int* heavy;     // [kHeavyCap] global indices of the slow-path tiles (any order)
@@ -1442,9 +1451,8 @@ __device__ __forceinline__ float remap_p(int v) { return v >= 127 ? (float)((v -
 // pixel lies inside the grid.
 constexpr int kRing = 8;
 constexpr int kGridMargin = kRing + 1;
-// Point slots per grid cell; a point that finds its cell full goes to its
-// view's overflow list, which every search scans in full.
-constexpr int kCellCap = 32;
+// Largest slot count of the fixed-slot grid layout.
+constexpr int kMaxCellCap = 1024;
 struct GridGeomP { float ox, oy, cell; };
 __device__ __forceinline__ GridGeomP grid_geom_p(const float* cr) {
   const float sc = cr[2];
@@ -1481,16 +1489,19 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
     const bool isb = boundary_p(d, vw, p);
     d.isb[pi] = isb ? 1 : 0;
     if (isb) {
-      // the point goes straight into a slot of its cell (or the overflow
-      // list): the nearest-point search takes the minimum of (distance,
-      // index), which does not depend on the order of a cell's points
+      // the boundary point's cell count; with fixed slots the point goes
+      // straight into a slot of its cell (or the overflow list): the
+      // nearest-point search takes the minimum of (distance, index), which
+      // does not depend on the order of a cell's points
       const int cl = cell_of_p(grid_geom_p(cr), vw.gx, vw.gy, p.x, p.y);
       if (cl >= 0) {
         const long long cell = vw.cell0 + cl;
         const unsigned int k = atomicAdd(d.cell_cnt + cell, 1u);
-        const f4 pt = mk4(p.x, p.y, __int_as_float(v), 0.f);
-        if (k < (unsigned int)kCellCap) d.cell_pts[cell * kCellCap + k] = pt;
-        else d.ov_pts[vw.vn0 + atomicAdd(d.ov_cnt + w, 1u)] = pt;  // at most one entry per vertex
+        if (d.cell_cap > 0) {
+          const f4 pt = mk4(p.x, p.y, __int_as_float(v), 0.f);
+          if (k < (unsigned int)d.cell_cap) d.cell_slots[cell * d.cell_cap + k] = pt;
+          else d.ov_pts[vw.vn0 + atomicAdd(d.ov_cnt + w, 1u)] = pt;  // at most one entry per vertex
+        }
       }
     }
     const float pu = p.x, pv = p.y;
@@ -1534,10 +1545,25 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
   if (threadIdx.x == 0) d.loss_pt[(size_t)w * gridDim.x + blockIdx.x] = t;
 }
 
+// Counting-sort layout: the boundary points into their cells' ranges.
+__global__ __launch_bounds__(kB) void sil_grid_scatter_p(Dev d, IO io) {
+  const int w = blockIdx.y;
+  const ViewD vw = d.vw[w];
+  const CharD c = d.ch[vw.ch];
+  const int v = blockIdx.x * kB + threadIdx.x;
+  if (v >= c.N || !d.isb[vw.vn0 + v]) return;
+  const f4 p = d.proj[vw.vn0 + v];
+  const int cl = cell_of_p(grid_geom_p(io.crops + (size_t)w * 7), vw.gx, vw.gy, p.x, p.y);
+  if (cl < 0) return;
+  const long long cell = vw.cell0 + cl;
+  d.cell_pts[d.cell_off[cell] + atomicAdd(d.cell_fill + cell, 1u)] = mk4(p.x, p.y, __int_as_float(v), 0.f);
+}
+
 constexpr int kDtTile = 16;      // DT pixels per side of a silhouette tile (one lane each)
 constexpr float kResidualFx = 1099511627776.f;  // 2^40: fixed-point unit of the data->model residual sums
 constexpr int kDtApron = kDtTile + 4;  // the tile plus the 2-pixel border of the 5x5 boundary test
 constexpr int kMaxRegionPts = 1280;  // regions with more points search the global grid
+constexpr int kMaxRegionRows = 64;
 constexpr int kMaxRegionCells = 1024;  // entries of the per-cell LDS offset table (rows x (cols + 1))
 
 __device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, float& bd, int& best, float& bx,
@@ -1565,7 +1591,16 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
   __shared__ f2 s_pxy[kMaxRegionPts];
   __shared__ int s_pid[kMaxRegionPts];
   __shared__ unsigned int s_coff[kMaxRegionCells];
-  __shared__ unsigned char s_dt[kDtApron * kDtApron];
+  // the DT tile with its apron (boundary test), then, for the counting-sort
+  // layout, each region row's first point (staging): one LDS block, so the
+  // kernel stays within 8 workgroups per CU
+  __shared__ union {
+    unsigned char dt[kDtApron * kDtApron];
+    unsigned int ra[kMaxRegionRows];
+  } s_u;
+  static_assert(sizeof(s_u.ra) <= sizeof(s_u.dt), "the row table must not grow the DT tile's block");
+  unsigned char* const s_dt = s_u.dt;
+  unsigned int* const s_ra = s_u.ra;
   __shared__ unsigned char s_bp[kDtTile * kDtTile];  // the tile's boundary pixels, compacted
   __shared__ unsigned int s_wave[kB / 64];
   __shared__ float s_red[kB / 64];
@@ -1619,20 +1654,29 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
     const int cy1 = min(vw.gy - 1, (int)floorf((yhi - gg.oy) / gg.cell) + 1);
     const int nrows = cy1 - cy0 + 1, ncols = cx1 - cx0 + 1;
     const int ncell = nrows * (ncols + 1);  // region cells row-major, plus an end entry per row
-    const bool shape_ok = nrows > 0 && ncols > 0 && ncell <= kMaxRegionCells;
-    // the region's cell counts (capped), scanned in row-major order: entry
-    // (r, k) of s_coff is the LDS offset of cell (cy0 + r, cx0 + k), entry
-    // (r, ncols) the end of row r; thread t scans entries 4t .. 4t + 3
+    const bool shape_ok = nrows > 0 && ncols > 0 && nrows <= kMaxRegionRows && ncell <= kMaxRegionCells;
+    const int cap = d.cell_cap;
+    // the region's cell counts (capped by the slots of the fixed-slot
+    // layout), scanned in row-major order: entry (r, k) of s_coff is the LDS
+    // offset of cell (cy0 + r, cx0 + k), entry (r, ncols) the end of row r;
+    // thread t scans entries 4t .. 4t + 3
     unsigned int cnt4[4] = {0u, 0u, 0u, 0u}, part = 0u;
-    if (shape_ok)
+    if (shape_ok) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = 4 * (int)threadIdx.x + j;
         const int r = i / (ncols + 1), k = i - r * (ncols + 1);
-        if (i < ncell && k < ncols)
-          cnt4[j] = min(d.cell_cnt[vw.cell0 + (long long)(cy0 + r) * vw.gx + cx0 + k], (unsigned int)kCellCap);
+        if (i < ncell && k < ncols) {
+          const unsigned int n = d.cell_cnt[vw.cell0 + (long long)(cy0 + r) * vw.gx + cx0 + k];
+          cnt4[j] = cap > 0 ? min(n, (unsigned int)cap) : n;
+        }
         part += cnt4[j];
       }
+      // counting-sort layout: a region row's cells are consecutive in the
+      // grid, so their points are one contiguous range
+      if (cap == 0 && (int)threadIdx.x < nrows)
+        s_ra[threadIdx.x] = d.cell_off[vw.cell0 + (long long)(cy0 + (int)threadIdx.x) * vw.gx + cx0];
+    }
     unsigned int total;
     unsigned int ex = block_scan256(part, s_wave, total);
     if (shape_ok)
@@ -1644,7 +1688,8 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       }
     __syncthreads();
     const bool in_lds = shape_ok && total <= (unsigned int)kMaxRegionPts;
-    const unsigned int n_ov = d.ov_cnt[w];  // points of this view whose cell was full (searched by every pixel)
+    // fixed-slot layout: points of this view whose cell was full (searched by every pixel)
+    const unsigned int n_ov = cap > 0 ? d.ov_cnt[w] : 0u;
     if (in_lds)
       for (unsigned int i = threadIdx.x; i < total; i += kB) {
         // the entry holding point i: the last one whose offset is <= i (an
@@ -1657,7 +1702,8 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
         }
         const int r = e / (ncols + 1), k = e - r * (ncols + 1);
         const long long cell = vw.cell0 + (long long)(cy0 + r) * vw.gx + cx0 + k;
-        const f4 it = d.cell_pts[cell * kCellCap + (i - s_coff[e])];
+        const f4 it = cap > 0 ? d.cell_slots[cell * cap + (i - s_coff[e])]
+                              : d.cell_pts[s_ra[r] + (i - s_coff[r * (ncols + 1)])];
         s_pxy[i] = mk2(it.x, it.y);
         s_pid[i] = __float_as_int(it.z);
       }
@@ -1733,9 +1779,10 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
               const int xx = cx + ox;
               if (xx < 0 || xx >= vw.gx) continue;
               const long long cell = vw.cell0 + yy * vw.gx + xx;
-              const unsigned int q1 = min(d.cell_cnt[cell], (unsigned int)kCellCap);
+              const f4* pts = cap > 0 ? d.cell_slots + cell * cap : d.cell_pts + d.cell_off[cell];
+              const unsigned int q1 = cap > 0 ? min(d.cell_cnt[cell], (unsigned int)cap) : d.cell_cnt[cell];
               for (unsigned int q = 0; q < q1; ++q) {
-                const f4 it = d.cell_pts[cell * kCellCap + q];
+                const f4 it = pts[q];
                 const float dx = it.x - ix, dyy = it.y - iy;
                 nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
               }
@@ -2310,9 +2357,13 @@ extern "C" dcx_status dcx_pipeline_create(int n_chars, const dcx_char_desc* char
   alloc(&d.fgmask, 4 * (size_t)mask_words);
   d.totMask = mask_words;
   alloc(&d.cell_cnt, 4 * (size_t)cells);
-  alloc(&d.cell_pts, sizeof(f4) * kCellCap * (size_t)cells);
+  alloc(&d.cell_off, 4 * (size_t)(cells + 1));
+  alloc(&d.cell_fill, 4 * (size_t)cells);
+  alloc(&d.cell_pts, sizeof(f4) * (size_t)vn);
   alloc(&d.ov_pts, sizeof(f4) * (size_t)vn);
   alloc(&d.ov_cnt, 4 * (size_t)p->V);
+  d.cell_cap = 0;  // counting sort until the autotuner or dcx_pipeline_set_grid picks fixed slots
+  d.cell_slots = nullptr;
   alloc(&d.corner_sum, sizeof(float) * kGrad * (size_t)p->sumF);
   alloc(&d.g_v, sizeof(float) * 3 * (size_t)p->sumN);
   alloc(&d.g_pts, sizeof(float) * 2 * (size_t)vn);
@@ -2356,7 +2407,7 @@ extern "C" dcx_status dcx_pipeline_sizes(const dcx_pipeline* p, long long* sizes
 static const char* const kStageNames[] = {
     "clear",          "node_fwd",      "vertex_fwd",       "normals",   "view_normals", "project",
     "tri_count",      "tile_scan",     "tri_scan",         "bin_scatter", "fine",       "sil_vertex",
-    "sil_pixel",      "face_sum",      "vertex_bwd_raster",
+    "sil_grid_scan",  "sil_grid_scatter", "sil_pixel",     "face_sum",  "vertex_bwd_raster",
     "vertex_bwd_sil", "joint_partial", "joint_reduce",     "node_bwd",  "finalize"};
 constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
@@ -2382,7 +2433,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   add(d.tile_cnt, d.totTiles);
   add(d.tile_fill, d.totTiles);
   add(d.cell_cnt, d.totCells);
-  add(d.ov_cnt, p->V);
+  if (d.cell_cap > 0) add(d.ov_cnt, p->V);
+  else add(d.cell_fill, d.totCells);
   add(d.heavy_n, 1);
   add(d.fgmask, d.totMask);
   int stage = 0;
@@ -2444,6 +2496,14 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   auto k_sil_vertex = [&](hipStream_t q) {
     hipLaunchKernelGGL(sil_vertex_kernel_p, dim3(d.gxN, V), dim3(kB), 0, q, d, io);
   };
+  // the counting-sort layout's cell offsets and scatter (nothing to do with fixed slots)
+  auto k_sil_grid_scan = [&](hipStream_t q) -> dcx_status {
+    return d.cell_cap > 0 ? DCX_OK : scan_big(d.cell_cnt, d.cell_off, d.scan_aux, d.totCells, q);
+  };
+  auto k_sil_grid_scatter = [&](hipStream_t q) {
+    if (d.cell_cap == 0)
+      hipLaunchKernelGGL(sil_grid_scatter_p, dim3(ceil_div(d.maxN, kB), V), dim3(kB), 0, q, d, io);
+  };
   auto k_sil_pixel = [&](hipStream_t q) {
     hipLaunchKernelGGL(sil_pixel_kernel_p, dim3(d.gxS, V), dim3(kB), 0, q, d, io);
   };
@@ -2482,6 +2542,9 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
     k_bin_scatter(s); mark();
     k_fine(s); mark();
     k_sil_vertex(s); mark();
+    if (dcx_status e = k_sil_grid_scan(s)) return e;
+    mark();
+    k_sil_grid_scatter(s); mark();
     k_sil_pixel(s); mark();
     k_face_sum(s); mark();
     k_vertex_bwd_raster(s); mark();
@@ -2536,6 +2599,8 @@ static dcx_status pipe_enqueue(dcx_pipeline* p, const dcx_pipeline_io* in, hipSt
   k_face_sum(sd);
   k_vertex_bwd_raster(sd);
   DCX_HIP(hipEventRecord(p->ev_join, sd));
+  if (dcx_status e = k_sil_grid_scan(s)) return e;
+  k_sil_grid_scatter(s);
   k_sil_pixel(s);
   // every loss partial is written: the totals go to the side stream
   if (dcx_status e = fork_rec()) return e;
@@ -2637,6 +2702,31 @@ extern "C" dcx_status dcx_pipeline_set_tile(dcx_pipeline* p, int tile_w) {
 
 extern "C" int dcx_pipeline_tile_width(const dcx_pipeline* p) { return p ? 1 << p->d.tws : 0; }
 
+extern "C" dcx_status dcx_pipeline_set_grid(dcx_pipeline* p, int cap) {
+  DCX_REQUIRE(p, "dcx_pipeline_set_grid: null pipeline");
+  DCX_REQUIRE(cap >= 0 && cap <= kMaxCellCap, "dcx_pipeline_set_grid: %d slots per cell is outside [0, %d]", cap,
+              kMaxCellCap);
+  Dev& d = p->d;
+  if (cap == d.cell_cap) return DCX_OK;
+  f4* slots = nullptr;
+  if (cap > 0) {
+    const hipError_t e = hipMalloc(&slots, sizeof(f4) * (size_t)cap * (size_t)d.totCells);
+    if (e != hipSuccess) return set_error(DCX_ERR_OOM, "dcx_pipeline_set_grid: %s", hipGetErrorString(e));
+  }
+  // a queued step (any stream, replays included) may still use the old layout
+  DCX_HIP(hipDeviceSynchronize());
+  bool placed = false;
+  for (void*& a : p->allocs)
+    if (d.cell_slots && a == d.cell_slots) { (void)hipFree(a); a = slots; placed = true; }
+  if (!placed && slots) p->allocs.push_back(slots);
+  d.cell_slots = slots;
+  d.cell_cap = cap;
+  drop_graph(p);  // kernel arguments changed: a captured step is stale
+  return DCX_OK;
+}
+
+extern "C" int dcx_pipeline_grid_cap(const dcx_pipeline* p) { return p ? p->d.cell_cap : -1; }
+
 // Number of (triangle, tile) pairs of the last step (the bins' fill).
 static dcx_status pairs_of_last_step(dcx_pipeline* p, hipStream_t s, unsigned long long* pairs) {
   unsigned int v = 0;
@@ -2705,10 +2795,50 @@ extern "C" dcx_status dcx_pipeline_autotune(dcx_pipeline* p, const dcx_pipeline_
     if (ms_out) ms_out[tws - kMinTileShift] = ms;
     if (st == DCX_OK && (best < 0 || ms < best_ms)) { best = tws; best_ms = ms; }
   }
+  if (st == DCX_OK) st = dcx_pipeline_set_tile(p, 1 << best);
+  // grid layout at that shape: the counting sort against fixed slots (32 or
+  // 64 per cell, when the step's cells would send at most kGridOverflowMax
+  // points of any view to the overflow list and the slots fit in
+  // kGridSlotBytesMax); the fastest is kept
+  constexpr long long kGridOverflowMax = 64;
+  constexpr size_t kGridSlotBytesMax = (size_t)1 << 30;
+  auto timed = [&](float* ms) -> dcx_status {
+    dcx_status t = pipe_enqueue(p, io, s);
+    if (t == DCX_OK && hipEventRecord(e0, s) != hipSuccess) t = set_error(DCX_ERR_HIP, "hipEventRecord");
+    for (int r = 0; r < reps && t == DCX_OK; ++r) t = pipe_enqueue(p, io, s);
+    if (t == DCX_OK && hipEventRecord(e1, s) != hipSuccess) t = set_error(DCX_ERR_HIP, "hipEventRecord");
+    if (t == DCX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+      t = set_error(DCX_ERR_HIP, "dcx_pipeline_autotune: event timing failed");
+    *ms /= (float)reps;
+    return t;
+  };
+  int best_cap = 0;
+  float best_grid_ms = 0.f;
+  if (st == DCX_OK) st = dcx_pipeline_set_grid(p, 0);
+  if (st == DCX_OK) st = timed(&best_grid_ms);
+  std::vector<unsigned int> cnt;
+  if (st == DCX_OK) {
+    cnt.resize((size_t)p->d.totCells);
+    if (hipMemcpy(cnt.data(), p->d.cell_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      st = set_error(DCX_ERR_HIP, "dcx_pipeline_autotune: cell counts");
+  }
+  for (int cap = 32; cap <= 64 && st == DCX_OK; cap *= 2) {
+    long long worst = 0;
+    for (const ViewD& v : p->vw_host) {
+      long long over = 0;
+      for (long long i = v.cell0; i < v.cell0 + (long long)v.gx * v.gy; ++i)
+        over += std::max(0ll, (long long)cnt[(size_t)i] - cap);
+      worst = std::max(worst, over);
+    }
+    if (worst > kGridOverflowMax || sizeof(f4) * (size_t)cap * (size_t)p->d.totCells > kGridSlotBytesMax) continue;
+    float ms = 0.f;
+    st = dcx_pipeline_set_grid(p, cap);
+    if (st == DCX_OK) st = timed(&ms);
+    if (st == DCX_OK && ms < best_grid_ms) { best_grid_ms = ms; best_cap = cap; }
+  }
+  if (st == DCX_OK) st = dcx_pipeline_set_grid(p, best_cap);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (st) return st;
-  st = dcx_pipeline_set_tile(p, 1 << best);
   if (st) return st;
   if (tile_w_out) *tile_w_out = 1 << best;
   if (pairs_out) *pairs_out = (long long)most_pairs;

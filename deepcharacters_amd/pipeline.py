@@ -304,10 +304,23 @@ class BatchedPipeline:
         _lib.call("dcx_pipeline_set_tile", self.handle, int(tile_w))
         self.captured_stream = None
 
+    @property
+    def grid_cap(self) -> int:
+        """Slots per cell of the silhouette grid (0: counting-sort layout)."""
+        return _lib.load().dcx_pipeline_grid_cap(self.handle)
+
+    def set_grid(self, cap: int):
+        """Silhouette grid layout: 0 for the counting sort, cap > 0 for fixed
+        slots per cell with a per-view overflow list (same results bit for
+        bit).  Drops a captured graph."""
+        _lib.call("dcx_pipeline_set_grid", self.handle, int(cap))
+        self.captured_stream = None
+
     def autotune(self, reps: int = 3) -> dict:
         """Times `reps` steps per tile shape on the current inputs, keeps the
         fastest, and grows the record storage of the bins if a shape needs more
-        (triangle, tile) pairs than it holds.  Drops a captured graph."""
+        (triangle, tile) pairs than it holds; then keeps the faster silhouette
+        grid layout at that shape.  Drops a captured graph."""
         tw = ctypes.c_int()
         ms = (ctypes.c_float * 3)()
         pairs = ctypes.c_longlong()
@@ -315,7 +328,7 @@ class BatchedPipeline:
                   ctypes.byref(tw), ms, ctypes.byref(pairs))
         self.captured_stream = None
         return {"tile_w": tw.value, "ms": dict(zip(self.TILE_WIDTHS, [float(x) for x in ms])),
-                "pairs": pairs.value}
+                "pairs": pairs.value, "grid_cap": self.grid_cap}
 
     def stage_times(self, repeats: int = 1) -> dict[str, float]:
         """Mean per-stage time (ms) over `repeats` eager steps, from HIP events

@@ -625,12 +625,29 @@ dcx_status dcx_pipeline_profile(dcx_pipeline* pipe, const dcx_pipeline_io* io, f
  * graph. */
 dcx_status dcx_pipeline_set_tile(dcx_pipeline* pipe, int tile_w);
 int dcx_pipeline_tile_width(const dcx_pipeline* pipe);
+/* Layout of the silhouette's nearest-boundary-point grid (the reference
+ * searches every boundary point per DT pixel,
+ * MultiViewSilhouetteLossGPUOp.cu:142-240; the grid and both layouts are
+ * this library's).  cap == 0 (the default): counting sort, points grouped by
+ * cell through a scan of the cell counts (two extra launches, memory for one
+ * entry per (view, vertex)).  cap in [1, 1024]: every cell has cap slots that
+ * the boundary points take in arrival order, with a per-view overflow list
+ * for the rest that every boundary pixel scans in full (no extra launch;
+ * cap x 16 bytes per grid cell).  Both layouts give the same results bit for
+ * bit: the search keeps the minimum (distance, point index).  Waits for all
+ * queued device work, drops a captured graph. */
+dcx_status dcx_pipeline_set_grid(dcx_pipeline* pipe, int cap);
+int dcx_pipeline_grid_cap(const dcx_pipeline* pipe);
 /* Autotuner: runs one warm-up step and `reps` timed steps per tile shape on
  * the given inputs, grows the bins' record storage to 1.25x the largest
  * (triangle, tile) pair count seen if it is short, and keeps the fastest
- * shape.  Outputs (each may be NULL): the chosen tile width, the mean step
- * time per shape ms[3] (widths 16, 32, 64), the largest pair count.
- * Synchronises; drops a captured graph.  No counterpart in the reference. */
+ * shape; then, at that shape, times the counting-sort grid against fixed
+ * slots (32 and 64 per cell, each only if the step's cells would send at most
+ * 64 points of any view to the overflow list and the slots take at most
+ * 1 GiB) and keeps the fastest layout (dcx_pipeline_grid_cap reads it).
+ * Outputs (each may be NULL): the chosen tile width, the mean step time per
+ * shape ms[3] (widths 16, 32, 64), the largest pair count.  Synchronises;
+ * drops a captured graph.  No counterpart in the reference. */
 dcx_status dcx_pipeline_autotune(dcx_pipeline* pipe, const dcx_pipeline_io* io, int reps, void* stream,
                                  int* tile_w, float* ms, long long* pairs);
 

@@ -55,6 +55,9 @@ def test_invalid_arguments_return_error_status_without_gpu():
     with pytest.raises(_lib.DcxError):
         _lib.call("dcx_lbs_fwd", None, None, None, None, None, None, 1, 1, 9, 1, None, None, None,
                   None, None)
+    with pytest.raises(_lib.DcxError, match="null pipeline"):
+        _lib.call("dcx_pipeline_set_grid", None, 32)
+    assert lib.dcx_pipeline_grid_cap(None) == -1
 
 
 def declared_params():

@@ -255,6 +255,7 @@ def test_autotune_picks_a_shape_and_grows_the_bins(works):
     assert res["pairs"] > 1024
     assert all(v > 0 for v in res["ms"].values())
     assert min(res["ms"], key=res["ms"].get) == res["tile_w"]
+    assert res["grid_cap"] in (0, 32, 64) and bp.grid_cap == res["grid_cap"]
     bp.step()
     torch.cuda.synchronize()
     got = _outputs(bp)

@@ -1,12 +1,13 @@
-"""The silhouette's nearest-point grid when its cells overflow.
+"""The silhouette's nearest-point grid in its two layouts.
 
-Each grid cell keeps its first 32 boundary points; later ones go to the
-view's overflow list, which every DT boundary pixel scans in full.  A DT
+Counting sort (cap 0): points grouped by cell through a scan.  Fixed slots
+(cap > 0): each cell keeps its first `cap` boundary points; later ones go to
+the view's overflow list, which every DT boundary pixel scans in full.  A DT
 grid of 6x6 over 256x192 views makes cells of ~160 px, so a few cells hold
 all of a view's boundary points: the batched step must still give the
-oracle's brute-force nearest points (losses and gradients), and the same
-bits on every run (the order in which points reach a cell must not
-matter)."""
+oracle's brute-force nearest points (losses and gradients) in both layouts,
+the same bits in both, and the same bits on every run (the order in which
+points reach a cell must not matter)."""
 import numpy as np
 import pytest
 import torch
@@ -15,7 +16,7 @@ from deepcharacters_amd import pipeline
 
 pytestmark = pytest.mark.gpu
 
-CELL_CAP = 32   # point slots per grid cell (the kernel's kCellCap)
+CELL_CAP = 32   # point slots per grid cell of the fixed-slot layout tested here
 RING = 8        # grid cells per search radius (kRing)
 MARGIN = RING + 1
 
@@ -60,9 +61,15 @@ def _max_cell_count(wk, pos):
     return best
 
 
-def test_overflowing_cells_give_the_oracle_nearest_points(coarse):
+NAMES = ("loss", "face", "bary", "render", "g_colors", "g_dq", "g_dT", "g_dA")
+
+
+@pytest.mark.parametrize("cap", [0, CELL_CAP])
+def test_overflowing_cells_give_the_oracle_nearest_points(coarse, cap):
     from oracle import pipeline_ref
     bp = pipeline.BatchedPipeline(coarse)
+    bp.set_grid(cap)
+    assert bp.grid_cap == cap
     bp.step()
     torch.cuda.synchronize()
     j0 = m0 = v0 = 0
@@ -83,14 +90,35 @@ def test_overflowing_cells_give_the_oracle_nearest_points(coarse):
     assert fullest > CELL_CAP, f"no cell overflows (fullest holds {fullest} points)"
 
 
-def test_overflowing_cells_repeat_bit_for_bit(coarse):
+def test_layouts_agree_bit_for_bit_and_repeat(coarse):
+    """Counting sort, 8 slots (most points overflow) and 32 slots: every
+    output identical, on repeated eager steps and on graph replays."""
     bp = pipeline.BatchedPipeline(coarse)
     bp.step()
     torch.cuda.synchronize()
-    names = ("loss", "g_dq", "g_dT", "g_dA", "g_colors")
-    first = [getattr(bp, n).clone() for n in names]
-    for _ in range(3):
-        bp.step()
+    first = [getattr(bp, n).clone() for n in NAMES]
+    for cap in (8, CELL_CAP, 0, CELL_CAP):
+        bp.set_grid(cap)
+        for _ in range(2):
+            bp.step()
+            torch.cuda.synchronize()
+            for n, a in zip(NAMES, first):
+                assert torch.equal(a, getattr(bp, n)), (cap, n)
+    s = torch.cuda.Stream()
+    bp.capture(s)
+    for _ in range(2):
+        bp.g_dq.zero_()
         torch.cuda.synchronize()
-        for n, a in zip(names, first):
-            assert torch.equal(a, getattr(bp, n)), n
+        bp.replay()
+        s.synchronize()
+        for n, a in zip(NAMES, first):
+            assert torch.equal(a, getattr(bp, n)), ("replay", n)
+
+
+def test_set_grid_checks_its_argument(coarse):
+    from deepcharacters_amd import _lib
+    bp = pipeline.BatchedPipeline(coarse[:1])
+    for bad in (-1, 1025):
+        with pytest.raises(_lib.DcxError, match="slots per cell"):
+            bp.set_grid(bad)
+    assert bp.grid_cap == 0
