@@ -845,13 +845,45 @@ __device__ __forceinline__ void bary_vjp_fast(const f4 P[3], const float* M, flo
   }
 }
 
+// Sum over a wavefront whose 64 lanes are all active, the same in every
+// lane: DPP adds within each row of 16 lanes (swap neighbours, swap pairs,
+// half-row mirror, row mirror: after each step a lane and its partner hold
+// the same sum), then the four rows' sums in order.  Fixed order, so the
+// result repeats bit for bit; ~11 instructions against ~36 for a
+// ds_bpermute butterfly.
+__device__ __forceinline__ float wave_sum_full(float v) {
+  auto dpp_add = [](float x, int ctrl) {
+    int y = 0;
+    switch (ctrl) {  // the control word must be a constant
+      case 0: y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xf, 0xf, false); break;   // [1,0,3,2]
+      case 1: y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xf, 0xf, false); break;   // [2,3,0,1]
+      case 2: y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xf, 0xf, false); break;  // half mirror
+      default: y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xf, 0xf, false); break; // mirror
+    }
+    return x + __int_as_float(y);
+  };
+  v = dpp_add(v, 0);
+  v = dpp_add(v, 1);
+  v = dpp_add(v, 2);
+  v = dpp_add(v, 3);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return ((r0 + r1) + r2) + r3;
+}
+
+// Sum over a 256-lane workgroup (every kernel of the step that calls it runs
+// 256 lanes and calls it with all of them).
+static_assert(kTilePix == kB, "the fine kernel's workgroups are kB lanes too");
 __device__ __forceinline__ float block_sum(float v, float* s_red) {
-  v = wave_sum(v);
+  v = wave_sum_full(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
   __syncthreads();
   float t = 0.f;
-  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += s_red[i];
+#pragma unroll
+  for (int i = 0; i < kB / 64; ++i) t += s_red[i];
   return t;
 }
 
