@@ -218,14 +218,26 @@ struct ClearList {
 // the block totals, add-back; out[n] receives the total.
 // ---------------------------------------------------------------------------
 
+// Inclusive scan over a wavefront whose 64 lanes are all active: DPP row
+// shifts by 1, 2, 4 and 8 scan each row of 16 lanes (a lane with no source
+// in its row adds 0), then the end of row 0 is added into row 1 and the end
+// of row 2 into row 3 (row_bcast:15), and the end of row 1 into rows 2 and 3
+// (row_bcast:31).  Integer sums, so the result is exact.
+__device__ __forceinline__ unsigned int wave_incl_scan(unsigned int v) {
+  v += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+  v += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+  return v;
+}
+
+// Exclusive scan over a 256-lane workgroup, all lanes active (every caller
+// runs kB lanes and calls it from all of them); total receives the sum.
 __device__ __forceinline__ unsigned int block_scan256(unsigned int v, unsigned int* s_wave, unsigned int& total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  unsigned int inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const unsigned int o = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += o;
-  }
+  const unsigned int inc = wave_incl_scan(v);
   if (lane == 63) s_wave[wid] = inc;
   __syncthreads();
   unsigned int base = 0;
