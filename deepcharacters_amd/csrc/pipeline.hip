@@ -731,6 +731,9 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
   const int spy = max(t.Y[0], max(t.Y[1], t.Y[2])) - min(t.Y[0], min(t.Y[1], t.Y[2]));
   EdgeWalk w;
   w.init(t, x0, y0);
+  // the record's 1/z in registers: the loops below would otherwise load it
+  // again after every key update
+  const float izv[3] = {t.iz[0], t.iz[1], t.iz[2]};
   if (spx <= kSmallSpan && spy <= kSmallSpan) {
     int row[3], dx[3], dy[3];
 #pragma unroll
@@ -744,7 +747,7 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
       for (int x = x0; x <= x1; ++x) {
         if (e[0] >= 0 && e[1] >= 0 && e[2] >= 0)
           atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)),
-                    make_key(interp_iz32(e[0] - w.bias[0], e[1] - w.bias[1], e[2] - w.bias[2], t.iz), (int)tag));
+                    make_key(interp_iz32(e[0] - w.bias[0], e[1] - w.bias[1], e[2] - w.bias[2], izv), (int)tag));
         e[0] += dx[0]; e[1] += dx[1]; e[2] += dx[2];
       }
       row[0] += dy[0]; row[1] += dy[1]; row[2] += dy[2];
@@ -756,7 +759,7 @@ __device__ __forceinline__ void raster_in_tile(const TriRec& t, int tx0, int ty0
     long long e[3] = {row[0], row[1], row[2]};
     for (int x = x0; x <= x1; ++x) {
       if (w.inside(e))
-        atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)), make_key(interp_iz(e[0], e[1], e[2], t.iz), (int)tag));
+        atomicMin(s_key + (((y - ty0) << tws) | (x - tx0)), make_key(interp_iz(e[0], e[1], e[2], izv), (int)tag));
       e[0] += w.dx[0]; e[1] += w.dx[1]; e[2] += w.dx[2];
     }
     row[0] += w.dy[0]; row[1] += w.dy[1]; row[2] += w.dy[2];
@@ -952,10 +955,14 @@ __device__ __forceinline__ void raster_row(const RowRec& q, int r, unsigned long
   const int b0 = -(q.bias & 1), b1 = -((q.bias >> 1) & 1), b2 = -((q.bias >> 2) & 1);
   int e0 = q.e[0] + step_y(q.dxy[0]) * r, e1 = q.e[1] + step_y(q.dxy[1]) * r, e2 = q.e[2] + step_y(q.dxy[2]) * r;
   const int d0 = step_x(q.dxy[0]), d1 = step_x(q.dxy[1]), d2 = step_x(q.dxy[2]);
+  // the record lives in LDS beside the keys: its 1/z and tag are read once,
+  // not again after every key update
+  const float izv[3] = {q.iz[0], q.iz[1], q.iz[2]};
+  const int tg = (int)q.tag;
   unsigned long long* row = s_key + (y << TWS);
   for (int x = x0; x <= x1; ++x) {
     if ((e0 | e1 | e2) >= 0)
-      atomicMin(row + x, make_key(interp_iz32(e0 - b0, e1 - b1, e2 - b2, q.iz), (int)q.tag));
+      atomicMin(row + x, make_key(interp_iz32(e0 - b0, e1 - b1, e2 - b2, izv), tg));
     e0 += d0; e1 += d1; e2 += d2;
   }
 }
