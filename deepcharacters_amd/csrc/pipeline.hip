@@ -959,6 +959,10 @@ __device__ __forceinline__ void raster_row(const RowRec& q, int r, unsigned long
   // not again after every key update
   const float izv[3] = {q.iz[0], q.iz[1], q.iz[2]};
   const int tg = (int)q.tag;
+  // wait for those LDS reads here (lgkmcnt 0, the other counters left
+  // alone): waited for inside the loop, the wait would also hold every
+  // iteration until the previous key update is done
+  __builtin_amdgcn_s_waitcnt(0xC07F);
   unsigned long long* row = s_key + (y << TWS);
   for (int x = x0; x <= x1; ++x) {
     if ((e0 | e1 | e2) >= 0)
