@@ -1621,9 +1621,28 @@ constexpr int kMaxRegionPts = 1280;  // regions with more points search the glob
 constexpr int kMaxRegionRows = 64;
 constexpr int kMaxRegionCells = 1024;  // entries of the per-cell LDS offset table (rows x (cols + 1))
 
-__device__ __forceinline__ void nearest_take(float dd, int v, float x, float y, float& bd, int& best, float& bx,
-                                             float& by) {
-  if (dd < bd || (dd == bd && best >= 0 && v < best)) { bd = dd; best = v; bx = x; by = y; }
+// Squared-distance bound of a best distance bd >= 0: a point whose squared
+// distance d2 is at least this has a correctly rounded distance sqrtf(d2) of
+// at least next_up(bd) (the bound is above next_up(bd)^2 exactly: next_up of
+// a rounded value lies above the exact one), so it can neither beat nor tie
+// bd.  Points below it are compared exactly, so the search skips the square
+// root for most points and still returns the same point.
+__device__ __forceinline__ float sq_bound(float bd) {
+  const float nb = __int_as_float(__float_as_int(bd) + 1);
+  const float t = nb * nb;
+  if (!(t < 3.0e38f)) return INFINITY;
+  return fmaxf(__int_as_float(__float_as_int(t) + 1), 1.17549435e-38f);  // at least the smallest normal
+}
+
+// Keep the point at squared distance d2 if it is nearer than (bd, best), or
+// as near with a lower index (d2 below the bound tq of bd; tq follows bd).
+__device__ __forceinline__ void nearest_take(float d2, int v, float x, float y, float& bd, int& best, float& bx,
+                                             float& by, float& tq) {
+  const float dd = sqrtf(d2);
+  if (dd < bd || (dd == bd && best >= 0 && v < best)) {
+    bd = dd; best = v; bx = x; by = y;
+    tq = sq_bound(bd);
+  }
 }
 
 // Data->model residual (MultiViewSilhouetteLossGPUOp.cu:142-240) + loss +
@@ -1790,6 +1809,7 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
       const int pu = tu0 + pp % kDtTile, pv = tv0 + pp / kDtTile;
       const float ix = ((float)pu - cr[0]) * sc + cr[3], iy = ((float)pv - cr[1]) * sc + cr[4];
       float bd = R, bx = 0.f, by = 0.f;
+      float tq = sq_bound(R);
       int best = -1;
       const int cx = (int)floorf((ix - gg.ox) / gg.cell), cy = (int)floorf((iy - gg.oy) / gg.cell);
       if (in_lds) {
@@ -1809,7 +1829,8 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
           for (unsigned int q = s_coff[rb + xa - cx0] + (unsigned int)sub; q < q1; q += (unsigned int)L) {
             const f2 it = s_pxy[q];
             const float dx = it.x - ix, dyy = it.y - iy;
-            nearest_take(sqrtf(dx * dx + dyy * dyy), s_pid[q], it.x, it.y, bd, best, bx, by);
+            const float d2 = dx * dx + dyy * dyy;
+            if (d2 < tq) nearest_take(d2, s_pid[q], it.x, it.y, bd, best, bx, by, tq);
           }
           gbd = fminf(gbd, bd);
           for (int o = L >> 1; o > 0; o >>= 1) gbd = fminf(gbd, __shfl_xor(gbd, o, 64));
@@ -1817,7 +1838,8 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
         for (unsigned int q = (unsigned int)sub; q < n_ov; q += (unsigned int)L) {
           const f4 it = d.ov_pts[vw.vn0 + q];
           const float dx = it.x - ix, dyy = it.y - iy;
-          nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+          const float d2 = dx * dx + dyy * dyy;
+          if (d2 < tq) nearest_take(d2, __float_as_int(it.z), it.x, it.y, bd, best, bx, by, tq);
         }
         for (int o = L >> 1; o > 0; o >>= 1) {
           const float obd = __shfl_xor(bd, o, 64), obx = __shfl_xor(bx, o, 64), oby = __shfl_xor(by, o, 64);
@@ -1839,7 +1861,8 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
               for (unsigned int q = 0; q < q1; ++q) {
                 const f4 it = pts[q];
                 const float dx = it.x - ix, dyy = it.y - iy;
-                nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+                const float d2 = dx * dx + dyy * dyy;
+                if (d2 < tq) nearest_take(d2, __float_as_int(it.z), it.x, it.y, bd, best, bx, by, tq);
               }
             }
           }
@@ -1849,7 +1872,8 @@ __global__ __launch_bounds__(kB) void sil_pixel_kernel_p(Dev d, IO io) {
         for (unsigned int q = 0; q < n_ov; ++q) {
           const f4 it = d.ov_pts[vw.vn0 + q];
           const float dx = it.x - ix, dyy = it.y - iy;
-          nearest_take(sqrtf(dx * dx + dyy * dyy), __float_as_int(it.z), it.x, it.y, bd, best, bx, by);
+          const float d2 = dx * dx + dyy * dyy;
+          if (d2 < tq) nearest_take(d2, __float_as_int(it.z), it.x, it.y, bd, best, bx, by, tq);
         }
       }
       if (sub == 0 && best >= 0) {
