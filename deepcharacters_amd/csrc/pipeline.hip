@@ -686,10 +686,22 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
         if (h >= 0) atomicAdd(s_cnt + h, 1u);
       }
   __syncthreads();
-  // one reservation per distinct tile
-  for (int i = threadIdx.x; i < kHash; i += kB) {
-    if (s_keys[i] != kHashEmpty) s_base[i] = atomicAdd(d.tile_fill + vw.tile0 + s_keys[i], s_cnt[i]);
-    s_cnt[i] = 0u;
+  // one reservation per distinct tile; a lane's (at most kHash / kB) memory
+  // atomics are all issued before it waits for any of them
+  {
+    static_assert(kHash % kB == 0, "each lane owns kHash / kB slots");
+    constexpr int kPer = kHash / kB;
+    unsigned int key[kPer], base[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) key[j] = s_keys[threadIdx.x + j * kB];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+      if (key[j] != kHashEmpty) base[j] = atomicAdd(d.tile_fill + vw.tile0 + key[j], s_cnt[threadIdx.x + j * kB]);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (key[j] != kHashEmpty) s_base[threadIdx.x + j * kB] = base[j];
+      s_cnt[threadIdx.x + j * kB] = 0u;
+    }
   }
   __syncthreads();
   // pass 2: slots
