@@ -709,9 +709,10 @@ __global__ __launch_bounds__(kB) void bin_scatter_kernel(Dev d) {
     for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) {
       const unsigned int lt = (unsigned int)(ty * vw.tiles_x + tx);
       const long long tile = vw.tile0 + lt;
+      const unsigned int toff = d.tile_off[tile];  // in flight during the hash probe and the rank atomic
       const int h = big ? -1 : hash_find(s_keys, lt, false);
       const unsigned int rank = h >= 0 ? s_base[h] + atomicAdd(s_cnt + h, 1u) : atomicAdd(d.tile_fill + tile, 1u);
-      const unsigned int slot = d.tile_off[tile] + rank;
+      const unsigned int slot = toff + rank;
       if (slot < d.capacity) d.bins[slot] = (unsigned int)ri;
     }
 }
