@@ -1550,9 +1550,14 @@ __global__ __launch_bounds__(kB) void sil_vertex_kernel_p(Dev d, IO io) {
   }
   const int v = blockIdx.x * kB + threadIdx.x;
   float lsum = 0.f;
+  // the view's crop record in registers, read before the atomics and stores
+  // below (read after them, each value would wait for them)
+  float crv[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) crv[k] = io.crops[(size_t)w * 7 + k];
+  const float* cr = crv;
   if (v < c.N) {
     const long long pi = vw.vn0 + v;
-    const float* cr = io.crops + (size_t)w * 7;
     const f4 p = d.proj[pi];
     const bool isb = boundary_p(d, vw, p);
     d.isb[pi] = isb ? 1 : 0;
