@@ -1273,8 +1273,11 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       const int* fv = d.faces + 3 * ((size_t)c.f0 + f);
       const int i0 = fv[0], i1 = fv[1], i2 = fv[2];
       const f4* pv = d.proj + vw.vn0;
+      // the three projections, read once before the output stores (read after
+      // them they would be loaded again)
+      const f4 P[3] = {pv[i0], pv[i1], pv[i2]};
       const float px = (float)x + 0.5f, py = (float)y + 0.5f;
-      const f2 ab = native_bary(pv[i0], pv[i1], pv[i2], px, py);
+      const f2 ab = native_bary(P[0], P[1], P[2], px, py);
       const float wgt[3] = {ab.x, ab.y, 1.f - ab.x - ab.y};
       const float* cb = io.col + 3 * (size_t)c.v0;
       const f3 c0 = ld3(cb + 3 * i0), c1 = ld3(cb + 3 * i1), c2 = ld3(cb + 3 * i2);
@@ -1298,7 +1301,6 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
 #pragma clang fp contract(fast)
       const float gw[3] = {dot(g, c0), dot(g, c1), dot(g, c2)};
       const float* M = d.ke + (size_t)w * 12;
-      const f4 P[3] = {pv[i0], pv[i1], pv[i2]};
       f3 gx[3];
       bary_vjp_fast(P, M, px, py, gw, gx);
       // model-to-data term (CUDABasedRasterizationGrad.cu:413-437): the
