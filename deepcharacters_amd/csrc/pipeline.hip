@@ -1923,25 +1923,43 @@ __global__ __launch_bounds__(kB) void face_sum_kernel(Dev d) {
   const CharD c = d.ch[blockIdx.y];
   const int f = blockIdx.x * kB + threadIdx.x;
   if (f >= c.F) return;
+  // the corners' slots are read first, so that their loads do not wait behind
+  // the stores at the end
+  const int* cs = d.corner_slot + 3 * ((size_t)c.f0 + f);
+  const int slot[3] = {cs[0], cs[1], cs[2]};
   float a[kGrad];
 #pragma unroll
   for (int q = 0; q < kGrad; ++q) a[q] = 0.f;
-  for (int k = 0; k < c.C; ++k) {
-    const long long ri = d.vw[c.view0 + k].vf0 + f;
-    const unsigned int e0 = d.tri_base[ri];
-    const unsigned int e1 = min(d.tri_base[ri + 1], d.capacity);
-    for (unsigned int e = e0; e < e1; ++e) {
-      if (!d.rec_vis[e]) continue;
-      const float* r = d.rec + (size_t)e * kGrad;
+  // a character's views are consecutive, F (view, face) entries apart: four
+  // views' record ranges are loaded before any is summed
+  const long long ri0 = d.vw[c.view0].vf0 + f;
+  for (int k0 = 0; k0 < c.C; k0 += 4) {
+    unsigned int lo[4], hi[4];
 #pragma unroll
-      for (int q = 0; q < kGrad; ++q) a[q] += r[q];
+    for (int j = 0; j < 4; ++j) {
+      lo[j] = hi[j] = 0u;
+      if (k0 + j < c.C) {
+        const long long ri = ri0 + (long long)(k0 + j) * c.F;
+        lo[j] = d.tri_base[ri];
+        hi[j] = d.tri_base[ri + 1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const unsigned int e1 = min(hi[j], d.capacity);
+      for (unsigned int e = lo[j]; e < e1; ++e) {
+        if (!d.rec_vis[e]) continue;
+        const float* r = d.rec + (size_t)e * kGrad;
+#pragma unroll
+        for (int q = 0; q < kGrad; ++q) a[q] += r[q];
+      }
     }
   }
   // each corner's six sums go to its slot of the vertex's incidence list, so
   // the vertex pass reads one contiguous range per vertex
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float2* o = reinterpret_cast<float2*>(d.corner_sum + 6 * (size_t)d.corner_slot[3 * ((size_t)c.f0 + f) + k]);
+    float2* o = reinterpret_cast<float2*>(d.corner_sum + 6 * (size_t)slot[k]);
     o[0] = make_float2(a[6 * k], a[6 * k + 1]);
     o[1] = make_float2(a[6 * k + 2], a[6 * k + 3]);
     o[2] = make_float2(a[6 * k + 4], a[6 * k + 5]);
