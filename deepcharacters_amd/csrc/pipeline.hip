@@ -1157,14 +1157,28 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   }
 
   // target colours of the tile plus a kImgFs apron (clamped reads; the apron is
-  // only used by pixels whose whole filter window lies inside the image).
-  // The fast path stages them while its triangle records are in flight.
-  auto stage_target = [&]() {
-    for (int i = threadIdx.x; i < an; i += kTilePix) {
-      const int ay = i / aw, ax = i - ay * aw;
-      const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
-      const f3 t = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
-      u2.tgt[i] = t.x; u2.tgt[an + i] = t.y; u2.tgt[2 * an + i] = t.z;
+  // only used by pixels whose whole filter window lies inside the image): a
+  // thread's kStage pixels are all loaded before any is written to LDS, and
+  // the fast path issues those loads beside its tile-list read, so that
+  // neither costs a round trip of its own.
+  constexpr int kStage = (an + kTilePix - 1) / kTilePix;
+  f3 stg[kStage];
+  auto load_target = [&]() {
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+      const int i = threadIdx.x + k * kTilePix;
+      if (i < an) {
+        const int ay = i / aw, ax = i - ay * aw;
+        const int gx = min(max(tx0 + ax - kImgFs, 0), vw.W - 1), gy = min(max(ty0 + ay - kImgFs, 0), vw.H - 1);
+        stg[k] = ld3(io.target + 3 * (vw.pix0 + (long long)gy * vw.W + gx));
+      }
+    }
+  };
+  auto store_target = [&]() {
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+      const int i = threadIdx.x + k * kTilePix;
+      if (i < an) { u2.tgt[i] = stg[k].x; u2.tgt[an + i] = stg[k].y; u2.tgt[2 * an + i] = stg[k].z; }
     }
   };
 
@@ -1183,15 +1197,18 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     unsigned int nrows = 0, ri = 0;
     bool wide = false;
     TriRec t;
+    unsigned int tb = 0;  // the triangle's first record slot, read beside the triangle
+    if (threadIdx.x < n_items) ri = d.bins[off + threadIdx.x];
+    load_target();
     if (threadIdx.x < n_items) {
-      ri = d.bins[off + threadIdx.x];
       t = d.tri[ri];
+      tb = d.tri_base[ri];
     }
-    stage_target();
+    store_target();
     if (threadIdx.x < n_items) {
       my_f = t.face_flags & kFaceMask;
       const TileRange r = tile_range(t, tws, ths);
-      my_e = d.tri_base[ri] + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
+      my_e = tb + (unsigned int)((ty - r.ty0) * r.ntx + (tx - r.tx0));
       const int x0 = max(t.bx0_by0 & 0xffff, tx0), y0 = max(t.bx0_by0 >> 16, ty0);
       const int x1 = min(t.bx1_by1 & 0xffff, tx0 + (1 << tws) - 1), y1 = min(t.bx1_by1 >> 16, ty0 + (1 << ths) - 1);
       const int spx = max(t.X[0], max(t.X[1], t.X[2])) - min(t.X[0], min(t.X[1], t.X[2]));
@@ -1241,7 +1258,8 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       bj = (int)(tag & ((1u << kTagBits) - 1u));
     }
   } else {
-    stage_target();
+    load_target();
+    store_target();
     s_key[threadIdx.x] = kEmptyKey;
     __syncthreads();
     for (unsigned int k = threadIdx.x; k < n_items; k += kTilePix) {
