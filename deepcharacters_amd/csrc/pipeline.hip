@@ -1056,6 +1056,19 @@ __device__ __forceinline__ void write_extras(const Dev& d, const IO& io, const C
   }
 }
 
+// The pixel's face, barycentrics and colour, stored with the non-temporal
+// hint: the step writes them once and reads them back never, so they stream
+// to memory rather than taking L2 lines from the triangle records and target
+// pixels that neighbouring tiles share.
+__device__ __forceinline__ void store_pixel(const IO& io, long long o, int f, float a, float b, f3 col) {
+  __builtin_nontemporal_store(f, io.face + o);
+  __builtin_nontemporal_store(a, io.bary + 2 * o);
+  __builtin_nontemporal_store(b, io.bary + 2 * o + 1);
+  __builtin_nontemporal_store(col.x, io.render + 3 * o);
+  __builtin_nontemporal_store(col.y, io.render + 3 * o + 1);
+  __builtin_nontemporal_store(col.z, io.render + 3 * o + 2);
+}
+
 // Face bits of the tile's rows (the mask starts the step cleared, so tiles
 // that no triangle touches store nothing): the first lane of each tile row in
 // the wavefront stores that row's bits from the wavefront's ballot (lanes are
@@ -1145,9 +1158,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     if (inimg) {
       const long long o = vw.pix0 + (long long)y * vw.W + x;
       const f3 tgt = ld3(io.target + 3 * o);
-      io.face[o] = -1;
-      io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
-      io.render[3 * o] = 0.f; io.render[3 * o + 1] = 0.f; io.render[3 * o + 2] = 0.f;
+      store_pixel(io, o, -1, 0.f, 0.f, mk3(0.f, 0.f, 0.f));
       lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
       if (EXTRA) write_extras(d, io, c, w, o, -1, 0, 0, 0, 0.f, 0.f, 0.f, 0.f);
     }
@@ -1282,9 +1293,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     const int ta = (py_i + kImgFs) * aw + (px_i + kImgFs);
     const f3 tgt = mk3(u2.tgt[ta], u2.tgt[an + ta], u2.tgt[2 * an + ta]);
     if (f < 0) {
-      io.face[o] = -1;
-      io.bary[2 * o] = 0.f; io.bary[2 * o + 1] = 0.f;
-      io.render[3 * o] = 0.f; io.render[3 * o + 1] = 0.f; io.render[3 * o + 2] = 0.f;
+      store_pixel(io, o, -1, 0.f, 0.f, mk3(0.f, 0.f, 0.f));
       lsum = io.w_col * (fabsf(tgt.x) + fabsf(tgt.y) + fabsf(tgt.z));
       if (EXTRA) write_extras(d, io, c, w, o, -1, 0, 0, 0, 0.f, 0.f, 0.f, 0.f);
     } else {
@@ -1301,9 +1310,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       const f3 c0 = ld3(cb + 3 * i0), c1 = ld3(cb + 3 * i1), c2 = ld3(cb + 3 * i2);
       const f3 col = mk3(c0.x * wgt[0] + c1.x * wgt[1] + c2.x * wgt[2], c0.y * wgt[0] + c1.y * wgt[1] + c2.y * wgt[2],
                          c0.z * wgt[0] + c1.z * wgt[1] + c2.z * wgt[2]);
-      io.face[o] = f;
-      io.bary[2 * o] = ab.x; io.bary[2 * o + 1] = ab.y;
-      io.render[3 * o] = col.x; io.render[3 * o + 1] = col.y; io.render[3 * o + 2] = col.z;
+      store_pixel(io, o, f, ab.x, ab.y, col);
       if (EXTRA) write_extras(d, io, c, w, o, f, i0, i1, i2, ab.x, ab.y, px, py);
       const f3 df = col - tgt;
       lsum = io.w_col * (fabsf(df.x) + fabsf(df.y) + fabsf(df.z));
