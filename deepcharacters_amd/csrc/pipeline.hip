@@ -1402,8 +1402,16 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       act &= ~m;
     }
     __syncthreads();
-    bool first = bj >= 0 && rank == 0;
-    for (int q = 0; q < wv; ++q) first = first && u2.srt.wcnt[q][max(bj, 0)] == 0;
+    // counts of this pixel's triangle in the earlier wavefronts: every
+    // wavefront's count is read, all together, and the later ones masked off
+    // (a short-circuit loop waited for each read in turn)
+    unsigned int before = 0;
+#pragma unroll
+    for (int q = 0; q < kTilePix / 64; ++q) {
+      const unsigned int n = u2.srt.wcnt[q][max(bj, 0)];
+      before |= q < wv ? n : 0u;
+    }
+    const bool first = bj >= 0 && rank == 0 && before == 0;
     unsigned int cnt = 0, pre[kTilePix / 64];
 #pragma unroll
     for (int q = 0; q < kTilePix / 64; ++q) {
@@ -1441,15 +1449,24 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
       const float* gk = s_g + (3 * k) * kGPad;
       float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       // kSumUnroll pixels' table reads are issued together (the LDS latency
-      // of one pixel's reads is paid once per group); the additions still run
-      // in pixel order
+      // of one pixel's reads is paid once per group), and the next group's
+      // pixel indices beside them (one LDS round trip per group, not two);
+      // the additions still run in pixel order
       constexpr int kSumUnroll = 4;
+      int pn[kSumUnroll];
+#pragma unroll
+      for (int u = 0; u < kSumUnroll; ++u) pn[u] = u2.srt.order[min(st + (unsigned int)u, en - 1u)];
       for (unsigned int r0 = st; r0 < en; r0 += kSumUnroll) {
         float v[kSumUnroll][6];
+        int pc[kSumUnroll];
 #pragma unroll
         for (int u = 0; u < kSumUnroll; ++u) {
-          const unsigned int r = min(r0 + (unsigned int)u, en - 1u);
-          const int p = u2.srt.order[r];
+          pc[u] = pn[u];
+          pn[u] = u2.srt.order[min(r0 + (unsigned int)(kSumUnroll + u), en - 1u)];
+        }
+#pragma unroll
+        for (int u = 0; u < kSumUnroll; ++u) {
+          const int p = pc[u];
           const float w0 = s_g[12 * kGPad + p], w1 = s_g[13 * kGPad + p];
           const float wk = (w0 * s0 + w1 * s1) + (1.f - w0 - w1) * s2;
           v[u][0] = gk[p];
