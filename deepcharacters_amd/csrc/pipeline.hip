@@ -1200,7 +1200,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
   unsigned long long* s_key = u1.r.key;
   if (fast) {
     // lane k < n_items builds the walk record of list entry k; the rows of
-    // all records are then dealt out to the lanes, 256 rows per round, so a
+    // all records are then dealt out to the lanes, 256 rows per pass, so a
     // wavefront's time follows its widest row rather than its largest
     // triangle.  Triangles wider or taller than kSmallSpan sub-pixels are
     // walked whole by their own lane afterwards (64-bit edge values).
@@ -1249,7 +1249,7 @@ __global__ __launch_bounds__(kTilePix) __attribute__((amdgpu_waves_per_eu(7))) v
     const unsigned int row0 = block_scan256(nrows, s_wave, n_rows);
     if (nrows) u1.r.rec[threadIdx.x].bias |= (int)(row0 << 8);
     for (unsigned int base = 0; base < n_rows; base += kTilePix) {
-      // entry k's rows [row0, row0 + nrows) that fall in this round
+      // entry k's rows [row0, row0 + nrows) that fall in this pass of 256
       const unsigned int lo = max(row0, base), hi = min(row0 + nrows, base + (unsigned int)kTilePix);
       for (unsigned int j = lo; j < hi; ++j) u1.r.item[j - base] = (unsigned char)threadIdx.x;
       __syncthreads();

@@ -617,8 +617,8 @@ const char* dcx_pipeline_stage_name(int stage);
 dcx_status dcx_pipeline_profile(dcx_pipeline* pipe, const dcx_pipeline_io* io, float* stage_ms, int n_stages,
                                 void* stream);
 /* Raster tile shape: tile_w x (256 / tile_w) pixels, tile_w in {16, 32, 64}
- * (default 32).  The tile-binned rasterizer the north-star design asks for
- * replaces the reference's per-triangle bounding-box loop
+ * (default 32).  This library's tile-binned rasterizer replaces the
+ * reference's per-triangle bounding-box loop
  * (CUDABasedRasterization.cu:215-451), which has no tiles.  Changing the
  * shape waits for all queued device work (hipDeviceSynchronize: any stream,
  * replays included) before it rewrites the view table, and drops a captured
